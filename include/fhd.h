@@ -1,0 +1,129 @@
+/*
+ * fhd.h -- C ABI of the MI355X (gfx950) fluctuating-hydrodynamics step library
+ * (libfhd.so).  Plain C types only: pointers, sizes, doubles.  Every function that
+ * returns int returns 1 on success and 0 on failure, the convention of the fluam
+ * scheme entry points; the reason for a failure is in fhd_last_error() (global API)
+ * or fhd_ctx_error() (context API).
+ *
+ * Reference interfaces replaced (fbusabiaga/fluam):
+ *   fhd_load_datamain                 <- bool loadDataMain(int, char*[])      src/loadDataMain.cu:173
+ *   fhd_scheme_incompressible         <- bool schemeIncompressible()           src/schemeIncompressible.cu:25, src/header.h:200
+ *   fhd_scheme_quasi_neutrally_buoyant<- bool schemeQuasiNeutrallyBuoyant()    src/schemeQuasiNeutrallyBuoyant.cu:27, src/header.h:222
+ *   fhd_scheme_stokes_limit           <- bool schemeStokesLimit()              src/schemeStokesLimit.cu:27, src/header.h:310
+ *   fhd_main                          <- int main(int, char*[]) scheme dispatch src/main.cpp:28-112
+ *   fhd_create/fhd_step/...           <- the runScheme*() time loops with their device
+ *                                        state held in globals (src/runSchemeIncompressible.cu:21,
+ *                                        src/runSchemeQuasiNeutrallyBuoyant.cu:21,
+ *                                        src/runSchemeStokesLimit.cu:21); here the state is a
+ *                                        re-entrant context instead of process globals.
+ *
+ * Host array layouts (double precision on the host side of the boundary):
+ *   velocity: 3*N values, component-major: vx[0..N) vy[N..2N) vz[2N..3N),
+ *             cell i = ix + mx*(iy + my*iz), v_x on the +x face of cell i
+ *             (src/createCells.cpp:68-70, src/stokesLimitFunctions.cu:767).
+ *   particles: 3*np values, component-major: rx[0..np) ry[np..2np) rz[2np..3np),
+ *             unwrapped coordinates, box centred at the origin.
+ */
+#ifndef FHD_H
+#define FHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FHD_SCHEME_INCOMPRESSIBLE 0
+#define FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT 1
+#define FHD_SCHEME_STOKES_LIMIT 2
+
+/* Random-finite-difference variant of the Stokes-limit thermal drift.
+ * FHD_RFD_DISPLACED: bin and weight the particle at q +- delta/2 W (the term
+ *   S(q + delta/2 W) W documented at src/boundaryParticlesFunctionStokesLimit.cu:152-153).
+ * FHD_RFD_REFERENCE: bin at the displaced position but evaluate the kernel weights at
+ *   the undisplaced q, as src/stokesLimitFunctions.cu:1741-1799,1872-1906 does. */
+#define FHD_RFD_DISPLACED 0
+#define FHD_RFD_REFERENCE 1
+
+/* Run parameters: the data.main keywords of the three schemes (src/loadDataMain.cu:38-170)
+ * plus the extension keywords listed next to each field. */
+typedef struct fhd_params {
+  int scheme;                 /* FHD_SCHEME_* */
+  int mx, my, mz;             /* cells */
+  double lx, ly, lz;          /* celldimension */
+  double densfluid;           /* densfluid */
+  double shearviscosity;      /* shearviscosity */
+  double temperature;         /* temperature (kT) */
+  double dt;                  /* dt */
+  unsigned long long seed;    /* seed */
+  long long numsteps;         /* numsteps */
+  long long numsteps_relaxation; /* numstepsRelaxation */
+  long long samplefreq;       /* samplefreq */
+  int initfluid;              /* initfluid (0: background velocity) */
+  double background[3];       /* backgroundvelocity */
+  int np;                     /* numberparticles */
+  int kernel_points;          /* extension keyword `kernel` 3|4 (Peskin 3- or 4-point) */
+  int nonbonded;              /* computeNonBondedForces */
+  double cutoff;              /* cutoff */
+  double wca_sigma;           /* extension keyword `sigma` (default 2*dx, src/initForcesNonBonded.cu:24) */
+  double wca_epsilon;         /* extension keyword `epsilon` (default kT, src/initForcesNonBonded.cu:25) */
+  double mass;                /* mass (excess mass; the build supports 0) */
+  double volume_particle;     /* volumeParticle */
+  double extra_mobility;      /* extraMobility (0 = off) */
+  double rfd_delta;           /* RFD displacement (0.001, src/stokesLimitFunctions.cu:1719) */
+  int rfd_mode;               /* extension keyword `rfdMode` FHD_RFD_* */
+  int max_part_in_cell;       /* maxNumberPartInCell (capacity hint, per spreading tile) */
+  int max_part_in_cell_nonbonded; /* maxNumberPartInCellNonBonded (capacity hint) */
+  int initial_positions;      /* 0: seeded uniform-random, 1: simple cubic (src/simpleCubic.cpp:25) */
+  char outputname[512];       /* outputname */
+  char coordinates[512];      /* coordinates file (loadparticles 1) */
+} fhd_params;
+
+/* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
+void fhd_default_params(fhd_params* p);
+
+/* ---------------- global (fluam-style) API ---------------- */
+int fhd_load_datamain(const char* path);          /* parse data.main into the global params */
+int fhd_get_params(fhd_params* out);
+int fhd_set_params(const fhd_params* in);
+int fhd_scheme_incompressible(void);
+int fhd_scheme_quasi_neutrally_buoyant(void);
+int fhd_scheme_stokes_limit(void);
+int fhd_main(const char* datamain_path);          /* load + dispatch in main.cpp order */
+const char* fhd_last_error(void);
+
+/* ---------------- re-entrant context API ---------------- */
+typedef struct fhd_ctx fhd_ctx;
+fhd_ctx* fhd_create(const fhd_params* p);         /* owns all device memory; NULL on error */
+void fhd_destroy(fhd_ctx* c);
+const char* fhd_ctx_error(const fhd_ctx* c);
+int fhd_set_velocity(fhd_ctx* c, const double* v);       /* 3*N, layout above */
+int fhd_get_velocity(fhd_ctx* c, double* v);
+int fhd_set_particles(fhd_ctx* c, const double* q);      /* 3*np unwrapped positions */
+int fhd_get_particles(fhd_ctx* c, double* q);
+int fhd_get_particle_velocities(fhd_ctx* c, double* u);  /* J v at the last step (QNB) */
+int fhd_get_forces(fhd_ctx* c, double* f);              /* last particle forces, 3*np */
+int fhd_step(fhd_ctx* c, int nsteps);                    /* enqueue nsteps steps (async) */
+int fhd_sync(fhd_ctx* c);                                /* wait, then check device error flags */
+long long fhd_get_step(const fhd_ctx* c);
+/* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
+ * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names
+ * into kern_names (each up to 47 chars + NUL, 48-byte stride); returns kernels reported. */
+int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms);
+int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern);
+
+/* Divergence diagnostics of the current velocity: max |D v| over cells, in units of
+ * 1/time (discrete staggered divergence, src/kernelUpdateVIncompressible.cu:21-72 invariant). */
+int fhd_max_divergence(fhd_ctx* c, double* maxdiv);
+
+/* ---------------- building blocks (exposed for parity tests) ---------------- */
+/* Batched 3D real FFT round trip on device: real field (mz,my,mx) -> half spectrum
+ * (mz,my,mx/2+1) complex (interleaved re,im) -> back; unnormalised both ways. */
+int fhd_fft_r2c(int mx, int my, int mz, const double* in, double* out_spec);
+int fhd_fft_c2r(int mx, int my, int mz, const double* in_spec, double* out);
+/* Counter-based normal generator used for every noise term (Philox4x32-10 + Box-Muller).
+ * Writes n normals of (seed, step, stream, index=0..n/4, sub) as the kernels draw them. */
+int fhd_normals(unsigned long long seed, long long step, int stream, int sub, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FHD_H */
