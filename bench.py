@@ -1,0 +1,138 @@
+"""Benchmark: the Stokes-limit (midpoint) step on one MI355X, 128^3 staggered grid, 65,536
+particles at seeded uniform-random positions, 4-point Peskin kernel, WCA (sigma=1,
+rc=2^(1/6), epsilon=kT), kT=1e-3, eta=1, dt=0.05, FP32 fields.
+
+Prints one JSON line: whole-job throughput (cell-updates/s; particle-steps/s alongside), the
+roofline of the dominant kernel measured with HIP events over the timed region, the whole-step
+algorithmic-byte roofline, and the FP64 oracle timed on the host as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def algorithmic_bytes(N, npart):
+    # per Stokes midpoint step: 192 B/cell (two solves, each: spread write, W + xy-fwd, z pass,
+    # xy-inverse, interpolation read, FP32, 3 components) + 200 B/particle
+    return 192.0 * N + 200.0 * npart
+
+
+def kernel_bytes(name, mx, my, mz, npart):
+    """Algorithmic HBM bytes of one launch of a kernel (FP32, 3 components)."""
+    N = mx * my * mz
+    P = mx // 2 + 1
+    spec = 3 * mz * my * P * 8
+    if name == "fft_z_solve":
+        return 2 * spec                      # read + write the half spectrum
+    if name == "fft_xy_fwd":
+        return 3 * N * 4 + spec              # read real W, write half spectrum
+    if name == "fft_xy_inv":
+        return spec + 3 * N * 4              # read half spectrum, write real v
+    if name == "fluid_w":
+        return 3 * N * 4 + 3 * N * 4         # read S F, write W (noise in registers)
+    if name == "spread":
+        return 3 * N * 4 + 32 * npart        # write S F, read sorted particles
+    return None
+
+
+def cpu_baseline(args):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
+
+    threads = O.lib().orc_set_threads(0)
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=3, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+    o = O.Oracle(p)
+    o.step(1)  # warm-up (page-in, plans)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        o.step(1)
+        n += 1
+        if time.perf_counter() - t0 > args.cpu_seconds or n >= 4:
+            break
+    dt = (time.perf_counter() - t0) / n
+    N = 128 ** 3
+    return {"value": N / dt, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
+            "particle_steps_per_s": 65536 / dt,
+            "sample": f"FP64 oracle, Stokes 128^3 + 65536 particles, {n} timed steps after 1 warm-up, "
+                      f"OpenMP {threads} threads, {dt:.2f} s/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the step is a single-GPU kernel pipeline "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import fluam_amd as F
+    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
+
+    mx = my = mz = 128
+    npart = 65536
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(mx, my, mz), np=npart, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=3, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+    sim = F.Simulation(p)
+    sim.step(args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    ev_ms = sim.time_steps(args.steps)  # HIP events on the context's stream
+    sim.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = ev_ms / args.steps
+    N = mx * my * mz
+    cups = N * args.steps / (ev_ms * 1e-3)
+    psps = npart * args.steps / (ev_ms * 1e-3)
+    # per-kernel times (HIP events around each launch group, same stream) over a short window
+    prof = sim.profile(min(args.steps, 10))
+    dom = max((k for k in prof if kernel_bytes(k, mx, my, mz, npart) is not None), key=lambda k: prof[k][0] * prof[k][1])
+    kb = kernel_bytes(dom, mx, my, mz, npart)
+    kms = prof[dom][0]
+    achieved = kb / (kms * 1e-3) / 1e9
+    step_gbs = algorithmic_bytes(N, npart) / (ms_per_step * 1e-3) / 1e9
+    line = {
+        "metric": "fluid cell-updates/s + IB particle-steps/s; batched-ensemble p50/p99 latency",
+        "value": cups,
+        "unit": "cell-updates/s",
+        "particle_steps_per_s": psps,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "wall_s": wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded uniform-random particle positions, zero initial velocity, Philox noise",
+        "config": {"workload": "stokesLimit midpoint step, 128^3 periodic staggered grid, 65536 particles, "
+                               "4-point Peskin kernel, WCA sigma=1 rc=2^(1/6) eps=kT, kT=1e-3, eta=1, dt=0.05",
+                   "cells": [mx, my, mz], "particles": npart},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},
+        "step_roofline": {"algorithmic_bytes": algorithmic_bytes(N, npart), "achieved_GBs": step_gbs,
+                          "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_ms": {k: round(v[0], 5) for k, v in prof.items()},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(args),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

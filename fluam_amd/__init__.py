@@ -1,0 +1,234 @@
+"""fluam_amd -- MI355X (gfx950) implementation of fluam's incompressible fluctuating-
+hydrodynamics time step (schemeIncompressible, schemeQuasiNeutrallyBuoyant,
+schemeStokesLimit) behind a C ABI (include/fhd.h, fluam_amd/lib/libfhd.so).
+
+This module is a thin ctypes layer over that ABI.  It never computes anything itself: if the
+HIP library is missing it raises, it does not fall back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (FhdParams, make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_NEUTRALLY_BUOYANT,
+                   SCHEME_STOKES_LIMIT, RFD_DISPLACED, RFD_REFERENCE)
+
+__all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "get_params", "main",
+           "scheme_incompressible", "scheme_quasi_neutrally_buoyant", "scheme_stokes_limit",
+           "SCHEME_INCOMPRESSIBLE", "SCHEME_QUASI_NEUTRALLY_BUOYANT", "SCHEME_STOKES_LIMIT",
+           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH"]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libfhd.so")
+_lib = None
+
+
+class FhdError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfhd.so from the package tree (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FhdError(f"{LIB_PATH} is missing: build it with `make -C fluam_amd/csrc` "
+                       "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    L.fhd_default_params.argtypes = [C.POINTER(FhdParams)]
+    L.fhd_default_params.restype = None
+    L.fhd_load_datamain.argtypes = [C.c_char_p]
+    L.fhd_get_params.argtypes = [C.POINTER(FhdParams)]
+    L.fhd_set_params.argtypes = [C.POINTER(FhdParams)]
+    for n in ("fhd_scheme_incompressible", "fhd_scheme_quasi_neutrally_buoyant", "fhd_scheme_stokes_limit"):
+        getattr(L, n).argtypes = []
+    L.fhd_main.argtypes = [C.c_char_p]
+    L.fhd_last_error.restype = C.c_char_p
+    L.fhd_create.argtypes = [C.POINTER(FhdParams)]
+    L.fhd_create.restype = vp
+    L.fhd_destroy.argtypes = [vp]
+    L.fhd_destroy.restype = None
+    L.fhd_ctx_error.argtypes = [vp]
+    L.fhd_ctx_error.restype = C.c_char_p
+    for n in ("fhd_set_velocity", "fhd_get_velocity", "fhd_set_particles", "fhd_get_particles",
+              "fhd_get_particle_velocities", "fhd_get_forces", "fhd_max_divergence"):
+        getattr(L, n).argtypes = [vp, dp]
+    L.fhd_step.argtypes = [vp, C.c_int]
+    L.fhd_sync.argtypes = [vp]
+    L.fhd_get_step.argtypes = [vp]
+    L.fhd_get_step.restype = C.c_longlong
+    L.fhd_time_steps.argtypes = [vp, C.c_int, dp]
+    L.fhd_profile_kernels.argtypes = [vp, C.c_int, dp, C.c_char_p, C.c_int]
+    L.fhd_fft_r2c.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    L.fhd_fft_c2r.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+    L.fhd_normals.argtypes = [C.c_ulonglong, C.c_longlong, C.c_int, C.c_int, C.c_int, dp]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _err(ctx=None):
+    L = lib()
+    msg = L.fhd_ctx_error(ctx) if ctx else L.fhd_last_error()
+    return (msg or b"").decode(errors="replace")
+
+
+def default_params():
+    p = FhdParams()
+    lib().fhd_default_params(C.byref(p))
+    return p
+
+
+def load_datamain(path):
+    """Parse a fluam data.main file into the library's global parameters (loadDataMain)."""
+    if not lib().fhd_load_datamain(os.fsencode(path)):
+        raise FhdError(_err())
+    return get_params()
+
+
+def get_params():
+    p = FhdParams()
+    if not lib().fhd_get_params(C.byref(p)):
+        raise FhdError(_err())
+    return p
+
+
+def set_params(p):
+    lib().fhd_set_params(C.byref(p))
+
+
+def scheme_incompressible():
+    return bool(lib().fhd_scheme_incompressible())
+
+
+def scheme_quasi_neutrally_buoyant():
+    return bool(lib().fhd_scheme_quasi_neutrally_buoyant())
+
+
+def scheme_stokes_limit():
+    return bool(lib().fhd_scheme_stokes_limit())
+
+
+def main(path="data.main"):
+    """`fluam data.main`: load the file and run the scheme it selects (src/main.cpp order)."""
+    return bool(lib().fhd_main(os.fsencode(path)))
+
+
+class Simulation:
+    """One re-entrant step context (all fields and particles resident on the GPU)."""
+
+    def __init__(self, params: FhdParams):
+        self.p = params.copy()
+        self.h = lib().fhd_create(C.byref(self.p))
+        if not self.h:
+            raise FhdError(_err())
+        self.N = self.p.mx * self.p.my * self.p.mz
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, ok):
+        if not ok:
+            raise FhdError(_err(self.h))
+
+    def step(self, n=1):
+        self._check(lib().fhd_step(self.h, int(n)))
+
+    def sync(self):
+        self._check(lib().fhd_sync(self.h))
+
+    @property
+    def nstep(self):
+        return lib().fhd_get_step(self.h)
+
+    def velocity(self):
+        v = np.zeros(3 * self.N)
+        self._check(lib().fhd_get_velocity(self.h, _p(v)))
+        return v
+
+    def set_velocity(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(lib().fhd_set_velocity(self.h, _p(v)))
+
+    def particles(self):
+        q = np.zeros(3 * self.p.np)
+        self._check(lib().fhd_get_particles(self.h, _p(q)))
+        return q
+
+    def set_particles(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        self._check(lib().fhd_set_particles(self.h, _p(q)))
+
+    def particle_velocities(self):
+        u = np.zeros(3 * self.p.np)
+        self._check(lib().fhd_get_particle_velocities(self.h, _p(u)))
+        return u
+
+    def forces(self):
+        f = np.zeros(3 * self.p.np)
+        self._check(lib().fhd_get_forces(self.h, _p(f)))
+        return f
+
+    def max_divergence(self):
+        d = C.c_double(0)
+        self._check(lib().fhd_max_divergence(self.h, C.byref(d)))
+        return d.value
+
+    def time_steps(self, n):
+        ms = C.c_double(0)
+        self._check(lib().fhd_time_steps(self.h, int(n), C.byref(ms)))
+        return ms.value
+
+    def profile(self, n, nkern=32):
+        kms = np.zeros(2 * nkern)
+        names = C.create_string_buffer(48 * nkern)
+        k = lib().fhd_profile_kernels(self.h, int(n), _p(kms), names, nkern)
+        if k <= 0:
+            raise FhdError(_err(self.h))
+        out = {}
+        raw = names.raw
+        for j in range(k):
+            nm = raw[48 * j:48 * (j + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (kms[2 * j], int(kms[2 * j + 1]))
+        return out
+
+
+def fft_r2c(field):
+    """GPU forward real FFT of a (mz, my, mx) field -> (mz, my, mx//2+1) complex (unnormalised)."""
+    mz, my, mx = field.shape
+    out = np.zeros(mz * my * (mx // 2 + 1) * 2)
+    a = np.ascontiguousarray(field, dtype=np.float64)
+    if not lib().fhd_fft_r2c(mx, my, mz, _p(a), _p(out)):
+        raise FhdError(_err())
+    return out.view(np.complex128).reshape(mz, my, mx // 2 + 1)
+
+
+def fft_c2r(spec, mx):
+    """GPU inverse of fft_r2c (unnormalised): half spectrum -> real (mz, my, mx)."""
+    mz, my, P = spec.shape
+    s = np.ascontiguousarray(spec, dtype=np.complex128).view(np.float64)
+    out = np.zeros(mz * my * mx)
+    if not lib().fhd_fft_c2r(mx, my, mz, _p(s), _p(out)):
+        raise FhdError(_err())
+    return out.reshape(mz, my, mx)
+
+
+def normals(seed, step, stream, sub, n):
+    out = np.zeros(4 * n)
+    if not lib().fhd_normals(seed, step, stream, sub, n, _p(out)):
+        raise FhdError(_err())
+    return out.reshape(n, 4)

@@ -1,0 +1,198 @@
+// fhd_device.h -- device helpers shared by the gfx950 kernels: the counter-based noise
+// generator, the Peskin kernels, staggered-grid geometry and the launch-parameter structs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FHD_DEV __device__ __forceinline__
+
+// Noise streams (counter word 1 = stream << 4 | sub).
+enum { FHD_STREAM_FLUID = 1, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4, FHD_STREAM_PART2 = 5,
+       FHD_STREAM_INIT_POS = 8 };
+
+// ---------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products map to
+// v_mad_u64_u32, so a round costs two wide multiplies and two 3-input xors.
+// ---------------------------------------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+FHD_DEV u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    u32x4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Box-Muller (1958) on 24-bit uniforms u1 in (0,1], u2 in [0,1): z0 = r cos(2 pi u2),
+// z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).  sincospi keeps the angle exact in turns.
+FHD_DEV float2 box_muller(uint32_t a, uint32_t b) {
+  const float inv24 = 1.0f / 16777216.0f;
+  float u1 = (float)((a >> 8) + 1u) * inv24;
+  float u2 = (float)(b >> 8) * inv24;
+  float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincospif(2.0f * u2, &s, &c);
+  return make_float2(r * c, r * s);
+}
+
+struct Key { uint32_t k0, k1; uint32_t step_lo, step_hi; };
+
+FHD_DEV u32x4 draw_words(const Key& key, uint32_t index, uint32_t stream, uint32_t sub) {
+  u32x4 c = {index, (stream << 4) | sub, key.step_lo, key.step_hi};
+  return philox4x32_10(c, key.k0, key.k1);
+}
+
+FHD_DEV void draw_normals4(const Key& key, uint32_t index, uint32_t stream, uint32_t sub, float n[4]) {
+  u32x4 w = draw_words(key, index, stream, sub);
+  float2 a = box_muller(w.x, w.y);
+  float2 b = box_muller(w.z, w.w);
+  n[0] = a.x; n[1] = a.y; n[2] = b.x; n[3] = b.y;
+}
+
+// ---------------------------------------------------------------------------------------
+// Peskin kernels (r in grid units).  3-point: src/deltaGPU.cu:57-68; 4-point: Peskin (2002),
+// the 3D tensor form of src/deltaGPU.cu:44-53.
+// ---------------------------------------------------------------------------------------
+FHD_DEV float phi3(float r) {
+  float x = fabsf(r);
+  if (x >= 1.5f) return 0.0f;
+  if (x < 0.5f) return (1.0f + sqrtf(fmaxf(1.0f - 3.0f * x * x, 0.0f))) * (1.0f / 3.0f);
+  float y = 1.0f - x;
+  return (5.0f - 3.0f * x - sqrtf(fmaxf(1.0f - 3.0f * y * y, 0.0f))) * (1.0f / 6.0f);
+}
+FHD_DEV float dphi3(float r) {  // d phi3 / dr
+  float x = fabsf(r);
+  if (x >= 1.5f) return 0.0f;
+  if (x <= 0.5f) return -r * rsqrtf(fmaxf(1.0f - 3.0f * r * r, 1e-30f));
+  float s = r > 0.0f ? 1.0f : -1.0f;
+  return -s * 0.5f * (1.0f + (1.0f - x) * rsqrtf(fmaxf(-2.0f + 6.0f * x - 3.0f * x * x, 1e-30f)));
+}
+FHD_DEV float phi4(float r) {
+  float x = fabsf(r);
+  if (x >= 2.0f) return 0.0f;
+  if (x <= 1.0f) return (3.0f - 2.0f * x + sqrtf(fmaxf(1.0f + 4.0f * x - 4.0f * x * x, 0.0f))) * 0.125f;
+  return (5.0f - 2.0f * x - sqrtf(fmaxf(-7.0f + 12.0f * x - 4.0f * x * x, 0.0f))) * 0.125f;
+}
+FHD_DEV float dphi4(float r) {
+  float x = fabsf(r);
+  if (x >= 2.0f) return 0.0f;
+  float s = r > 0.0f ? 1.0f : -1.0f;
+  if (x <= 1.0f)
+    return s * (-2.0f + (2.0f - 4.0f * x) * rsqrtf(fmaxf(1.0f + 4.0f * x - 4.0f * x * x, 1e-30f))) * 0.125f;
+  return s * (-2.0f - (6.0f - 4.0f * x) * rsqrtf(fmaxf(-7.0f + 12.0f * x - 4.0f * x * x, 1e-30f))) * 0.125f;
+}
+
+FHD_DEV int wrapi(int i, int m) {
+  int r = i % m;
+  return r < 0 ? r + m : r;
+}
+
+// ---------------------------------------------------------------------------------------
+// Grid description passed by value to kernels.
+// ---------------------------------------------------------------------------------------
+struct GridDev {
+  int mx, my, mz, N, P;        // P = mx/2 + 1 (half-spectrum row length)
+  float dx, dy, dz;            // cell sizes
+  float idx, idy, idz;         // inverses
+  float lx, ly, lz;
+};
+
+// Particle position: integer cell (unwrapped) + offset from the cell centre in
+// [-h/2, h/2) per axis.  The unwrapped coordinate is (c + 1/2) h - L/2 + d.
+struct PartPos {
+  int4* cell;    // x, y, z, w unused
+  float4* off;   // x, y, z, w unused
+};
+
+// 1D weights of one axis for one velocity component: nodes base .. base+np-1 (unwrapped).
+struct AxisWeights {
+  int base;
+  float w[4];
+};
+
+// Weights along an axis with node offset `stag` (0: cell-centred nodes, 1: face nodes at +h/2)
+// for a particle in cell c at offset frac = d/h in [-1/2, 1/2).  Returns nodes for the 3-point
+// (npts = 3) or 4-point (npts = 4) kernel.  Same stencil choice as the FP64 restatement: the
+// 3-point stencil is centred on the nearest node, the 4-point one on the two nearest.
+template <int NPTS>
+FHD_DEV void axis_weights(int c, float frac, int stag, AxisWeights& aw) {
+  // t = position in node units relative to node index 0 of this staggering.
+  // centred nodes: t = c + frac; face nodes: t = c - 1/2 + frac.
+  if (NPTS == 3) {
+    int j0;
+    float f;  // t - j0, in [-1/2, 1/2)
+    if (!stag) { j0 = c; f = frac; }
+    else if (frac >= 0.0f) { j0 = c; f = frac - 0.5f; }
+    else { j0 = c - 1; f = frac + 0.5f; }
+    aw.base = j0 - 1;
+    aw.w[0] = phi3(f + 1.0f);
+    aw.w[1] = phi3(f);
+    aw.w[2] = phi3(f - 1.0f);
+    aw.w[3] = 0.0f;
+  } else {
+    int j0;
+    float f;  // t - j0 in [0, 1)
+    if (!stag) {
+      if (frac >= 0.0f) { j0 = c; f = frac; } else { j0 = c - 1; f = frac + 1.0f; }
+    } else {
+      j0 = c - 1; f = frac + 0.5f;
+    }
+    aw.base = j0 - 1;
+    aw.w[0] = phi4(f + 1.0f);
+    aw.w[1] = phi4(f);
+    aw.w[2] = phi4(f - 1.0f);
+    aw.w[3] = phi4(f - 2.0f);
+  }
+}
+
+// Derivative weights dphi/dq (per unit length: divide by h at the call site).
+template <int NPTS>
+FHD_DEV void axis_dweights(int c, float frac, int stag, AxisWeights& aw) {
+  if (NPTS == 3) {
+    int j0;
+    float f;
+    if (!stag) { j0 = c; f = frac; }
+    else if (frac >= 0.0f) { j0 = c; f = frac - 0.5f; }
+    else { j0 = c - 1; f = frac + 0.5f; }
+    aw.base = j0 - 1;
+    aw.w[0] = dphi3(f + 1.0f);
+    aw.w[1] = dphi3(f);
+    aw.w[2] = dphi3(f - 1.0f);
+    aw.w[3] = 0.0f;
+  } else {
+    int j0;
+    float f;
+    if (!stag) {
+      if (frac >= 0.0f) { j0 = c; f = frac; } else { j0 = c - 1; f = frac + 1.0f; }
+    } else {
+      j0 = c - 1; f = frac + 0.5f;
+    }
+    aw.base = j0 - 1;
+    aw.w[0] = dphi4(f + 1.0f);
+    aw.w[1] = dphi4(f);
+    aw.w[2] = dphi4(f - 1.0f);
+    aw.w[3] = dphi4(f - 2.0f);
+  }
+}
+
+// Renormalise (cell, offset) after a displacement so the offset lies in [-h/2, h/2).
+FHD_DEV void renorm_axis(int& c, float& d, float h, float ih) {
+  float k = floorf(d * ih + 0.5f);
+  // Guard against non-finite displacements: keep the particle where it was.
+  if (!(fabsf(k) < 1.0e9f)) { k = 0.0f; d = 0.0f; }
+  c += (int)k;
+  d -= k * h;
+  // rounding may leave d marginally outside; fold once more
+  if (d >= 0.5f * h) { c += 1; d -= h; }
+  else if (d < -0.5f * h) { c -= 1; d += h; }
+}

@@ -1,0 +1,1185 @@
+// fhd_host.cpp -- host side of libfhd.so: data.main front end, the re-entrant step context
+// that owns all device memory, the three scheme drivers and the C ABI of include/fhd.h.
+//
+// Everything runs on one HIP stream per context; a step is a fixed sequence of kernel
+// launches with no host synchronisation, so it can be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/fhd.h"
+#include "fhd_kernels.h"
+
+using namespace fhd;
+
+namespace {
+
+thread_local std::string g_last_error;
+fhd_params g_params;
+bool g_params_loaded = false;
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess) {                                                            \
+      throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    }                                                                                  \
+  } while (0)
+
+// Radix plan for an n-point complex FFT: stages of 8, then 4, 2, 3, packed 4 bits each.
+bool make_plan(int n, uint32_t* plan, int* nst) {
+  uint32_t p = 0;
+  int k = 0;
+  int m = n;
+  const int rs[4] = {8, 4, 2, 3};
+  for (int r : rs)
+    while (m % r == 0) {
+      if (k >= 8) return false;
+      p |= (uint32_t)r << (4 * k);
+      ++k;
+      m /= r;
+    }
+  if (m != 1) return false;
+  *plan = p;
+  *nst = k;
+  return true;
+}
+
+int tile_edge(int m) {
+  if (m % 8 == 0) return 8;
+  if (m % 4 == 0) return 4;
+  if (m <= 8) return m;
+  return -1;
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// Context.
+// ------------------------------------------------------------------------------------------
+struct fhd_ctx {
+  fhd_params p;
+  std::string err;
+  hipStream_t s = nullptr;
+  GridDev g;
+  Tiles t;
+  FftArgs fa;
+  double dx, dy, dz, dV;
+  // fields (3N each)
+  float *v = nullptr, *v2 = nullptr, *W = nullptr, *SF = nullptr, *adv[2] = {nullptr, nullptr};
+  float* div = nullptr;
+  float2* spec = nullptr;
+  // Fourier tables
+  float2 *twx = nullptr, *twy = nullptr, *twz = nullptr, *ex = nullptr, *ey = nullptr, *ez = nullptr;
+  float *gx = nullptr, *gy = nullptr, *gz = nullptr;
+  // particles: position sets 0 (q^n) and 1 (q^{n+1/2})
+  int np = 0;
+  int4* cell[2] = {nullptr, nullptr};
+  float4* off[2] = {nullptr, nullptr};
+  float4* F[2] = {nullptr, nullptr};
+  float4* u = nullptr;
+  int *count = nullptr, *slot = nullptr, *tmp = nullptr;
+  int* start[2] = {nullptr, nullptr};
+  int* sid[2] = {nullptr, nullptr};
+  int4* scell[2] = {nullptr, nullptr};
+  float4* soff[2] = {nullptr, nullptr};
+  int* errd = nullptr;
+  long long step = 0;
+  bool projected = false;
+  bool qnb_started = false;
+  int adv_cur = 0;
+  int last_force_set = 0;
+  std::vector<std::pair<std::string, hipEvent_t>> marks;  // per-kernel profiling
+  bool profiling = false;
+};
+
+namespace {
+
+void mark(fhd_ctx* c, const char* name) {
+  if (!c->profiling) return;
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(e, c->s));
+  c->marks.emplace_back(name, e);
+}
+
+Key make_key(const fhd_ctx* c) {
+  Key k;
+  k.k0 = (uint32_t)c->p.seed;
+  k.k1 = (uint32_t)(c->p.seed >> 32);
+  k.step_lo = (uint32_t)(uint64_t)c->step;
+  k.step_hi = (uint32_t)((uint64_t)c->step >> 32);
+  return k;
+}
+
+void validate(const fhd_params& p) {
+  if (p.scheme < 0 || p.scheme > 2) throw std::runtime_error("unknown scheme");
+  if (p.mx < 2 || p.my < 1 || p.mz < 1) throw std::runtime_error("cells must be positive (mx >= 2)");
+  if (p.mx % 2) throw std::runtime_error("mx must be even (real-to-complex FFT along x)");
+  if (p.mx > kMaxFftLen || p.my > kMaxFftLen || p.mz > kMaxFftLen)
+    throw std::runtime_error("cells per axis must be <= 256");
+  uint32_t pl;
+  int ns;
+  if (!make_plan(p.mx / 2, &pl, &ns) || !make_plan(p.my, &pl, &ns) || !make_plan(p.mz, &pl, &ns))
+    throw std::runtime_error("cells per axis must be of the form 2^a 3^b");
+  if ((p.mx / 2 + 1) * p.my > 8448) throw std::runtime_error("xy plane too large for one workgroup");
+  if (3 * p.mz > 8448) throw std::runtime_error("mz too large");
+  if (!(p.lx > 0 && p.ly > 0 && p.lz > 0)) throw std::runtime_error("celldimension must be positive");
+  if (!(p.dt > 0)) throw std::runtime_error("dt must be positive");
+  if (p.densfluid <= 0 || p.shearviscosity <= 0) throw std::runtime_error("densfluid and shearviscosity must be positive");
+  if (p.np < 0) throw std::runtime_error("numberparticles must be >= 0");
+  if (p.np > 0) {
+    if (p.scheme == FHD_SCHEME_INCOMPRESSIBLE)
+      throw std::runtime_error("particles with incompressible: use quasiNeutrallyBuoyant or stokesLimit");
+    if (p.kernel_points != 3 && p.kernel_points != 4) throw std::runtime_error("kernel must be 3 or 4");
+    if (tile_edge(p.mx) < 0 || tile_edge(p.my) < 0 || tile_edge(p.mz) < 0)
+      throw std::runtime_error("with particles, cells per axis must be a multiple of 4 or at most 8");
+    if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT && p.mass != 0.0)
+      throw std::runtime_error("quasiNeutrallyBuoyant supports mass 0 (neutrally buoyant) only");
+    if (p.nonbonded && p.cutoff > 0) {
+      double L[3] = {p.lx, p.ly, p.lz};
+      int m[3] = {p.mx, p.my, p.mz};
+      for (int a = 0; a < 3; ++a) {
+        int te = tile_edge(m[a]);
+        int nt = m[a] / te;
+        double h = L[a] / m[a];
+        if (nt >= 3 && p.cutoff > te * h) throw std::runtime_error("cutoff larger than a binning tile");
+        if (p.cutoff >= 0.5 * L[a]) throw std::runtime_error("cutoff must be below half the box");
+      }
+    }
+  }
+}
+
+void build_tables(fhd_ctx* c) {
+  const int mx = c->p.mx, my = c->p.my, mz = c->p.mz;
+  auto tw = [](int n) {
+    std::vector<float2> t(n);
+    for (int k = 0; k < n; ++k) {
+      double a = -2.0 * M_PI * (double)k / (double)n;
+      t[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    return t;
+  };
+  auto gk = [](int n, double h) {
+    std::vector<float> t(n);
+    for (int k = 0; k < n; ++k) t[k] = (float)(2.0 / h * sin(M_PI * (double)k / (double)n));
+    return t;
+  };
+  auto ek = [](int n) {
+    std::vector<float2> t(n);
+    for (int k = 0; k < n; ++k) {
+      double a = M_PI * (double)k / (double)n;
+      t[k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    return t;
+  };
+  auto up2 = [&](const std::vector<float2>& h, float2*& d) {
+    d = dalloc<float2>(h.size());
+    HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+  };
+  auto up1 = [&](const std::vector<float>& h, float*& d) {
+    d = dalloc<float>(h.size());
+    HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  };
+  up2(tw(mx), c->twx);
+  up2(tw(my), c->twy);
+  up2(tw(mz), c->twz);
+  up2(ek(mx), c->ex);
+  up2(ek(my), c->ey);
+  up2(ek(mz), c->ez);
+  up1(gk(mx, c->dx), c->gx);
+  up1(gk(my, c->dy), c->gy);
+  up1(gk(mz, c->dz), c->gz);
+  FftArgs& a = c->fa;
+  a.g = c->g;
+  a.twx = c->twx; a.twy = c->twy; a.twz = c->twz;
+  a.gx = c->gx; a.gy = c->gy; a.gz = c->gz;
+  a.ex = c->ex; a.ey = c->ey; a.ez = c->ez;
+  make_plan(mx / 2, &a.plan_h, &a.nst_h);
+  make_plan(my, &a.plan_y, &a.nst_y);
+  make_plan(mz, &a.plan_z, &a.nst_z);
+  a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
+  a.stokes = 0;
+  a.eta = (float)c->p.shearviscosity;
+  a.cn_coef = 0.0f;
+  a.inv_n = (float)(1.0 / (double)c->g.N);
+}
+
+void free_ctx(fhd_ctx* c) {
+  void* ptrs[] = {c->v, c->v2, c->W, c->SF, c->adv[0], c->adv[1], c->div, c->spec, c->twx, c->twy, c->twz,
+                  c->ex, c->ey, c->ez, c->gx, c->gy, c->gz, c->cell[0], c->cell[1], c->off[0], c->off[1],
+                  c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
+                  c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  for (auto& m : c->marks) (void)hipEventDestroy(m.second);
+  if (c->s) (void)hipStreamDestroy(c->s);
+}
+
+// Host position <-> (cell, offset).
+void to_cell(double q, double L, double h, int& cc, float& d) {
+  double t = (q + 0.5 * L) / h;
+  double f = floor(t);
+  cc = (int)f;
+  double off = q - ((f + 0.5) * h - 0.5 * L);
+  d = (float)off;
+  if (d >= 0.5f * (float)h) { cc += 1; d -= (float)h; }
+  if (d < -0.5f * (float)h) { cc -= 1; d += (float)h; }
+}
+double from_cell(int cc, float d, double L, double h) { return ((double)cc + 0.5) * h - 0.5 * L + (double)d; }
+
+void initial_positions(const fhd_params& p, std::vector<double>& q);
+
+// ---------------- solves ----------------
+void fft_forward(fhd_ctx* c, const float* in) {
+  launch_fft_xy_fwd(c->fa, in, c->spec, c->s);
+  mark(c, "fft_xy_fwd");
+}
+void fft_solve(fhd_ctx* c, int stokes, float cn_coef) {
+  FftArgs a = c->fa;
+  a.stokes = stokes;
+  a.cn_coef = cn_coef;
+  launch_fft_z_solve(a, c->spec, c->s);
+  mark(c, "fft_z_solve");
+}
+void fft_inverse(fhd_ctx* c, float* out, const float* aux, float alpha, float beta) {
+  launch_fft_xy_inv(c->fa, c->spec, out, aux, alpha, beta, c->s);
+  mark(c, "fft_xy_inv");
+}
+
+void project_initial(fhd_ctx* c) {
+  // src/runSchemeIncompressible.cu:79-97: project the initial velocity onto div-free fields
+  fft_forward(c, c->v);
+  fft_solve(c, 0, 0.0f);
+  fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
+  c->projected = true;
+}
+
+FluidWArgs base_w(fhd_ctx* c) {
+  FluidWArgs w;
+  memset(&w, 0, sizeof(w));
+  w.g = c->g;
+  w.key = make_key(c);
+  w.W = c->W;
+  w.zchunk = 8;
+  return w;
+}
+
+// ---------------- particles ----------------
+void sort_set(fhd_ctx* c, int k) {
+  launch_bin(c->g, c->t, c->cell[k], c->count, c->slot, c->np, c->s);
+  launch_scan(c->t, c->count, c->start[k], c->s);
+  launch_scatter(c->g, c->t, c->cell[k], c->slot, c->start[k], c->tmp, c->np, c->s);
+  launch_tile_sort(c->t, c->start[k], c->tmp, c->cell[k], c->off[k], c->sid[k], c->scell[k], c->soff[k], c->s);
+  mark(c, "sort");
+}
+SortedSet sset(fhd_ctx* c, int k) {
+  SortedSet s;
+  s.cell = c->scell[k];
+  s.off = c->soff[k];
+  s.id = c->sid[k];
+  s.start = c->start[k];
+  return s;
+}
+void wca(fhd_ctx* c, int k, float4* F) {
+  if (!c->p.nonbonded || c->p.cutoff <= 0) {
+    launch_zero_f4(F, c->np, c->s);
+    return;
+  }
+  WcaArgs a;
+  a.g = c->g;
+  a.t = c->t;
+  a.set = sset(c, k);
+  a.rc2 = (float)(c->p.cutoff * c->p.cutoff);
+  a.sig2 = (float)(c->p.wca_sigma * c->p.wca_sigma);
+  a.eps48 = (float)(48.0 * c->p.wca_epsilon);
+  a.F = F;
+  a.np = c->np;
+  launch_wca(a, c->s);
+  mark(c, "wca");
+}
+SpreadArgs spread_base(fhd_ctx* c) {
+  SpreadArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.t = c->t;
+  a.kernel_points = c->p.kernel_points;
+  a.key = make_key(c);
+  a.out_coef = (float)(1.0 / c->dV);
+  a.out = c->SF;
+  return a;
+}
+InterpArgs interp_base(fhd_ctx* c, int eval_set) {
+  InterpArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.t = c->t;
+  a.kernel_points = c->p.kernel_points;
+  a.eval = sset(c, eval_set);
+  a.key = make_key(c);
+  a.err = c->errd;
+  a.np = c->np;
+  return a;
+}
+
+// ---------------- schemes ----------------
+// Incompressible: src/runSchemeIncompressible.cu:105-139.
+void step_incompressible(fhd_ctx* c) {
+  if (!c->projected) project_initial(c);
+  const fhd_params& p = c->p;
+  const double dV = c->dV, rho = p.densfluid;
+  FluidWArgs w = base_w(c);
+  w.v = c->v;
+  w.lap_coef = (float)(0.5 * p.dt * p.shearviscosity / rho);
+  w.vadv = c->v;
+  w.adv_coef = (float)(-p.dt);
+  w.noise = 1;
+  w.trace_free = 1;
+  w.f1 = (float)sqrt(4.0 * p.temperature * p.shearviscosity * p.dt / (dV * rho * rho));
+  w.f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity * p.dt / (dV * rho * rho));
+  const float cn = (float)(0.5 * p.dt * p.shearviscosity / rho);
+  launch_fluid_w(w, c->s);
+  mark(c, "fluid_w");
+  fft_forward(c, c->W);
+  fft_solve(c, 0, cn);
+  fft_inverse(c, c->v2, c->v, 0.5f, 0.5f);  // v* = (v~ + v^n) / 2
+  w.vadv = c->v2;
+  launch_fluid_w(w, c->s);
+  mark(c, "fluid_w");
+  fft_forward(c, c->W);
+  fft_solve(c, 0, cn);
+  fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
+}
+
+// Stokes limit, midpoint: src/runSchemeStokesLimit.cu:163-276,
+// src/boundaryParticlesFunctionStokesLimit.cu:21-249.
+void step_stokes(fhd_ctx* c) {
+  const fhd_params& p = c->p;
+  const double dV = c->dV, dt = p.dt;
+  const float f1 = (float)sqrt(4.0 * p.temperature * p.shearviscosity / (dV * dt));
+  const float f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity / (dV * dt));
+  const float fact2 = (float)sqrt(p.temperature * p.extra_mobility * dt);
+  const bool xm = p.extra_mobility != 0.0;
+  FluidWArgs w = base_w(c);
+  w.force = c->SF;
+  w.force_coef = 1.0f;
+  w.trace_free = 0;
+  if (c->np > 0) {
+    sort_set(c, 0);
+    wca(c, 0, c->F[0]);
+    SpreadArgs sa = spread_base(c);
+    sa.ngroups = 1;
+    sa.grp[0].set = sset(c, 0);
+    sa.grp[0].mode = SRC_VALUE;
+    sa.grp[0].val = c->F[0];
+    sa.grp[0].coef = 1.0f;
+    launch_spread(sa, c->s);
+    mark(c, "spread");
+  } else {
+    launch_zero_f(c->SF, 3 * (size_t)c->g.N, c->s);
+  }
+  w.noise = 1;
+  w.f1 = (float)(sqrt(2.0) * f1);
+  w.f4 = (float)(sqrt(2.0) * f4);
+  launch_fluid_w(w, c->s);
+  mark(c, "fluid_w");
+  fft_forward(c, c->W);
+  fft_solve(c, 1, 0.0f);
+  fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
+  if (c->np > 0) {
+    InterpArgs ia = interp_base(c, 0);
+    ia.base_cell = c->cell[0];
+    ia.base_off = c->off[0];
+    ia.out_cell = c->cell[1];
+    ia.out_off = c->off[1];
+    ia.f1 = c->v;
+    ia.a1 = (float)(0.5 * dt);
+    if (xm) {
+      ia.F = c->F[0];
+      ia.aF = (float)(0.5 * dt * p.extra_mobility);
+      ia.aN = fact2;
+      ia.nnoise = 1;
+    }
+    launch_interp(ia, c->s);
+    mark(c, "interp");
+    sort_set(c, 1);
+    wca(c, 1, c->F[1]);
+    SpreadArgs sa = spread_base(c);
+    sa.ngroups = 2;
+    sa.grp[0].set = sset(c, 1);
+    sa.grp[0].mode = SRC_VALUE;
+    sa.grp[0].val = c->F[1];
+    sa.grp[0].coef = 1.0f;
+    sa.grp[1].set = sset(c, 0);
+    sa.grp[1].mode = SRC_RFD;
+    sa.grp[1].coef = (float)(p.temperature / p.rfd_delta);
+    sa.grp[1].disp = (float)(0.5 * p.rfd_delta);
+    sa.grp[1].rfd_quirk = p.rfd_mode == FHD_RFD_REFERENCE;
+    launch_spread(sa, c->s);
+    mark(c, "spread");
+  }
+  w.noise = 2;
+  w.f1 = (float)(sqrt(0.5) * f1);
+  w.f4 = (float)(sqrt(0.5) * f4);
+  launch_fluid_w(w, c->s);
+  mark(c, "fluid_w");
+  fft_forward(c, c->W);
+  fft_solve(c, 1, 0.0f);
+  fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
+  if (c->np > 0) {
+    InterpArgs ia = interp_base(c, 1);
+    ia.base_cell = c->cell[0];
+    ia.base_off = c->off[0];
+    ia.out_cell = c->cell[0];
+    ia.out_off = c->off[0];
+    ia.f1 = c->v;
+    ia.a1 = (float)dt;
+    ia.u_out = c->u;
+    if (xm) {
+      ia.F = c->F[1];
+      ia.aF = (float)(dt * p.extra_mobility);
+      ia.aN = fact2;
+      ia.nnoise = 2;
+    }
+    launch_interp(ia, c->s);
+    mark(c, "interp");
+    c->last_force_set = 1;
+  }
+}
+
+// Quasi-neutrally-buoyant particles, excess mass 0: src/runSchemeQuasiNeutrallyBuoyant.cu:178-677
+// with the first-step bootstrap of src/firstStepQuasiNeutrallyBuoyant.cu:35-1152.
+void step_qnb(fhd_ctx* c) {
+  if (!c->projected) project_initial(c);
+  const fhd_params& p = c->p;
+  const double dV = c->dV, rho = p.densfluid, dt = p.dt;
+  FluidWArgs w = base_w(c);
+  w.v = c->v;
+  w.lap_coef = (float)(0.5 * dt * p.shearviscosity / rho);
+  w.noise = 1;
+  w.trace_free = 1;
+  w.f1 = (float)sqrt(4.0 * p.temperature * p.shearviscosity * dt / (dV * rho * rho));
+  w.f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity * dt / (dV * rho * rho));
+  const float cn = (float)(0.5 * dt * p.shearviscosity / rho);
+  if (c->np > 0) {
+    sort_set(c, 0);
+    InterpArgs ia = interp_base(c, 0);
+    ia.base_cell = c->cell[0];
+    ia.base_off = c->off[0];
+    ia.out_cell = c->cell[1];
+    ia.out_off = c->off[1];
+    ia.f1 = c->v;
+    ia.a1 = (float)(0.5 * dt);
+    launch_interp(ia, c->s);
+    mark(c, "interp");
+    sort_set(c, 1);
+    wca(c, 1, c->F[1]);
+    const double mf = 1.5 * dV * p.volume_particle * rho;
+    const double ratio = (p.mass + mf) != 0.0 ? mf / (p.mass + mf) : 1.0;
+    SpreadArgs sa = spread_base(c);
+    sa.ngroups = 2;
+    sa.grp[0].set = sset(c, 1);
+    sa.grp[0].mode = SRC_VALUE;
+    sa.grp[0].val = c->F[1];
+    sa.grp[0].coef = 1.0f;
+    sa.grp[1].set = sset(c, 1);
+    sa.grp[1].mode = SRC_DRIFT;
+    sa.grp[1].coef = (float)(p.temperature * ratio);
+    launch_spread(sa, c->s);
+    mark(c, "spread");
+    w.force = c->SF;
+    w.force_coef = (float)(dt / rho);
+    c->last_force_set = 1;
+  }
+  if (!c->qnb_started) {
+    // predictor with first-order advection; stores A(v^n) as the Adams-Bashforth history
+    w.vadv = c->v;
+    w.adv_coef = (float)(-dt);
+    w.adv_store = c->adv[c->adv_cur];
+    launch_fluid_w(w, c->s);
+    mark(c, "fluid_w");
+    fft_forward(c, c->W);
+    fft_solve(c, 0, cn);
+    fft_inverse(c, c->v2, c->v, 0.5f, 0.5f);  // v^{n+1/2}
+    // corrector: advection at v^{n+1/2}
+    w.vadv = c->v2;
+    w.adv_store = nullptr;
+    launch_fluid_w(w, c->s);
+    mark(c, "fluid_w");
+    c->qnb_started = true;
+  } else {
+    w.vadv = c->v;
+    w.adv_coef = (float)(-1.5 * dt);
+    w.adv_prev = c->adv[c->adv_cur];
+    w.adv_prev_coef = (float)(0.5 * dt);
+    w.adv_store = c->adv[1 - c->adv_cur];
+    launch_fluid_w(w, c->s);
+    mark(c, "fluid_w");
+    c->adv_cur = 1 - c->adv_cur;
+  }
+  fft_forward(c, c->W);
+  fft_solve(c, 0, cn);
+  fft_inverse(c, c->v2, nullptr, 1.0f, 0.0f);  // v~^{n+1}
+  if (c->np > 0) {
+    // q^{n+1} = q^n + dt/2 J(q^{n+1/2}) (v^n + v~); u = J(q^{n+1/2}) v~
+    InterpArgs ia = interp_base(c, 1);
+    ia.base_cell = c->cell[0];
+    ia.base_off = c->off[0];
+    ia.out_cell = c->cell[0];
+    ia.out_off = c->off[0];
+    ia.f1 = c->v;
+    ia.a1 = (float)(0.5 * dt);
+    ia.f2 = c->v2;
+    ia.a2 = (float)(0.5 * dt);
+    ia.u_out = c->u;
+    launch_interp(ia, c->s);
+    mark(c, "interp");
+  }
+  std::swap(c->v, c->v2);
+}
+
+void do_step(fhd_ctx* c) {
+  switch (c->p.scheme) {
+    case FHD_SCHEME_INCOMPRESSIBLE: step_incompressible(c); break;
+    case FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT: step_qnb(c); break;
+    case FHD_SCHEME_STOKES_LIMIT: step_stokes(c); break;
+    default: throw std::runtime_error("unknown scheme");
+  }
+  c->step++;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// data.main front end (grammar of src/loadDataMain.cu:173-683).
+// ------------------------------------------------------------------------------------------
+extern "C" unsigned long fhd_params_size(void) { return (unsigned long)sizeof(fhd_params); }
+
+extern "C" void fhd_default_params(fhd_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->scheme = -1;
+  p->densfluid = 1.0;
+  p->shearviscosity = 1.0;
+  p->temperature = 0.0;
+  p->dt = 0.0;
+  p->seed = 0;
+  p->kernel_points = 3;
+  p->nonbonded = 1;
+  p->cutoff = 0.0;
+  p->wca_sigma = -1.0;   // -1: 2 dx (src/initForcesNonBonded.cu:24)
+  p->wca_epsilon = -1.0; // -1: kT  (src/initForcesNonBonded.cu:25)
+  p->mass = 0.0;
+  p->volume_particle = 8.0;  // src/createParticles.cpp:141-143
+  p->rfd_delta = 1e-3;
+  p->rfd_mode = FHD_RFD_DISPLACED;
+  p->max_part_in_cell = 6;
+  p->max_part_in_cell_nonbonded = 6;
+  p->initial_positions = 1;  // simple cubic unless coordinates are given
+  strcpy(p->outputname, "run");
+}
+
+namespace {
+
+bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = std::string("cannot open ") + path;
+    return false;
+  }
+  fhd_params p;
+  fhd_default_params(&p);
+  bool incompressible = false, qnb = false, stokes = false, setparticles = false, seed_set = false;
+  bool vol_set = false;
+  double vol = 8.0;
+  std::string word, last;
+  auto need = [&](bool ok) {
+    if (!ok) throw std::runtime_error("ERROR INPUT FILE: bad value after " + word);
+  };
+  const char* unsupported[] = {"binaryMixture", "binaryMixtureWall", "giantFluctuations", "continuousGradient",
+                               "thermostat", "incompressibleBoundaryRK2", "quasiNeutrallyBuoyant2D",
+                               "quasiNeutrallyBuoyant4pt2D", "quasi2D", "stokesLimit2D", "particlesWall",
+                               "freeEnergyCompressibleParticles", "semiImplicitCompressibleParticles",
+                               "momentumCoupling", "stokesLimitFirstOrder", "incompressibleBinaryMixture",
+                               "incompressibleBinaryMixtureMidPoint", "ghost", "boundary", "bondedForces"};
+  try {
+    while (f >> word) {
+      if (word[0] == '#') {
+        std::string rest;
+        std::getline(f, rest);
+        continue;
+      }
+      bool handled = true;
+      if (word == "incompressible") incompressible = true;
+      else if (word == "quasiNeutrallyBuoyant") qnb = true;
+      else if (word == "stokesLimit") { stokes = true; setparticles = true; }
+      else if (word == "particles") { int v; need(bool(f >> v)); setparticles = v != 0; }
+      else if (word == "numberparticles") need(bool(f >> p.np));
+      else if (word == "mass") need(bool(f >> p.mass));
+      else if (word == "volumeParticle") { need(bool(f >> vol)); vol_set = true; }
+      else if (word == "cutoff") need(bool(f >> p.cutoff));
+      else if (word == "maxNumberPartInCell") need(bool(f >> p.max_part_in_cell));
+      else if (word == "maxNumberPartInCellNonBonded") need(bool(f >> p.max_part_in_cell_nonbonded));
+      else if (word == "densfluid") need(bool(f >> p.densfluid));
+      else if (word == "shearviscosity") need(bool(f >> p.shearviscosity));
+      else if (word == "bulkviscosity") { double v; need(bool(f >> v)); }
+      else if (word == "temperature") need(bool(f >> p.temperature));
+      else if (word == "cells") need(bool(f >> p.mx >> p.my >> p.mz));
+      else if (word == "celldimension") need(bool(f >> p.lx >> p.ly >> p.lz));
+      else if (word == "initfluid") need(bool(f >> p.initfluid));
+      else if (word == "backgroundvelocity") need(bool(f >> p.background[0] >> p.background[1] >> p.background[2]));
+      else if (word == "seed") { need(bool(f >> p.seed)); seed_set = true; }
+      else if (word == "numstepsRelaxation") need(bool(f >> p.numsteps_relaxation));
+      else if (word == "numsteps") need(bool(f >> p.numsteps));
+      else if (word == "samplefreq") need(bool(f >> p.samplefreq));
+      else if (word == "savefreq") { long long v; need(bool(f >> v)); }
+      else if (word == "dt") need(bool(f >> p.dt));
+      else if (word == "saveVTK" || word == "saveFluid" || word == "setDevice") { int v; need(bool(f >> v)); }
+      else if (word == "outputname") { std::string s; need(bool(f >> s)); snprintf(p.outputname, sizeof(p.outputname), "%s", s.c_str()); }
+      else if (word == "loadparticles") { int v; need(bool(f >> v)); if (v) p.initial_positions = 2; }
+      else if (word == "coordinates") { std::string s; need(bool(f >> s)); snprintf(p.coordinates, sizeof(p.coordinates), "%s", s.c_str()); }
+      else if (word == "velocities") { std::string s; need(bool(f >> s)); }
+      else if (word == "computeNonBondedForces") need(bool(f >> p.nonbonded));
+      else if (word == "extraMobility") need(bool(f >> p.extra_mobility));
+      // extension keywords
+      else if (word == "kernel") need(bool(f >> p.kernel_points));
+      else if (word == "sigma") need(bool(f >> p.wca_sigma));
+      else if (word == "epsilon") need(bool(f >> p.wca_epsilon));
+      else if (word == "rfdMode") need(bool(f >> p.rfd_mode));
+      else if (word == "rfdDelta") need(bool(f >> p.rfd_delta));
+      else if (word == "initialPositions") need(bool(f >> p.initial_positions));
+      else handled = false;
+      if (!handled) {
+        for (const char* u : unsupported)
+          if (word == u) throw std::runtime_error("scheme or option '" + word + "' is outside this library (fluam's " +
+                                                  "incompressible, quasiNeutrallyBuoyant and stokesLimit paths only)");
+        throw std::runtime_error("ERROR INPUT FILE; LAST WORD READ: " + last + "; UNKNOWN: " + word);
+      }
+      last = word;
+    }
+  } catch (const std::exception& e) {
+    err = e.what();
+    return false;
+  }
+  // scheme precedence of src/main.cpp:70-99 restricted to the three schemes
+  if (setparticles && incompressible) {
+    err = "particles with incompressible select fluam's incompressibleBoundary scheme, which is outside this library";
+    return false;
+  }
+  if (incompressible) p.scheme = FHD_SCHEME_INCOMPRESSIBLE;
+  else if (qnb) p.scheme = FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT;
+  else if (stokes) p.scheme = FHD_SCHEME_STOKES_LIMIT;
+  else {
+    err = "no scheme keyword (incompressible, quasiNeutrallyBuoyant or stokesLimit)";
+    return false;
+  }
+  if (!setparticles && p.scheme != FHD_SCHEME_STOKES_LIMIT) p.np = 0;
+  if (!seed_set) p.seed = (unsigned long long)(time(nullptr) % 18671);  // src/initializeRandomNumber.cpp:24-28
+  double dx = p.lx / (p.mx > 0 ? p.mx : 1);
+  double cvol = (p.lx / std::max(p.mx, 1)) * (p.ly / std::max(p.my, 1)) * (p.lz / std::max(p.mz, 1));
+  p.volume_particle = vol_set ? vol / cvol : 8.0;  // src/createParticles.cpp:127-143
+  if (p.wca_sigma < 0) p.wca_sigma = 2.0 * dx;
+  if (p.wca_epsilon < 0) p.wca_epsilon = p.temperature;
+  if (p.initial_positions == 2 && p.coordinates[0] == 0) {
+    err = "loadparticles 1 needs a coordinates file";
+    return false;
+  }
+  *out = p;
+  return true;
+}
+
+// coordinates file: first number np, then np lines "x y z" (src/createParticles.cpp:81-100).
+bool read_coordinates(const fhd_params& p, std::vector<double>& q, std::string& err) {
+  std::ifstream f(p.coordinates);
+  if (!f) {
+    err = std::string("cannot open coordinates file ") + p.coordinates;
+    return false;
+  }
+  int n = 0;
+  f >> n;
+  if (n != p.np) {
+    err = "coordinates file particle count differs from numberparticles";
+    return false;
+  }
+  q.assign(3 * (size_t)n, 0.0);
+  for (int i = 0; i < n; ++i)
+    if (!(f >> q[i] >> q[n + i] >> q[2 * (size_t)n + i])) {
+      err = "coordinates file truncated";
+      return false;
+    }
+  return true;
+}
+
+}  // namespace
+
+// Philox-based uniform draw shared with the FP64 restatement (seeded uniform positions).
+namespace {
+void philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1,
+             n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void initial_positions(const fhd_params& p, std::vector<double>& q) {
+  const int np = p.np;
+  q.assign(3 * (size_t)np, 0.0);
+  if (p.initial_positions == 1) {
+    // simple cubic lattice, src/simpleCubic.cpp:25-63
+    double sigma = pow(p.lx * p.ly * p.lz / np, 1.0 / 3.0);
+    int nx = (int)(p.lx / sigma) ? (int)(p.lx / sigma) : 1;
+    int ny = (int)(p.ly / sigma) ? (int)(p.ly / sigma) : 1;
+    int nz = (int)(p.lz / sigma) ? (int)(p.lz / sigma) : 1;
+    while (nx * ny * nz < np) {
+      if (nx * ny * nz < np) nx++;
+      if (nx * ny * nz < np) ny++;
+      if (nx * ny * nz < np) nz++;
+    }
+    double ddx = p.lx / nx, ddy = p.ly / ny, ddz = p.lz / nz;
+    int n = 0;
+    for (int i = 0; i < nz && n < np; ++i)
+      for (int j = 0; j < ny && n < np; ++j)
+        for (int k = 0; k < nx && n < np; ++k) {
+          q[n] = (k + 0.5) * ddx - p.lx / 2;
+          q[np + n] = (j + 0.5) * ddy - p.ly / 2;
+          q[2 * (size_t)np + n] = (i + 0.5) * ddz - p.lz / 2;
+          ++n;
+        }
+    return;
+  }
+  // seeded uniform positions: Philox draw (seed, step 0, stream 8, index i), 24-bit uniforms
+  const uint32_t key[2] = {(uint32_t)p.seed, (uint32_t)(p.seed >> 32)};
+  for (int i = 0; i < np; ++i) {
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)FHD_STREAM_INIT_POS << 4, 0u, 0u}, w[4];
+    philox_host(ctr, key, w);
+    q[i] = ((double)(w[0] >> 8) / 16777216.0 - 0.5) * p.lx;
+    q[np + i] = ((double)(w[1] >> 8) / 16777216.0 - 0.5) * p.ly;
+    q[2 * (size_t)np + i] = ((double)(w[2] >> 8) / 16777216.0 - 0.5) * p.lz;
+  }
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI.
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* fhd_last_error(void) { return g_last_error.c_str(); }
+
+int fhd_load_datamain(const char* path) {
+  fhd_params p;
+  std::string err;
+  if (!parse_datamain(path, &p, err)) {
+    g_last_error = err;
+    return 0;
+  }
+  g_params = p;
+  g_params_loaded = true;
+  return 1;
+}
+
+int fhd_get_params(fhd_params* out) {
+  if (!g_params_loaded) {
+    g_last_error = "no data.main loaded";
+    return 0;
+  }
+  *out = g_params;
+  return 1;
+}
+
+int fhd_set_params(const fhd_params* in) {
+  g_params = *in;
+  g_params_loaded = true;
+  return 1;
+}
+
+fhd_ctx* fhd_create(const fhd_params* pin) {
+  fhd_ctx* c = new fhd_ctx();
+  try {
+    c->p = *pin;
+    if (c->p.wca_sigma < 0) c->p.wca_sigma = 2.0 * c->p.lx / c->p.mx;
+    if (c->p.wca_epsilon < 0) c->p.wca_epsilon = c->p.temperature;
+    validate(c->p);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device");
+    HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    const fhd_params& p = c->p;
+    c->dx = p.lx / p.mx;
+    c->dy = p.ly / p.my;
+    c->dz = p.lz / p.mz;
+    c->dV = c->dx * c->dy * c->dz;
+    GridDev& g = c->g;
+    g.mx = p.mx; g.my = p.my; g.mz = p.mz;
+    g.N = p.mx * p.my * p.mz;
+    g.P = p.mx / 2 + 1;
+    g.dx = (float)c->dx; g.dy = (float)c->dy; g.dz = (float)c->dz;
+    g.idx = (float)(1.0 / c->dx); g.idy = (float)(1.0 / c->dy); g.idz = (float)(1.0 / c->dz);
+    g.lx = (float)p.lx; g.ly = (float)p.ly; g.lz = (float)p.lz;
+    const size_t N3 = 3 * (size_t)g.N;
+    c->v = dalloc<float>(N3);
+    c->v2 = dalloc<float>(N3);
+    c->W = dalloc<float>(N3);
+    c->SF = dalloc<float>(N3);
+    c->div = dalloc<float>(g.N);
+    c->spec = dalloc<float2>(3 * (size_t)p.mz * p.my * g.P);
+    if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) {
+      c->adv[0] = dalloc<float>(N3);
+      c->adv[1] = dalloc<float>(N3);
+    }
+    c->errd = dalloc<int>(8);
+    HIPCHK(hipMemset(c->errd, 0, 8 * sizeof(int)));
+    build_tables(c);
+    // initial velocity: initfluid 0 -> background velocity (src/initializeFluid.cpp:31-60)
+    std::vector<float> v0(N3, 0.0f);
+    if (p.scheme != FHD_SCHEME_STOKES_LIMIT)
+      for (int a = 0; a < 3; ++a) std::fill(v0.begin() + (size_t)a * g.N, v0.begin() + (size_t)(a + 1) * g.N, (float)p.background[a]);
+    HIPCHK(hipMemcpy(c->v, v0.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->v2, 0, N3 * sizeof(float)));
+    HIPCHK(hipMemset(c->SF, 0, N3 * sizeof(float)));
+    c->np = p.np;
+    if (c->np > 0) {
+      Tiles& t = c->t;
+      t.tx = tile_edge(p.mx); t.ty = tile_edge(p.my); t.tz = tile_edge(p.mz);
+      t.ntx = p.mx / t.tx; t.nty = p.my / t.ty; t.ntz = p.mz / t.tz;
+      t.n = t.ntx * t.nty * t.ntz;
+      for (int k = 0; k < 2; ++k) {
+        c->cell[k] = dalloc<int4>(c->np);
+        c->off[k] = dalloc<float4>(c->np);
+        c->F[k] = dalloc<float4>(c->np);
+        c->start[k] = dalloc<int>(t.n + 1);
+        c->sid[k] = dalloc<int>(c->np);
+        c->scell[k] = dalloc<int4>(c->np);
+        c->soff[k] = dalloc<float4>(c->np);
+        HIPCHK(hipMemset(c->F[k], 0, sizeof(float4) * c->np));
+      }
+      c->u = dalloc<float4>(c->np);
+      HIPCHK(hipMemset(c->u, 0, sizeof(float4) * c->np));
+      c->count = dalloc<int>(t.n);
+      HIPCHK(hipMemset(c->count, 0, sizeof(int) * t.n));
+      c->slot = dalloc<int>(c->np);
+      c->tmp = dalloc<int>(c->np);
+      std::vector<double> q;
+      if (p.initial_positions == 2) {
+        std::string err;
+        if (!read_coordinates(p, q, err)) throw std::runtime_error(err);
+      } else {
+        initial_positions(p, q);
+      }
+      if (!fhd_set_particles(c, q.data())) throw std::runtime_error(c->err);
+    }
+    HIPCHK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    free_ctx(c);
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+void fhd_destroy(fhd_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->s);
+  free_ctx(c);
+  delete c;
+}
+
+const char* fhd_ctx_error(const fhd_ctx* c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+#define CTX_TRY(c, ...)                  \
+  do {                                   \
+    if (!(c)) {                          \
+      g_last_error = "null context";     \
+      return 0;                          \
+    }                                    \
+    try {                                \
+      __VA_ARGS__;                       \
+    } catch (const std::exception& e) {  \
+      (c)->err = e.what();               \
+      g_last_error = e.what();           \
+      return 0;                          \
+    }                                    \
+  } while (0)
+
+int fhd_set_velocity(fhd_ctx* c, const double* v) {
+  CTX_TRY(c, {
+    const size_t n = 3 * (size_t)c->g.N;
+    std::vector<float> h(n);
+    for (size_t i = 0; i < n; ++i) h[i] = (float)v[i];
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(c->v, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    c->projected = false;
+  });
+  return 1;
+}
+
+int fhd_get_velocity(fhd_ctx* c, double* v) {
+  CTX_TRY(c, {
+    const size_t n = 3 * (size_t)c->g.N;
+    std::vector<float> h(n);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(h.data(), c->v, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) v[i] = h[i];
+  });
+  return 1;
+}
+
+int fhd_set_particles(fhd_ctx* c, const double* q) {
+  CTX_TRY(c, {
+    const int np = c->np;
+    std::vector<int4> hc(np);
+    std::vector<float4> ho(np);
+    const double L[3] = {c->p.lx, c->p.ly, c->p.lz}, h[3] = {c->dx, c->dy, c->dz};
+    for (int i = 0; i < np; ++i) {
+      int cc[3];
+      float d[3];
+      for (int a = 0; a < 3; ++a) {
+        double qa = q[(size_t)a * np + i];
+        if (!std::isfinite(qa)) throw std::runtime_error("non-finite particle position");
+        to_cell(qa, L[a], h[a], cc[a], d[a]);
+      }
+      hc[i] = make_int4(cc[0], cc[1], cc[2], 0);
+      ho[i] = make_float4(d[0], d[1], d[2], 0.0f);
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (np > 0) {
+      HIPCHK(hipMemcpy(c->cell[0], hc.data(), np * sizeof(int4), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->off[0], ho.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+    }
+  });
+  return 1;
+}
+
+int fhd_get_particles(fhd_ctx* c, double* q) {
+  CTX_TRY(c, {
+    const int np = c->np;
+    std::vector<int4> hc(np);
+    std::vector<float4> ho(np);
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (np > 0) {
+      HIPCHK(hipMemcpy(hc.data(), c->cell[0], np * sizeof(int4), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(ho.data(), c->off[0], np * sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    for (int i = 0; i < np; ++i) {
+      q[i] = from_cell(hc[i].x, ho[i].x, c->p.lx, c->dx);
+      q[np + i] = from_cell(hc[i].y, ho[i].y, c->p.ly, c->dy);
+      q[2 * (size_t)np + i] = from_cell(hc[i].z, ho[i].z, c->p.lz, c->dz);
+    }
+  });
+  return 1;
+}
+
+static int get_f4(fhd_ctx* c, const float4* d, double* out) {
+  CTX_TRY(c, {
+    const int np = c->np;
+    std::vector<float4> h(np);
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (np > 0 && d) HIPCHK(hipMemcpy(h.data(), d, np * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < np; ++i) {
+      out[i] = h[i].x;
+      out[np + i] = h[i].y;
+      out[2 * (size_t)np + i] = h[i].z;
+    }
+  });
+  return 1;
+}
+
+int fhd_get_particle_velocities(fhd_ctx* c, double* u) { return get_f4(c, c ? c->u : nullptr, u); }
+int fhd_get_forces(fhd_ctx* c, double* f) { return get_f4(c, c ? c->F[c->last_force_set] : nullptr, f); }
+
+int fhd_step(fhd_ctx* c, int nsteps) {
+  CTX_TRY(c, {
+    for (int k = 0; k < nsteps; ++k) do_step(c);
+    HIPCHK(hipGetLastError());
+  });
+  return 1;
+}
+
+int fhd_sync(fhd_ctx* c) {
+  CTX_TRY(c, {
+    HIPCHK(hipStreamSynchronize(c->s));
+    int e[8];
+    HIPCHK(hipMemcpy(e, c->errd, sizeof(e), hipMemcpyDeviceToHost));
+    if (e[0]) throw std::runtime_error("non-finite particle displacement detected on the device");
+  });
+  return 1;
+}
+
+long long fhd_get_step(const fhd_ctx* c) { return c ? c->step : -1; }
+
+int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms) {
+  CTX_TRY(c, {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    HIPCHK(hipEventRecord(a, c->s));
+    for (int k = 0; k < nsteps; ++k) do_step(c);
+    HIPCHK(hipEventRecord(b, c->s));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  });
+  return 1;
+}
+
+int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern) {
+  int nout = 0;
+  CTX_TRY(c, {
+    for (auto& m : c->marks) (void)hipEventDestroy(m.second);
+    c->marks.clear();
+    c->profiling = true;
+    mark(c, "begin");
+    for (int k = 0; k < nsteps; ++k) do_step(c);
+    c->profiling = false;
+    HIPCHK(hipStreamSynchronize(c->s));
+    std::vector<std::string> names;
+    std::vector<double> tot;
+    std::vector<int> cnt;
+    for (size_t i = 1; i < c->marks.size(); ++i) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, c->marks[i - 1].second, c->marks[i].second));
+      const std::string& nm = c->marks[i].first;
+      size_t j = 0;
+      for (; j < names.size(); ++j)
+        if (names[j] == nm) break;
+      if (j == names.size()) { names.push_back(nm); tot.push_back(0); cnt.push_back(0); }
+      tot[j] += t;
+      cnt[j] += 1;
+    }
+    for (size_t j = 0; j < names.size() && (int)j < nkern; ++j) {
+      kern_ms[2 * j] = tot[j] / std::max(cnt[j], 1);
+      kern_ms[2 * j + 1] = cnt[j];
+      snprintf(kern_names + 48 * j, 48, "%s", names[j].c_str());
+      ++nout;
+    }
+    for (auto& m : c->marks) (void)hipEventDestroy(m.second);
+    c->marks.clear();
+  });
+  return nout;
+}
+
+int fhd_max_divergence(fhd_ctx* c, double* maxdiv) {
+  CTX_TRY(c, {
+    launch_divergence(c->g, c->v, c->div, c->s);
+    std::vector<float> h(c->g.N);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(h.data(), c->div, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    double m = 0;
+    for (float x : h) m = std::max(m, (double)fabsf(x));
+    *maxdiv = m;
+  });
+  return 1;
+}
+
+// ---------------- building blocks for parity tests ----------------
+int fhd_fft_r2c(int mx, int my, int mz, const double* in, double* out_spec) {
+  fhd_params p;
+  fhd_default_params(&p);
+  p.scheme = FHD_SCHEME_STOKES_LIMIT;
+  p.mx = mx; p.my = my; p.mz = mz;
+  p.lx = mx; p.ly = my; p.lz = mz;
+  p.dt = 1.0;
+  fhd_ctx* c = fhd_create(&p);
+  if (!c) return 0;
+  int ok = 1;
+  try {
+    const size_t N3 = 3 * (size_t)c->g.N;
+    std::vector<float> h(N3, 0.0f);
+    for (size_t i = 0; i < (size_t)c->g.N; ++i) h[i] = (float)in[i];
+    HIPCHK(hipMemcpy(c->W, h.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
+    fft_forward(c, c->W);
+    const size_t ns = (size_t)mz * my * c->g.P;
+    std::vector<float2> sp(ns);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(sp.data(), c->spec, ns * sizeof(float2), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < ns; ++i) { out_spec[2 * i] = sp[i].x; out_spec[2 * i + 1] = sp[i].y; }
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    ok = 0;
+  }
+  fhd_destroy(c);
+  return ok;
+}
+
+int fhd_fft_c2r(int mx, int my, int mz, const double* in_spec, double* out) {
+  fhd_params p;
+  fhd_default_params(&p);
+  p.scheme = FHD_SCHEME_STOKES_LIMIT;
+  p.mx = mx; p.my = my; p.mz = mz;
+  p.lx = mx; p.ly = my; p.lz = mz;
+  p.dt = 1.0;
+  fhd_ctx* c = fhd_create(&p);
+  if (!c) return 0;
+  int ok = 1;
+  try {
+    const size_t ns = (size_t)mz * my * c->g.P;
+    std::vector<float2> sp(3 * ns, make_float2(0.0f, 0.0f));
+    for (size_t i = 0; i < ns; ++i) sp[i] = make_float2((float)in_spec[2 * i], (float)in_spec[2 * i + 1]);
+    HIPCHK(hipMemcpy(c->spec, sp.data(), 3 * ns * sizeof(float2), hipMemcpyHostToDevice));
+    fft_inverse(c, c->W, nullptr, 1.0f, 0.0f);
+    std::vector<float> h(c->g.N);
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(h.data(), c->W, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    ok = 0;
+  }
+  fhd_destroy(c);
+  return ok;
+}
+
+int fhd_normals(unsigned long long seed, long long step, int stream, int sub, int n, double* out) {
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device");
+    Key k;
+    k.k0 = (uint32_t)seed;
+    k.k1 = (uint32_t)(seed >> 32);
+    k.step_lo = (uint32_t)(uint64_t)step;
+    k.step_hi = (uint32_t)((uint64_t)step >> 32);
+    float* d = dalloc<float>(4 * (size_t)n);
+    launch_normals(k, stream, sub, n, d, 0);
+    std::vector<float> h(4 * (size_t)n);
+    HIPCHK(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(d));
+    for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 0;
+  }
+  return 1;
+}
+
+// Internal: lets the scheme drivers report through fhd_last_error().
+void fhd_internal_set_error(const char* e) { g_last_error = e ? e : ""; }
+
+}  // extern "C"

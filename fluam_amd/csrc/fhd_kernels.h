@@ -1,0 +1,130 @@
+// fhd_kernels.h -- kernel argument structs and host-side launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fhd_device.h"
+
+namespace fhd {
+
+constexpr int kMaxFftLen = 256;
+
+struct FftArgs {
+  GridDev g;
+  const float2 *twx, *twy, *twz;  // e^{-2 pi i m / m_a}, m < m_a
+  const float *gx, *gy, *gz;      // g_a(k) = (2/h_a) sin(pi k / m_a)
+  const float2 *ex, *ey, *ez;     // e^{+i pi k / m_a}
+  uint32_t plan_h, plan_y, plan_z;  // radices, 4 bits per stage
+  int nst_h, nst_y, nst_z;
+  int ztile;                      // kx columns per pass-B workgroup
+  int stokes;                     // 1: v = P W / (eta |g|^2), mode 0 -> 0
+  float eta;
+  float cn_coef;                  // incompressible: 1 - cn_coef * L (0: projection only)
+  float inv_n;
+};
+
+struct FluidWArgs {
+  GridDev g;
+  const float* v;          // W += v + lap_coef * L v (nullptr: term off)
+  float lap_coef;
+  const float* vadv;       // W += adv_coef * A(vadv) (nullptr: off)
+  float adv_coef;
+  const float* adv_prev;   // W += adv_prev_coef * adv_prev (nullptr: off)
+  float adv_prev_coef;
+  float* adv_store;        // stores A(vadv) when non-null
+  const float* force;      // W += force_coef * force (nullptr: off)
+  float force_coef;
+  int noise;               // 0 none, 1 Sigma_1, 2 Sigma_1 + Sigma_2
+  int trace_free;
+  float f1, f4;            // diagonal / off-diagonal noise prefactors
+  Key key;
+  float* W;
+  int zchunk;
+};
+
+// Tile decomposition used for particle binning and deterministic spreading.
+struct Tiles {
+  int tx, ty, tz;       // tile edge in cells
+  int ntx, nty, ntz;    // tiles per axis
+  int n;                // total tiles
+};
+
+struct SortedSet {
+  const int4* cell;     // sorted copies
+  const float4* off;
+  const int* id;        // original particle index of each sorted slot
+  const int* start;     // tile_start[ntiles + 1]
+};
+
+// Source groups for the spreading kernel.
+enum { SRC_VALUE = 0, SRC_RFD = 1, SRC_DRIFT = 2 };
+struct SpreadGroup {
+  SortedSet set;
+  int mode;
+  const float4* val;    // SRC_VALUE: per-particle vector value (by id)
+  float coef;           // multiplies the value (SRC_RFD: kT/delta; SRC_DRIFT: kT ratio)
+  float disp;           // SRC_RFD: displacement magnitude delta/2
+  int rfd_quirk;        // SRC_RFD: weights at the undisplaced position
+};
+
+struct SpreadArgs {
+  GridDev g;
+  Tiles t;
+  int kernel_points;
+  int ngroups;
+  SpreadGroup grp[2];
+  Key key;
+  float out_coef;       // multiplies the accumulated field (1/dV)
+  float* out;           // 3N field, every node written
+};
+
+struct InterpArgs {
+  GridDev g;
+  Tiles t;
+  int kernel_points;
+  SortedSet eval;        // positions where J is evaluated (sorted for locality)
+  const int4* base_cell; // base positions (by id)
+  const float4* base_off;
+  int4* out_cell;        // out = base + a1 J f1 + a2 J f2 + extra (by id)
+  float4* out_off;
+  const float* f1; float a1;
+  const float* f2; float a2;
+  float4* u_out;         // J f2 (or J f1 when f2 is null), optional
+  const float4* F; float aF;           // extra mobility: + aF F
+  float aN; int nnoise;                // + aN (Z_1 [+ Z_2]) particle noise
+  Key key;
+  int* err;
+  int np;
+};
+
+struct WcaArgs {
+  GridDev g;
+  Tiles t;
+  SortedSet set;
+  float rc2, sig2, eps48;
+  float4* F;   // by id
+  int np;
+};
+
+// Launch helpers (defined in the .hip files).
+void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s);
+void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s);
+void launch_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha,
+                       float beta, hipStream_t s);
+void launch_fluid_w(const FluidWArgs& a, hipStream_t s);
+void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t s);
+void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipStream_t s);
+void launch_zero_f(float* p, size_t n, hipStream_t s);
+
+void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, int* slot, int np,
+                hipStream_t s);
+void launch_scan(const Tiles& t, int* count, int* start, hipStream_t s);
+void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, const int* start,
+                    int* tmp, int np, hipStream_t s);
+void launch_tile_sort(const Tiles& t, const int* start, const int* tmp, const int4* cell, const float4* off,
+                      int* sid, int4* scell, float4* soff, hipStream_t s);
+void launch_wca(const WcaArgs& a, hipStream_t s);
+void launch_spread(const SpreadArgs& a, hipStream_t s);
+void launch_interp(const InterpArgs& a, hipStream_t s);
+void launch_zero_f4(float4* p, int n, hipStream_t s);
+
+}  // namespace fhd

@@ -1,0 +1,211 @@
+// fluid.hip -- the staggered-grid right-hand side W of each solve (fluam's kernelConstructW
+// family) with the stochastic stress generated in registers.
+//
+// One workgroup owns a 16 x 16 column of cells and marches up a chunk of z-planes.  The 6
+// stress normals of every cell of plane z+1 (plus the one-cell ring the divergence needs) are
+// drawn from the counter-based generator into an LDS ring of three planes, so each normal is
+// drawn ~1.25 times instead of once per neighbour that reads it.
+//
+// Restates: kernelConstructW_1/_2 (src/kernelConstructW.cu:32-521), the QNB variants
+// (src/kernelConstructWQuasiNeutrallyBuoyant.cu:26-705,1366-2025) and
+// kernelConstructWstokesLimit/_2 (src/stokesLimitFunctions.cu:2281-2378,3922-4069).
+#include "fhd_device.h"
+#include "fhd_kernels.h"
+
+namespace fhd {
+
+constexpr int TX = 16, TY = 16;
+constexpr int PX = TX + 2, PY = TY + 2;
+constexpr int PLANE = PX * PY;
+
+// stress slot components: 0 sXX, 1 XY, 2 XZ, 3 sYY, 4 YZ, 5 sZZ
+FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz) {
+  const GridDev& g = a.g;
+  const int z = wrapi(zz, g.mz);
+  const int nown = TX * TY, ntot = nown + 2 * TX + 2 * TY;
+  for (int e = threadIdx.x; e < ntot; e += blockDim.x) {
+    int lx, ly;
+    if (e < nown) { lx = e % TX; ly = e / TX; }
+    else {
+      int r = e - nown;
+      if (r < TY) { lx = -1; ly = r; }
+      else if (r < 2 * TY) { lx = TX; ly = r - TY; }
+      else if (r < 2 * TY + TX) { lx = r - 2 * TY; ly = -1; }
+      else { lx = r - 2 * TY - TX; ly = TY; }
+    }
+    int x = wrapi(x0 + lx, g.mx), y = wrapi(y0 + ly, g.my);
+    uint32_t cell = (uint32_t)(x + g.mx * (y + g.my * z));
+    float n[12];
+    draw_normals4(a.key, cell, FHD_STREAM_FLUID, 0, &n[0]);
+    draw_normals4(a.key, cell, FHD_STREAM_FLUID, 1, &n[4]);
+    float s[6];
+    if (a.noise == 2) {
+      draw_normals4(a.key, cell, FHD_STREAM_FLUID, 2, &n[8]);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] = n[c] + n[6 + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c] = n[c];
+    }
+    if (a.trace_free) {
+      float tr = (s[0] + s[3] + s[5]) * (1.0f / 3.0f);
+      s[0] -= tr;
+      s[3] -= tr;
+      s[5] -= tr;
+    }
+    int p = (lx + 1) + PX * (ly + 1);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) slot[c * PLANE + p] = s[c];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fluid_w(FluidWArgs a) {
+  __shared__ float nbuf[3][6 * PLANE];
+  const GridDev& g = a.g;
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+  const int zs = blockIdx.z * a.zchunk;
+  const int ze = min(zs + a.zchunk, g.mz);
+  const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
+  const int x = x0 + lx, y = y0 + ly;
+  const bool own = (x < g.mx) && (y < g.my);
+  const int N = g.N;
+  if (a.noise) {
+    gen_plane(a, nbuf[(zs - 1 + 3) % 3], x0, y0, zs - 1);
+    gen_plane(a, nbuf[zs % 3], x0, y0, zs);
+  }
+  const int xm = wrapi(x - 1, g.mx), xp = wrapi(x + 1, g.mx);
+  const int ym = wrapi(y - 1, g.my), yp = wrapi(y + 1, g.my);
+  for (int z = zs; z < ze; ++z) {
+    if (a.noise) gen_plane(a, nbuf[(z + 1) % 3], x0, y0, z + 1);
+    __syncthreads();
+    if (own) {
+      const int zm = wrapi(z - 1, g.mz), zp = wrapi(z + 1, g.mz);
+      const int i = x + g.mx * (y + g.my * z);
+      float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+      if (a.noise) {
+        const float* cur = nbuf[z % 3];
+        const float* prv = nbuf[(z + 2) % 3];
+        const float* nxt = nbuf[(z + 1) % 3];
+        const int p = (lx + 1) + PX * (ly + 1);
+        const float f1 = a.f1, f4 = a.f4;
+        wx = g.idx * f1 * (cur[0 * PLANE + p + 1] - cur[0 * PLANE + p]) +
+             g.idy * f4 * (cur[1 * PLANE + p] - cur[1 * PLANE + p - PX]) +
+             g.idz * f4 * (cur[2 * PLANE + p] - prv[2 * PLANE + p]);
+        wy = g.idy * f1 * (cur[3 * PLANE + p + PX] - cur[3 * PLANE + p]) +
+             g.idx * f4 * (cur[1 * PLANE + p] - cur[1 * PLANE + p - 1]) +
+             g.idz * f4 * (cur[4 * PLANE + p] - prv[4 * PLANE + p]);
+        wz = g.idz * f1 * (nxt[5 * PLANE + p] - cur[5 * PLANE + p]) +
+             g.idx * f4 * (cur[2 * PLANE + p] - cur[2 * PLANE + p - 1]) +
+             g.idy * f4 * (cur[4 * PLANE + p] - cur[4 * PLANE + p - PX]);
+      }
+      const int n0 = x + g.mx * (y + g.my * zm), n5 = x + g.mx * (y + g.my * zp);
+      const int n1 = x + g.mx * (ym + g.my * z), n4 = x + g.mx * (yp + g.my * z);
+      const int n2 = xm + g.mx * (y + g.my * z), n3 = xp + g.mx * (y + g.my * z);
+      if (a.v) {
+        const float* V = a.v;
+        const float ax = g.idx * g.idx, ay = g.idy * g.idy, az = g.idz * g.idz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float* f = V + (size_t)c * N;
+          float fc = f[i];
+          float L = ax * (f[n3] - 2.0f * fc + f[n2]) + ay * (f[n4] - 2.0f * fc + f[n1]) +
+                    az * (f[n5] - 2.0f * fc + f[n0]);
+          float t = fc + a.lap_coef * L;
+          if (c == 0) wx += t; else if (c == 1) wy += t; else wz += t;
+        }
+      }
+      if (a.vadv) {
+        const float* VX = a.vadv;
+        const float* VY = a.vadv + N;
+        const float* VZ = a.vadv + 2 * (size_t)N;
+        const int pxmy = xp + g.mx * (ym + g.my * z), pxmz = xp + g.mx * (y + g.my * zm);
+        const int mxpy = xm + g.mx * (yp + g.my * z), mxpz = xm + g.mx * (y + g.my * zp);
+        const int pymz = x + g.mx * (yp + g.my * zm), mypz = x + g.mx * (ym + g.my * zp);
+        float vx = VX[i], vy = VY[i], vz = VZ[i];
+        float vx0 = VX[n0], vx1 = VX[n1], vx2 = VX[n2], vx3 = VX[n3], vx4 = VX[n4], vx5 = VX[n5];
+        float vy0 = VY[n0], vy1 = VY[n1], vy2 = VY[n2], vy3 = VY[n3], vy4 = VY[n4], vy5 = VY[n5];
+        float vz0 = VZ[n0], vz1 = VZ[n1], vz2 = VZ[n2], vz3 = VZ[n3], vz4 = VZ[n4], vz5 = VZ[n5];
+        // advection (dt/4) sum_b D_b[(v_a + v_a')(v_b + v_b')], src/kernelConstructW.cu:120-136
+        float Ax = g.idx * ((vx3 + vx) * (vx3 + vx) - (vx + vx2) * (vx + vx2)) +
+                   g.idy * ((vx4 + vx) * (vy3 + vy) - (vx + vx1) * (VY[pxmy] + vy1)) +
+                   g.idz * ((vx5 + vx) * (vz3 + vz) - (vx + vx0) * (VZ[pxmz] + vz0));
+        float Ay = g.idx * ((vy3 + vy) * (vx4 + vx) - (vy + vy2) * (VX[mxpy] + vx2)) +
+                   g.idy * ((vy4 + vy) * (vy4 + vy) - (vy + vy1) * (vy + vy1)) +
+                   g.idz * ((vy5 + vy) * (vz4 + vz) - (vy + vy0) * (VZ[pymz] + vz0));
+        float Az = g.idx * ((vz3 + vz) * (vx5 + vx) - (vz + vz2) * (VX[mxpz] + vx2)) +
+                   g.idy * ((vz4 + vz) * (vy5 + vy) - (vz + vz1) * (VY[mypz] + vy1)) +
+                   g.idz * ((vz5 + vz) * (vz5 + vz) - (vz + vz0) * (vz + vz0));
+        const float q = 0.25f;  // the dt factor is folded into adv_coef by the caller
+        Ax *= q; Ay *= q; Az *= q;
+        wx += a.adv_coef * Ax;
+        wy += a.adv_coef * Ay;
+        wz += a.adv_coef * Az;
+        if (a.adv_store) {
+          a.adv_store[i] = Ax;
+          a.adv_store[i + N] = Ay;
+          a.adv_store[i + 2 * (size_t)N] = Az;
+        }
+      }
+      if (a.adv_prev) {
+        wx += a.adv_prev_coef * a.adv_prev[i];
+        wy += a.adv_prev_coef * a.adv_prev[i + N];
+        wz += a.adv_prev_coef * a.adv_prev[i + 2 * (size_t)N];
+      }
+      if (a.force) {
+        wx += a.force_coef * a.force[i];
+        wy += a.force_coef * a.force[i + N];
+        wz += a.force_coef * a.force[i + 2 * (size_t)N];
+      }
+      a.W[i] = wx;
+      a.W[i + N] = wy;
+      a.W[i + 2 * (size_t)N] = wz;
+    }
+    __syncthreads();
+  }
+}
+
+// Discrete staggered divergence (diagnostic), src/kernelUpdateVIncompressible.cu invariant.
+__global__ void k_divergence(GridDev g, const float* __restrict__ v, float* __restrict__ div) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.N) return;
+  int x = i % g.mx, y = (i / g.mx) % g.my, z = i / (g.mx * g.my);
+  int xm = wrapi(x - 1, g.mx), ym = wrapi(y - 1, g.my), zm = wrapi(z - 1, g.mz);
+  const float* VX = v;
+  const float* VY = v + g.N;
+  const float* VZ = v + 2 * (size_t)g.N;
+  div[i] = (VX[i] - VX[xm + g.mx * (y + g.my * z)]) * g.idx + (VY[i] - VY[x + g.mx * (ym + g.my * z)]) * g.idy +
+           (VZ[i] - VZ[x + g.mx * (y + g.my * zm)]) * g.idz;
+}
+
+// Normals of the shared generator, for parity tests: out[4 i + k] = draw(index i)[k].
+__global__ void k_normals(Key key, int stream, int sub, int n, float* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v[4];
+  draw_normals4(key, (uint32_t)i, (uint32_t)stream, (uint32_t)sub, v);
+  for (int k = 0; k < 4; ++k) out[4 * i + k] = v[k];
+}
+
+__global__ void k_zero_f(float* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_normals, dim3((n + 255) / 256), dim3(256), 0, s, key, stream, sub, n, out);
+}
+
+void launch_zero_f(float* p, size_t n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+void launch_fluid_w(const FluidWArgs& a, hipStream_t s) {
+  dim3 grid((a.g.mx + TX - 1) / TX, (a.g.my + TY - 1) / TY, (a.g.mz + a.zchunk - 1) / a.zchunk);
+  hipLaunchKernelGGL(k_fluid_w, grid, dim3(256), 0, s, a);
+}
+
+void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t s) {
+  hipLaunchKernelGGL(k_divergence, dim3((g.N + 255) / 256), dim3(256), 0, s, g, v, div);
+}
+
+}  // namespace fhd

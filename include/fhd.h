@@ -79,6 +79,8 @@ typedef struct fhd_params {
 
 /* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
 void fhd_default_params(fhd_params* p);
+/* sizeof(fhd_params) as compiled into the library (binding sanity check; no fluam counterpart). */
+unsigned long fhd_params_size(void);
 
 /* ---------------- global (fluam-style) API ---------------- */
 int fhd_load_datamain(const char* path);          /* parse data.main into the global params */

@@ -84,25 +84,20 @@ static void uniforms4(uint64_t seed, int64_t step, int stream, int sub, uint32_t
   for (int k = 0; k < 4; ++k) out[k] = (double)(w[k] >> 8) * (1.0 / 16777216.0);
 }
 
-enum { STREAM_FLUID1 = 1, STREAM_FLUID2 = 2, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
+enum { STREAM_FLUID1 = 1, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
        STREAM_INIT_POS = 8 };
 
-// Six stress normals per cell (slot c*N + i of fluam's d_rand, src/kernelConstructW.cu:148-198):
-// components XX=0, XY=1, XZ=2, YY=3, YZ=4, ZZ=5. Sub-draw 0 gives components 0..3, sub-draw 1
-// gives 4,5.
-static void fluid_noise(uint64_t seed, int64_t step, int stream, int N, std::vector<double>& Z) {
+// Stress noise of one cell and step: 12 normals n_k, k = 0..11, where n_k is word k%4 of the
+// Philox draw with sub = k/4 (stream STREAM_FLUID1).  Sigma_1 = n_0..n_5 and Sigma_2 = n_6..n_11
+// in the component order of fluam's d_rand slots (src/kernelConstructW.cu:148-198):
+// XX=0, XY=1, XZ=2, YY=3, YZ=4, ZZ=5.  which = 0 returns Sigma_1, which = 1 returns Sigma_2.
+static void fluid_noise(uint64_t seed, int64_t step, int which, int N, std::vector<double>& Z) {
   Z.assign((size_t)6 * N, 0.0);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; ++i) {
-    double a[4], b[4];
-    orc_normals4(seed, step, stream, 0, (uint32_t)i, a);
-    orc_normals4(seed, step, stream, 1, (uint32_t)i, b);
-    Z[(size_t)0 * N + i] = a[0];
-    Z[(size_t)1 * N + i] = a[1];
-    Z[(size_t)2 * N + i] = a[2];
-    Z[(size_t)3 * N + i] = a[3];
-    Z[(size_t)4 * N + i] = b[0];
-    Z[(size_t)5 * N + i] = b[1];
+    double n[12];
+    for (int sub = 0; sub < 3; ++sub) orc_normals4(seed, step, STREAM_FLUID1, sub, (uint32_t)i, &n[4 * sub]);
+    for (int c = 0; c < 6; ++c) Z[(size_t)c * N + i] = n[6 * which + c];
   }
 }
 
@@ -569,6 +564,7 @@ static void wca_forces(const orc_ctx* c, const std::vector<double>* q, std::vect
             double r[3], r2 = 0;
             for (int a = 0; a < 3; ++a) { r[a] = wrapd(q[a][i] - q[a][jj], L[a]); r2 += r[a] * r[a]; }
             if (r2 >= rc2 || r2 == 0.0) continue;
+            r2 = std::max(r2, 1e-4 * sig2);  // force evaluated at sigma/100 below that separation
             double s2 = sig2 / r2, s6 = s2 * s2 * s2;
             double fr = 48.0 * eps * (s6 * s6 - 0.5 * s6) / r2;
             for (int a = 0; a < 3; ++a) f[a] += fr * r[a];
@@ -761,7 +757,7 @@ static void step_incompressible(orc_ctx* c) {
   const double fact1 = sqrt(4.0 * c->p.temperature * c->p.shearviscosity * c->p.dt / (dV * rho * rho));
   const double fact4 = sqrt(2.0 * c->p.temperature * c->p.shearviscosity * c->p.dt / (dV * rho * rho));
   std::vector<double> Z;
-  fluid_noise(c->p.seed, c->step, STREAM_FLUID1, c->N, Z);
+  fluid_noise(c->p.seed, c->step, 0, c->N, Z);
   std::vector<double> W[3], A[3], vt[3], vstar[3];
   // first substep: W = v + lap(v) - A(v) + D.Sigma
   lap_term(c, c->v, W);
@@ -809,8 +805,8 @@ static void step_stokes(orc_ctx* c) {
   const double fact4 = sqrt(2.0 * c->p.temperature * c->p.shearviscosity / (dV * dt));
   const double fact2 = sqrt(c->p.temperature * c->p.extra_mobility * dt);
   std::vector<double> Z1, Z2;
-  fluid_noise(c->p.seed, c->step, STREAM_FLUID1, N, Z1);
-  fluid_noise(c->p.seed, c->step, STREAM_FLUID2, N, Z2);
+  fluid_noise(c->p.seed, c->step, 0, N, Z1);
+  fluid_noise(c->p.seed, c->step, 1, N, Z2);
   std::vector<double> Wr[3], P1[3], P2[3];
   for (int a = 0; a < 3; ++a) { Wr[a].resize(np); P1[a].resize(np); P2[a].resize(np); }
   for (int i = 0; i < np; ++i) {
@@ -908,7 +904,7 @@ static void step_qnb(orc_ctx* c) {
   const double fact1 = sqrt(4.0 * c->p.temperature * c->p.shearviscosity * dt / (dV * rho * rho));
   const double fact4 = sqrt(2.0 * c->p.temperature * c->p.shearviscosity * dt / (dV * rho * rho));
   std::vector<double> Z;
-  fluid_noise(c->p.seed, c->step, STREAM_FLUID1, N, Z);
+  fluid_noise(c->p.seed, c->step, 0, N, Z);
   std::vector<double> qh[3], f[3], W[3], A[3], vt[3], vh[3];
   // q^{n+1/2} = q^n + dt/2 J(q^n) v^n (src/quasiNeutrallyBuoyantFunctions.cu:2618-2624)
   qnb_half_positions(c, c->v, 0.5, c->q, c->q, qh);
@@ -1011,9 +1007,9 @@ extern "C" int orc_noise_divergence(orc_ctx* c, const double* Z6, double fact1, 
   pack3(c, O, out);
   return 1;
 }
-extern "C" int orc_fluid_noise(uint64_t seed, int64_t step, int stream, int N, double* Z6) {
+extern "C" int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6) {
   std::vector<double> Z;
-  fluid_noise(seed, step, stream, N, Z);
+  fluid_noise(seed, step, which, N, Z);
   memcpy(Z6, Z.data(), sizeof(double) * 6 * (size_t)N);
   return 1;
 }

