@@ -12,73 +12,13 @@
 //   B  z-solve    : one workgroup per (ky, tile of kx) for all three components: FFT along z,
 //                   staggering shifts + projection + viscous / Stokes operator, inverse FFT.
 //   C  xy-inverse : inverse of A, with an epilogue that blends the result with another field.
-// Half-spectrum layout in HBM: spec[c][z][ky][kx], kx in [0, mx/2], row pitch P = mx/2 + 1.
+// Half-spectrum layout in HBM: spec[c][ky][z][kx], kx in [0, mx/2], row pitch P = mx/2 + 1, so
+// a pass-B tile (fixed ky, a run of kx, every z) is one compact block of rows.
 #include "fhd_device.h"
 #include "fhd_kernels.h"
+#include "fft_common.h"
 
 namespace fhd {
-
-// Complex helpers.
-FHD_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-FHD_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-FHD_DEV float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-FHD_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-FHD_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
-// a * (s i), s = +-1
-FHD_DEV float2 cmul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
-
-// In-register DFTs with kernel e^{s 2 pi i / R}.
-template <int R> struct Dft;
-template <> struct Dft<2> {
-  static FHD_DEV void run(float2* v, float) {
-    float2 a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  }
-};
-template <> struct Dft<3> {
-  static FHD_DEV void run(float2* v, float s) {
-    const float c3 = 0.86602540378443864676f;
-    float2 t = cadd(v[1], v[2]);
-    float2 u = csub(v[1], v[2]);
-    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-    float2 r = cmul_si(cscale(u, c3), s);
-    v[0] = cadd(v[0], t);
-    v[1] = cadd(m, r);
-    v[2] = csub(m, r);
-  }
-};
-template <> struct Dft<4> {
-  static FHD_DEV void run(float2* v, float s) {
-    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    float2 t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), s);
-    v[0] = cadd(t0, t2);
-    v[2] = csub(t0, t2);
-    v[1] = cadd(t1, t3);
-    v[3] = csub(t1, t3);
-  }
-};
-template <> struct Dft<8> {
-  static FHD_DEV void run(float2* v, float s) {
-    float2 e[4] = {v[0], v[2], v[4], v[6]};
-    float2 o[4] = {v[1], v[3], v[5], v[7]};
-    Dft<4>::run(e, s);
-    Dft<4>::run(o, s);
-    const float h = 0.70710678118654752440f;
-    // w^k = e^{s 2 pi i k / 8}
-    float2 o1 = cmul(o[1], make_float2(h, s * h));
-    float2 o2 = cmul_si(o[2], s);
-    float2 o3 = cmul(o[3], make_float2(-h, s * h));
-    v[0] = cadd(e[0], o[0]);
-    v[4] = csub(e[0], o[0]);
-    v[1] = cadd(e[1], o1);
-    v[5] = csub(e[1], o1);
-    v[2] = cadd(e[2], o2);
-    v[6] = csub(e[2], o2);
-    v[3] = cadd(e[3], o3);
-    v[7] = csub(e[3], o3);
-  }
-};
 
 // Lines in LDS: line l starts at (l / L1) * S1 + (l % L1) * S0, element stride es.
 struct LineMap {
@@ -198,8 +138,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_xy_fwd(FftArgs a, const float*
   // y: columns kx = 0..P-1
   LineMap cols = {P, P, 0, 1, P};
   lds_fft(buf, cols, my, a.plan_y, a.nst_y, twy, 1, -1.0f);
-  float2* dst = spec + ((size_t)c * mz + z) * (size_t)my * P;
-  for (int e = threadIdx.x; e < P * my; e += blockDim.x) dst[e] = buf[e];
+  for (int e = threadIdx.x; e < P * my; e += blockDim.x) {
+    int ky = e / P, kx = e - ky * P;
+    spec[(((size_t)c * my + ky) * mz + z) * P + kx] = buf[e];
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -223,12 +165,22 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_solve(FftArgs a, float2* __r
     int c = e / (mz * T), r = e - c * mz * T, z = r / T, t = r - z * T;
     int kx = kx0 + t;
     float2 val = make_float2(0.0f, 0.0f);
-    if (kx < P) val = spec[(((size_t)c * mz + z) * my + ky) * P + kx];
+    if (kx < P) val = spec[(((size_t)c * my + ky) * mz + z) * P + kx];
     buf[e] = val;
   }
   __syncthreads();
   LineMap zl = {3 * T, T, mz * T, 1, T};
-  lds_fft(buf, zl, mz, a.plan_z, a.nst_z, twz, 1, -1.0f);
+  if (a.zmode != 2) lds_fft(buf, zl, mz, a.plan_z, a.nst_z, twz, 1, -1.0f);
+  if (a.zmode != 0) {
+    // transform-only modes (building blocks for the parity tests): no operator, no scaling
+    if (a.zmode == 2) lds_fft(buf, zl, mz, a.plan_z, a.nst_z, twz, 1, 1.0f);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      int c = e / (mz * T), r = e - c * mz * T, z = r / T, t = r - z * T;
+      int kx = kx0 + t;
+      if (kx < P) spec[(((size_t)c * my + ky) * mz + z) * P + kx] = buf[e];
+    }
+    return;
+  }
   // Fourier-space operator in the staggered (shifted) basis; 1/N normalisation folded in.
   const float gy = a.gy[ky];
   const float2 ey = a.ey[ky];
@@ -268,7 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_solve(FftArgs a, float2* __r
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int c = e / (mz * T), r = e - c * mz * T, z = r / T, t = r - z * T;
     int kx = kx0 + t;
-    if (kx < P) spec[(((size_t)c * mz + z) * my + ky) * P + kx] = buf[e];
+    if (kx < P) spec[(((size_t)c * my + ky) * mz + z) * P + kx] = buf[e];
   }
 }
 
@@ -285,8 +237,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_xy_inv(FftArgs a, const float2
   const int mx = a.g.mx, my = a.g.my, mz = a.g.mz, P = a.g.P, h = mx / 2;
   for (int i = threadIdx.x; i < mx; i += blockDim.x) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < my; i += blockDim.x) twy[i] = a.twy[i];
-  const float2* src = spec + ((size_t)c * mz + z) * (size_t)my * P;
-  for (int e = threadIdx.x; e < P * my; e += blockDim.x) buf[e] = src[e];
+  for (int e = threadIdx.x; e < P * my; e += blockDim.x) {
+    int ky = e / P, kx = e - ky * P;
+    buf[e] = spec[(((size_t)c * my + ky) * mz + z) * P + kx];
+  }
   __syncthreads();
   LineMap cols = {P, P, 0, 1, P};
   lds_fft(buf, cols, my, a.plan_y, a.nst_y, twy, 1, 1.0f);
@@ -321,13 +275,16 @@ __global__ __launch_bounds__(kThreads) void k_fft_xy_inv(FftArgs a, const float2
 }
 
 void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s) {
+  if (!a.generic && fast_fft_xy_fwd(a, in, spec, s)) return;
   hipLaunchKernelGGL(k_fft_xy_fwd, dim3(a.g.mz, 3), dim3(kThreads), 0, s, a, in, spec);
 }
 void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s) {
+  if (!a.generic && fast_fft_z_solve(a, spec, s)) return;
   hipLaunchKernelGGL(k_fft_z_solve, dim3((a.g.P + a.ztile - 1) / a.ztile, a.g.my), dim3(kThreads), 0, s, a, spec);
 }
 void launch_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha, float beta,
                        hipStream_t s) {
+  if (!a.generic && fast_fft_xy_inv(a, spec, out, aux, alpha, beta, s)) return;
   hipLaunchKernelGGL(k_fft_xy_inv, dim3(a.g.mz, 3), dim3(kThreads), 0, s, a, spec, out, aux, alpha, beta);
 }
 

@@ -1,0 +1,78 @@
+// fft_common.h -- complex arithmetic and in-register DFT butterflies shared by the generic and
+// the size-specialised Stockham FFT kernels.
+#pragma once
+#include "fhd_device.h"
+
+namespace fhd {
+
+FHD_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+FHD_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+FHD_DEV float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+FHD_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+FHD_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// a * (s i), s = +-1
+FHD_DEV float2 cmul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// In-register DFTs with kernel e^{s 2 pi i / R}.
+template <int R> struct Dft;
+template <> struct Dft<2> {
+  static FHD_DEV void run(float2* v, float) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <> struct Dft<3> {
+  static FHD_DEV void run(float2* v, float s) {
+    const float c3 = 0.86602540378443864676f;
+    float2 t = cadd(v[1], v[2]);
+    float2 u = csub(v[1], v[2]);
+    float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
+    float2 r = cmul_si(cscale(u, c3), s);
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+template <> struct Dft<4> {
+  static FHD_DEV void run(float2* v, float s) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), s);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+template <> struct Dft<8> {
+  static FHD_DEV void run(float2* v, float s) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4>::run(e, s);
+    Dft<4>::run(o, s);
+    const float h = 0.70710678118654752440f;
+    // w^k = e^{s 2 pi i k / 8}
+    float2 o1 = cmul(o[1], make_float2(h, s * h));
+    float2 o2 = cmul_si(o[2], s);
+    float2 o3 = cmul(o[3], make_float2(-h, s * h));
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
+  }
+};
+
+// Radix plan: 8s first, then 4, 2, 3 (the same order the host uses for the generic kernels).
+constexpr int plan_radix(int n, int st) {
+  for (int i = 0;; ++i) {
+    int r = (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : 1;
+    if (i == st || r == 1) return r;
+    n /= r;
+  }
+}
+
+}  // namespace fhd

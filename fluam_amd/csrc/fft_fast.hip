@@ -1,0 +1,307 @@
+// fft_fast.hip -- size-specialised versions of the three FFT passes (see fft.hip for the
+// algorithm).  Transform lengths, radix plans and LDS extents are template parameters, so every
+// index division in the Stockham stages is by a constant and the register arrays are sized for
+// the actual plane instead of the worst case.  Instantiated for cubic planes
+// m x m, m in {16, 24, 32, 48, 64, 96, 128} and pencils of those lengths; other shapes use the
+// generic kernels, which share the spectrum layout spec[c][ky][z][kx].
+#include "fhd_device.h"
+#include "fhd_kernels.h"
+#include "fft_common.h"
+
+namespace fhd {
+
+// One radix-R Stockham stage, lines l = 0..lines-1 at base l * LS, element stride ES.
+template <int N, int R, int NS, int ES, int NT, int MAXL, int TWM>
+FHD_DEV void fstage(float2* buf, int lines, int LS, const float2* tw, float s) {
+  constexpr int NBL = N / R;
+  constexpr int KMAX = (MAXL * NBL + NT - 1) / NT;
+  constexpr int STRIDE = N / (NS * R);
+  const int total = lines * NBL;
+  float2 v[KMAX][R];
+  int wb[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int b = threadIdx.x + k * NT;
+    wb[k] = -1;
+    if (b < total) {
+      const int l = b / NBL;
+      const int j = b - l * NBL;
+      const int base = l * LS;
+      const int kk = j % NS;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        float2 x = buf[base + (j + r * NBL) * ES];
+        if (NS > 1 && r > 0) {
+          float2 w = tw[(kk * r * STRIDE) * TWM];
+          w.y = s > 0.0f ? -w.y : w.y;
+          x = cmul(x, w);
+        }
+        v[k][r] = x;
+      }
+      Dft<R>::run(v[k], s);
+      wb[k] = base + ((j / NS) * NS * R + kk) * ES;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k)
+    if (wb[k] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[wb[k] + r * NS * ES] = v[k][r];
+    }
+  __syncthreads();
+}
+
+template <int N, int ST, int NS, int ES, int NT, int MAXL, int TWM>
+struct FStages {
+  static FHD_DEV void run(float2* buf, int lines, int LS, const float2* tw, float s) {
+    constexpr int R = plan_radix(N, ST);
+    if constexpr (R > 1) {
+      fstage<N, R, NS, ES, NT, MAXL, TWM>(buf, lines, LS, tw, s);
+      FStages<N, ST + 1, NS * R, ES, NT, MAXL, TWM>::run(buf, lines, LS, tw, s);
+    }
+  }
+};
+
+template <int N, int ES, int NT, int MAXL, int TWM>
+FHD_DEV void ffft(float2* buf, int lines, int LS, const float2* tw, float s) {
+  FStages<N, 0, 1, ES, NT, MAXL, TWM>::run(buf, lines, LS, tw, s);
+}
+
+// ---------------- pass A: real plane -> half spectrum (x) -> FFT along y ----------------
+template <int MX, int MY, int NT>
+__global__ __launch_bounds__(NT) void k_fftA(FftArgs a, const float* __restrict__ in, float2* __restrict__ spec) {
+  constexpr int H = MX / 2, P = H + 1;
+  __shared__ float2 buf[MY * P];
+  __shared__ float2 twx[MX];
+  __shared__ float2 twy[MY];
+  const int z = blockIdx.x, c = blockIdx.y, mz = a.g.mz;
+  for (int i = threadIdx.x; i < MX; i += NT) twx[i] = a.twx[i];
+  for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
+  const float4* src = reinterpret_cast<const float4*>(in + ((size_t)c * mz + z) * (size_t)(MX * MY));
+  constexpr int Q = MX / 4;
+  for (int e = threadIdx.x; e < MY * Q; e += NT) {
+    const int y = e / Q, x4 = e - y * Q;
+    const float4 v = src[e];
+    buf[y * P + 2 * x4] = make_float2(v.x, v.y);
+    buf[y * P + 2 * x4 + 1] = make_float2(v.z, v.w);
+  }
+  __syncthreads();
+  ffft<H, 1, NT, MY, 2>(buf, MY, P, twx, -1.0f);
+  constexpr int NP = H / 2 + 1;
+  for (int e = threadIdx.x; e < MY * NP; e += NT) {
+    const int y = e / NP, k = e - y * NP;
+    float2* row = buf + y * P;
+    const float2 A = row[k];
+    const float2 B = cconj(row[k == 0 ? 0 : H - k]);
+    const float2 E = cscale(cadd(A, B), 0.5f);
+    const float2 D = csub(A, B);
+    const float2 O = make_float2(0.5f * D.y, -0.5f * D.x);
+    const float2 wO = cmul(twx[k], O);
+    if (k == 0) {
+      row[0] = make_float2(E.x + O.x, 0.0f);
+      row[H] = make_float2(E.x - O.x, 0.0f);
+    } else {
+      row[k] = cadd(E, wO);
+      if (H - k != k) row[H - k] = cconj(csub(E, wO));
+    }
+  }
+  __syncthreads();
+  ffft<MY, P, NT, P, 1>(buf, P, 1, twy, -1.0f);
+  float2* dst = spec + (size_t)c * MY * mz * P + (size_t)z * P;
+  for (int e = threadIdx.x; e < MY * P; e += NT) {
+    const int ky = e / P, kx = e - ky * P;
+    dst[(size_t)ky * mz * P + kx] = buf[e];
+  }
+}
+
+// ---------------- pass C: FFT along y (inverse) -> C2R along x -> out = alpha r + beta aux ----
+template <int MX, int MY, int NT>
+__global__ __launch_bounds__(NT) void k_fftC(FftArgs a, const float2* __restrict__ spec, float* __restrict__ out,
+                                             const float* __restrict__ aux, float alpha, float beta) {
+  constexpr int H = MX / 2, P = H + 1;
+  __shared__ float2 buf[MY * P];
+  __shared__ float2 twx[MX];
+  __shared__ float2 twy[MY];
+  const int z = blockIdx.x, c = blockIdx.y, mz = a.g.mz;
+  for (int i = threadIdx.x; i < MX; i += NT) twx[i] = a.twx[i];
+  for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
+  const float2* src = spec + (size_t)c * MY * mz * P + (size_t)z * P;
+  for (int e = threadIdx.x; e < MY * P; e += NT) {
+    const int ky = e / P, kx = e - ky * P;
+    buf[e] = src[(size_t)ky * mz * P + kx];
+  }
+  __syncthreads();
+  ffft<MY, P, NT, P, 1>(buf, P, 1, twy, 1.0f);
+  constexpr int NP = H / 2 + 1;
+  for (int e = threadIdx.x; e < MY * NP; e += NT) {
+    const int y = e / NP, k = e - y * NP;
+    float2* row = buf + y * P;
+    const float2 Xk = row[k], Xh = row[H - k];
+    const float2 wk = cconj(twx[k]);
+    const float2 B = cconj(Xh);
+    const float2 Zk = cadd(cadd(Xk, B), cmul_si(cmul(csub(Xk, B), wk), 1.0f));
+    if (k != 0 && H - k != k) {
+      const float2 B2 = cconj(Xk);
+      const float2 wk2 = make_float2(-wk.x, wk.y);
+      row[H - k] = cadd(cadd(Xh, B2), cmul_si(cmul(csub(Xh, B2), wk2), 1.0f));
+    }
+    row[k] = Zk;
+  }
+  __syncthreads();
+  ffft<H, 1, NT, MY, 2>(buf, MY, P, twx, 1.0f);
+  const size_t off = ((size_t)c * mz + z) * (size_t)(MX * MY);
+  float4* dst = reinterpret_cast<float4*>(out + off);
+  const float4* ax = aux ? reinterpret_cast<const float4*>(aux + off) : nullptr;
+  constexpr int Q = MX / 4;
+  for (int e = threadIdx.x; e < MY * Q; e += NT) {
+    const int y = e / Q, x4 = e - y * Q;
+    const float2 p0 = buf[y * P + 2 * x4], p1 = buf[y * P + 2 * x4 + 1];
+    float4 r = make_float4(alpha * p0.x, alpha * p0.y, alpha * p1.x, alpha * p1.y);
+    if (ax) {
+      const float4 b = ax[e];
+      r.x += beta * b.x;
+      r.y += beta * b.y;
+      r.z += beta * b.z;
+      r.w += beta * b.w;
+    }
+    dst[e] = r;
+  }
+}
+
+// ---------------- pass B: FFT along z, operator, inverse FFT along z ----------------
+// Workgroup (tile i, ky): kx columns i*T .. i*T+T-1, plus column ntile*T + i when P is not a
+// multiple of T (the Nyquist column for P = mx/2 + 1).  LDS lines: (c, t) at (c*(T+1)+t)*(MZ+1).
+template <int MZ, int NT>
+__global__ __launch_bounds__(NT) void k_fftB(FftArgs a, float2* __restrict__ spec, int T) {
+  constexpr int TM = 17;  // T <= 16 plus one extra column
+  constexpr int LS = MZ + 1;
+  __shared__ float2 buf[3 * TM * LS];
+  __shared__ float2 twz[MZ], ezs[MZ];
+  __shared__ float gzs[MZ];
+  const int P = a.g.P, my = a.g.my;
+  const int ky = blockIdx.y, tile = blockIdx.x;
+  const int ntile = P / T, rem = P - ntile * T;
+  const int TC = T + (tile < rem ? 1 : 0);
+  for (int i = threadIdx.x; i < MZ; i += NT) {
+    twz[i] = a.twz[i];
+    ezs[i] = a.ez[i];
+    gzs[i] = a.gz[i];
+  }
+  const size_t cstride = (size_t)my * MZ * P;
+  float2* base = spec + (size_t)ky * MZ * P;
+  const int tot = 3 * MZ * TC;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
+    const int kx = t < T ? tile * T + t : ntile * T + tile;
+    buf[(c * TM + t) * LS + z] = base[c * cstride + (size_t)z * P + kx];
+  }
+  __syncthreads();
+  const int lines = 3 * TM;
+  if (a.zmode != 2) ffft<MZ, 1, NT, 3 * TM, 1>(buf, lines, LS, twz, -1.0f);
+  if (a.zmode == 0) {
+    const float gy = a.gy[ky];
+    const float2 ey = a.ey[ky];
+    for (int e = threadIdx.x; e < MZ * TC; e += NT) {
+      const int t = e / MZ, z = e - t * MZ;
+      const int kx = t < T ? tile * T + t : ntile * T + tile;
+      const float gx = a.gx[kx];
+      const float2 ex = a.ex[kx];
+      const float gz = gzs[z];
+      const float2 ez = ezs[z];
+      const int i0 = (0 * TM + t) * LS + z, i1 = (1 * TM + t) * LS + z, i2 = (2 * TM + t) * LS + z;
+      const float2 w0 = cmul(buf[i0], cconj(ex));
+      const float2 w1 = cmul(buf[i1], cconj(ey));
+      const float2 w2 = cmul(buf[i2], cconj(ez));
+      const float L = -(gx * gx + gy * gy + gz * gz);
+      float2 o0, o1, o2;
+      if (kx == 0 && ky == 0 && z == 0) {
+        const float keep = a.stokes ? 0.0f : a.inv_n;
+        o0 = cscale(w0, keep);
+        o1 = cscale(w1, keep);
+        o2 = cscale(w2, keep);
+      } else {
+        const float2 GW = make_float2(gx * w0.x + gy * w1.x + gz * w2.x, gx * w0.y + gy * w1.y + gz * w2.y);
+        const float den = a.stokes ? (-a.eta * L) : (1.0f - a.cn_coef * L);
+        const float sc = a.inv_n / den;
+        const float iL = 1.0f / L;
+        o0 = cscale(make_float2(w0.x + gx * GW.x * iL, w0.y + gx * GW.y * iL), sc);
+        o1 = cscale(make_float2(w1.x + gy * GW.x * iL, w1.y + gy * GW.y * iL), sc);
+        o2 = cscale(make_float2(w2.x + gz * GW.x * iL, w2.y + gz * GW.y * iL), sc);
+      }
+      buf[i0] = cmul(o0, ex);
+      buf[i1] = cmul(o1, ey);
+      buf[i2] = cmul(o2, ez);
+    }
+    __syncthreads();
+  }
+  if (a.zmode != 1) ffft<MZ, 1, NT, 3 * TM, 1>(buf, lines, LS, twz, 1.0f);
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
+    const int kx = t < T ? tile * T + t : ntile * T + tile;
+    base[c * cstride + (size_t)z * P + kx] = buf[(c * TM + t) * LS + z];
+  }
+}
+
+// ---------------- dispatch ----------------
+template <int M>
+static bool launch_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
+                      float alpha, float beta, hipStream_t s) {
+  constexpr int NT = M >= 64 ? 512 : 256;
+  if (which == 0) hipLaunchKernelGGL((k_fftA<M, M, NT>), dim3(a.g.mz, 3), dim3(NT), 0, s, a, in, spec);
+  else hipLaunchKernelGGL((k_fftC<M, M, NT>), dim3(a.g.mz, 3), dim3(NT), 0, s, a, spec, out, aux, alpha, beta);
+  return true;
+}
+
+static bool fast_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
+                    float alpha, float beta, hipStream_t s) {
+  if (a.g.mx != a.g.my) return false;
+  switch (a.g.mx) {
+    case 16: return launch_AC<16>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 24: return launch_AC<24>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 32: return launch_AC<32>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 48: return launch_AC<48>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 64: return launch_AC<64>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 96: return launch_AC<96>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 128: return launch_AC<128>(a, which, in, spec, out, aux, alpha, beta, s);
+    default: return false;
+  }
+}
+
+bool fast_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s) {
+  return fast_AC(a, 0, in, spec, nullptr, nullptr, 0.0f, 0.0f, s);
+}
+bool fast_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha, float beta,
+                     hipStream_t s) {
+  return fast_AC(a, 1, nullptr, const_cast<float2*>(spec), out, aux, alpha, beta, s);
+}
+
+// Largest column-tile width T <= 16 whose leftover columns (P mod T) fit one per tile.
+static int column_tile(int P) {
+  for (int T = 16; T > 1; --T)
+    if (T <= P && P % T <= P / T) return T;
+  return 1;
+}
+
+template <int MZ>
+static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
+  const int P = a.g.P;
+  const int T = column_tile(P);
+  hipLaunchKernelGGL((k_fftB<MZ, 256>), dim3(P / T, a.g.my), dim3(256), 0, s, a, spec, T);
+  return true;
+}
+
+bool fast_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s) {
+  switch (a.g.mz) {
+    case 16: return launch_B2<16>(a, spec, s);
+    case 24: return launch_B2<24>(a, spec, s);
+    case 32: return launch_B2<32>(a, spec, s);
+    case 48: return launch_B2<48>(a, spec, s);
+    case 64: return launch_B2<64>(a, spec, s);
+    case 96: return launch_B2<96>(a, spec, s);
+    case 128: return launch_B2<128>(a, spec, s);
+    default: return false;
+  }
+}
+
+}  // namespace fhd

@@ -219,6 +219,9 @@ void build_tables(fhd_ctx* c) {
   make_plan(my, &a.plan_y, &a.nst_y);
   make_plan(mz, &a.plan_z, &a.nst_z);
   a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
+  a.zmode = 0;
+  const char* gen = getenv("FHD_FFT_GENERIC");  // 1: use the runtime-size FFT kernels everywhere
+  a.generic = (gen && gen[0] == '1') ? 1 : 0;
   a.stokes = 0;
   a.eta = (float)c->p.shearviscosity;
   a.cn_coef = 0.0f;
@@ -262,6 +265,11 @@ void fft_solve(fhd_ctx* c, int stokes, float cn_coef) {
   launch_fft_z_solve(a, c->spec, c->s);
   mark(c, "fft_z_solve");
 }
+void fft_ztransform(fhd_ctx* c, int zmode) {
+  FftArgs a = c->fa;
+  a.zmode = zmode;
+  launch_fft_z_solve(a, c->spec, c->s);
+}
 void fft_inverse(fhd_ctx* c, float* out, const float* aux, float alpha, float beta) {
   launch_fft_xy_inv(c->fa, c->spec, out, aux, alpha, beta, c->s);
   mark(c, "fft_xy_inv");
@@ -290,7 +298,8 @@ void sort_set(fhd_ctx* c, int k) {
   launch_bin(c->g, c->t, c->cell[k], c->count, c->slot, c->np, c->s);
   launch_scan(c->t, c->count, c->start[k], c->s);
   launch_scatter(c->g, c->t, c->cell[k], c->slot, c->start[k], c->tmp, c->np, c->s);
-  launch_tile_sort(c->t, c->start[k], c->tmp, c->cell[k], c->off[k], c->sid[k], c->scell[k], c->soff[k], c->s);
+  launch_tile_sort(c->g, c->t, c->start[k], c->tmp, c->cell[k], c->off[k], c->sid[k], c->scell[k], c->soff[k],
+                   c->s);
   mark(c, "sort");
 }
 SortedSet sset(fhd_ctx* c, int k) {
@@ -1116,11 +1125,20 @@ int fhd_fft_r2c(int mx, int my, int mz, const double* in, double* out_spec) {
     for (size_t i = 0; i < (size_t)c->g.N; ++i) h[i] = (float)in[i];
     HIPCHK(hipMemcpy(c->W, h.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
     fft_forward(c, c->W);
+    fft_ztransform(c, 1);
     const size_t ns = (size_t)mz * my * c->g.P;
     std::vector<float2> sp(ns);
     HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipMemcpy(sp.data(), c->spec, ns * sizeof(float2), hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < ns; ++i) { out_spec[2 * i] = sp[i].x; out_spec[2 * i + 1] = sp[i].y; }
+    // device layout [ky][z][kx] -> host layout [z][ky][kx]
+    const int P = c->g.P;
+    for (int z = 0; z < mz; ++z)
+      for (int ky = 0; ky < my; ++ky)
+        for (int kx = 0; kx < P; ++kx) {
+          size_t h = ((size_t)z * my + ky) * P + kx, d = ((size_t)ky * mz + z) * P + kx;
+          out_spec[2 * h] = sp[d].x;
+          out_spec[2 * h + 1] = sp[d].y;
+        }
   } catch (const std::exception& e) {
     g_last_error = e.what();
     ok = 0;
@@ -1142,8 +1160,15 @@ int fhd_fft_c2r(int mx, int my, int mz, const double* in_spec, double* out) {
   try {
     const size_t ns = (size_t)mz * my * c->g.P;
     std::vector<float2> sp(3 * ns, make_float2(0.0f, 0.0f));
-    for (size_t i = 0; i < ns; ++i) sp[i] = make_float2((float)in_spec[2 * i], (float)in_spec[2 * i + 1]);
+    const int P = c->g.P;
+    for (int z = 0; z < mz; ++z)
+      for (int ky = 0; ky < my; ++ky)
+        for (int kx = 0; kx < P; ++kx) {
+          size_t h = ((size_t)z * my + ky) * P + kx, d = ((size_t)ky * mz + z) * P + kx;
+          sp[d] = make_float2((float)in_spec[2 * h], (float)in_spec[2 * h + 1]);
+        }
     HIPCHK(hipMemcpy(c->spec, sp.data(), 3 * ns * sizeof(float2), hipMemcpyHostToDevice));
+    fft_ztransform(c, 2);
     fft_inverse(c, c->W, nullptr, 1.0f, 0.0f);
     std::vector<float> h(c->g.N);
     HIPCHK(hipStreamSynchronize(c->s));

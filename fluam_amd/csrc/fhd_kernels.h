@@ -16,6 +16,8 @@ struct FftArgs {
   uint32_t plan_h, plan_y, plan_z;  // radices, 4 bits per stage
   int nst_h, nst_y, nst_z;
   int ztile;                      // kx columns per pass-B workgroup
+  int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
+  int generic;                    // 1: force the generic (runtime-size) kernels
   int stokes;                     // 1: v = P W / (eta |g|^2), mode 0 -> 0
   float eta;
   float cn_coef;                  // incompressible: 1 - cn_coef * L (0: projection only)
@@ -105,6 +107,12 @@ struct WcaArgs {
   int np;
 };
 
+// Size-specialised FFT passes (fft_fast.hip); return false when the shape has no specialisation.
+bool fast_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s);
+bool fast_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s);
+bool fast_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha, float beta,
+                     hipStream_t s);
+
 // Launch helpers (defined in the .hip files).
 void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s);
 void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s);
@@ -120,8 +128,8 @@ void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, 
 void launch_scan(const Tiles& t, int* count, int* start, hipStream_t s);
 void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, const int* start,
                     int* tmp, int np, hipStream_t s);
-void launch_tile_sort(const Tiles& t, const int* start, const int* tmp, const int4* cell, const float4* off,
-                      int* sid, int4* scell, float4* soff, hipStream_t s);
+void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
+                      const float4* off, int* sid, int4* scell, float4* soff, hipStream_t s);
 void launch_wca(const WcaArgs& a, hipStream_t s);
 void launch_spread(const SpreadArgs& a, hipStream_t s);
 void launch_interp(const InterpArgs& a, hipStream_t s);

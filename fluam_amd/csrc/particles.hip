@@ -67,8 +67,10 @@ __global__ void k_scatter(GridDev g, Tiles t, const int4* __restrict__ cell, con
   tmp[start[tile_of(g, t, cell[i])] + slot[i]] = i;
 }
 
-// One wavefront per tile: rank each particle by its index within the tile, then gather.
-__global__ __launch_bounds__(256) void k_tile_sort(int ntiles, const int* __restrict__ start,
+// One wavefront per tile: rank each particle by its index within the tile, then gather.  The
+// sorted copies carry the cell wrapped into the box (0 <= c < m), which is all the spatial
+// passes need; the unwrapped cell stays in the per-particle arrays.
+__global__ __launch_bounds__(256) void k_tile_sort(GridDev g, int ntiles, const int* __restrict__ start,
                                                    const int* __restrict__ tmp, const int4* __restrict__ cell,
                                                    const float4* __restrict__ off, int* __restrict__ sid,
                                                    int4* __restrict__ scell, float4* __restrict__ soff) {
@@ -80,15 +82,18 @@ __global__ __launch_bounds__(256) void k_tile_sort(int ntiles, const int* __rest
     int id = tmp[s0 + e];
     int rank = 0;
     for (int f = 0; f < n; ++f) rank += (tmp[s0 + f] < id);
+    int4 c = cell[id];
     sid[s0 + rank] = id;
-    scell[s0 + rank] = cell[id];
+    scell[s0 + rank] = make_int4(wrapi(c.x, g.mx), wrapi(c.y, g.my), wrapi(c.z, g.mz), 0);
     soff[s0 + rank] = off[id];
   }
 }
 
+// d = difference of two wrapped cell coordinates, |d| < m: nearest periodic image.
 FHD_DEV int min_image(int d, int m) {
-  int r = wrapi(d, m);
-  return (2 * r > m) ? r - m : r;
+  if (2 * d > m) return d - m;
+  if (2 * d < -m) return d + m;
+  return d;
 }
 
 FHD_DEV void nb_range(int nt, int& lo, int& hi) {
@@ -105,11 +110,29 @@ __global__ __launch_bounds__(256) void k_wca(WcaArgs a) {
   const int id = a.set.id[s];
   const int4 ci = a.set.cell[s];
   const float4 di = a.set.off[s];
-  const int tx = wrapi(ci.x, g.mx) / t.tx, ty = wrapi(ci.y, g.my) / t.ty, tz = wrapi(ci.z, g.mz) / t.tz;
+  const int tx = ci.x / t.tx, ty = ci.y / t.ty, tz = ci.z / t.tz;
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
   nb_range(t.nty, loy, hiy);
   nb_range(t.ntz, loz, hiz);
+  // with three or more tiles along an axis, visit a neighbouring tile only when the particle
+  // lies within the cutoff of that face
+  const float rc = sqrtf(a.rc2);
+  if (t.ntx >= 3) {
+    const float lo = ((float)(ci.x - tx * t.tx) + 0.5f) * g.dx + di.x;
+    if (lo >= rc) lox = 0;
+    if ((float)t.tx * g.dx - lo >= rc) hix = 0;
+  }
+  if (t.nty >= 3) {
+    const float lo = ((float)(ci.y - ty * t.ty) + 0.5f) * g.dy + di.y;
+    if (lo >= rc) loy = 0;
+    if ((float)t.ty * g.dy - lo >= rc) hiy = 0;
+  }
+  if (t.ntz >= 3) {
+    const float lo = ((float)(ci.z - tz * t.tz) + 0.5f) * g.dz + di.z;
+    if (lo >= rc) loz = 0;
+    if ((float)t.tz * g.dz - lo >= rc) hiz = 0;
+  }
   float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   for (int dz = loz; dz <= hiz; ++dz)
     for (int dy = loy; dy <= hiy; ++dy)
@@ -168,37 +191,22 @@ FHD_DEV void source_weights(int4 c, float3 frac, SrcWeights& w, bool deriv) {
 
 constexpr int kTileMax = 512;  // cells per spreading tile (8 x 8 x 8)
 
-template <int NPTS>
-FHD_DEV void accumulate_source(unsigned long long* acc, const GridDev& g, const Tiles& t, int ox, int oy, int oz,
-                               const SrcWeights& w, float3 val, float scale, bool deriv) {
-  const float vals[3] = {val.x, val.y, val.z};
-#pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    float vc = vals[comp] * scale;
-    if (vc == 0.0f) continue;
-    const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
-    const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
-    const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
-    (void)deriv;
-    for (int kz = 0; kz < NPTS; ++kz) {
-      int lz = wrapi(wz.base + kz - oz, g.mz);
-      if (lz >= t.tz) continue;
-      float wzv = wz.w[kz] * vc;
-      for (int ky = 0; ky < NPTS; ++ky) {
-        int ly = wrapi(wy.base + ky - oy, g.my);
-        if (ly >= t.ty) continue;
-        float wyz = wy.w[ky] * wzv;
-#pragma unroll
-        for (int kx = 0; kx < NPTS; ++kx) {
-          int lx = wrapi(wx.base + kx - ox, g.mx);
-          if (lx >= t.tx) continue;
-          long long q = __float2ll_rn(wx.w[kx] * wyz);
-          atomicAdd(&acc[comp * kTileMax + lx + t.tx * (ly + t.ty * lz)], (unsigned long long)q);
-        }
-      }
-    }
-  }
+// Fold a node index offset into [0, m) (the offsets here stay within a few boxes of the tile).
+FHD_DEV int fold(int l, int m) {
+  while (l < 0) l += m;
+  while (l >= m) l -= m;
+  return l;
 }
+
+// Candidate source staged in LDS: per axis the centred and face weights with their node bases,
+// and the value (already multiplied by the fixed-point scale).
+struct Cand {
+  float w[2][3][4];   // [0: centred, 1: face][axis][k]
+  int base[2][3];
+  float val[3];
+};
+
+constexpr int kCandCap = 256;
 
 FHD_DEV float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 
@@ -213,6 +221,8 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   __shared__ int nb_start[2][27], nb_count[2][27], nb_pref[2][28];
   __shared__ int n_nb[2];
   __shared__ unsigned int maxbits;
+  __shared__ Cand cand[kCandCap];
+  __shared__ int ncand;
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tile = blockIdx.x;
@@ -278,55 +288,115 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     frexpf(M * (float)(8 * nsrc_total + 1), &e2);
     e2 = max(e2, -60);
     const float scale = ldexpf(1.0f, 62 - e2);
-    for (int gi = 0; gi < a.ngroups; ++gi) {
-      const SpreadGroup& G = a.grp[gi];
-      const int tot = nb_pref[gi][n_nb[gi]];
-      const int nsub = G.mode == SRC_RFD ? 2 : 1;
-      for (int e = threadIdx.x; e < tot * nsub; e += blockDim.x) {
-        int src = e / nsub, sub = e - src * nsub;
+    // Flattened source items over all groups: (group, neighbour-tile slot, sub-source).
+    int items[2];
+    for (int gi = 0; gi < 2; ++gi)
+      items[gi] = gi < a.ngroups ? nb_pref[gi][n_nb[gi]] * (a.grp[gi].mode == SRC_RFD ? 2 : 1) : 0;
+    const int nitems = items[0] + items[1];
+    for (int cursor = 0; cursor < nitems; cursor += blockDim.x) {
+      if (threadIdx.x == 0) ncand = 0;
+      __syncthreads();
+      // stage: every source whose stencil can reach this tile becomes a candidate
+      const int e = cursor + threadIdx.x;
+      if (e < nitems) {
+        const int gi = e < items[0] ? 0 : 1;
+        const int el = gi == 0 ? e : e - items[0];
+        const SpreadGroup& G = a.grp[gi];
+        const int nsub = G.mode == SRC_RFD ? 2 : 1;
+        const int src = el / nsub, sub = el - src * nsub;
         int k = 0;
         while (nb_pref[gi][k + 1] <= src) ++k;
-        int sidx = nb_start[gi][k] + (src - nb_pref[gi][k]);
-        int id = G.set.id[sidx];
+        const int sidx = nb_start[gi][k] + (src - nb_pref[gi][k]);
+        const int id = G.set.id[sidx];
         int4 c = G.set.cell[sidx];
         float4 d = G.set.off[sidx];
-        SrcWeights w;
         float3 val;
+        float nrm[4] = {0, 0, 0, 0};
+        int4 cw = c;  // cell the stencil is built on
+        float3 dw = make_float3(d.x, d.y, d.z);
         if (G.mode == SRC_VALUE) {
-          float4 v = G.val[id];
+          const float4 v = G.val[id];
           val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
-          source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
         } else if (G.mode == SRC_DRIFT) {
-          // kT d/dq_c [phi phi phi]: derivative weights on the component's own axis
-          source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, true);
           val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
         } else {
-          float n[4];
-          rfd_normal(a.key, id, n);
+          rfd_normal(a.key, id, nrm);
           const float sg = sub == 0 ? 1.0f : -1.0f;
-          val = make_float3(sg * G.coef * n[0], sg * G.coef * n[1], sg * G.coef * n[2]);
-          int4 cd = c;
-          float3 dd = make_float3(d.x + sg * G.disp * n[0], d.y + sg * G.disp * n[1], d.z + sg * G.disp * n[2]);
-          renorm_axis(cd.x, dd.x, g.dx, g.idx);
-          renorm_axis(cd.y, dd.y, g.dy, g.idy);
-          renorm_axis(cd.z, dd.z, g.dz, g.idz);
-          float3 fd = make_float3(dd.x * g.idx, dd.y * g.idy, dd.z * g.idz);
-          source_weights<NPTS>(cd, fd, w, false);
-          if (G.rfd_quirk) {
+          val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
+          dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
+          renorm_axis(cw.x, dw.x, g.dx, g.idx);
+          renorm_axis(cw.y, dw.y, g.dy, g.idy);
+          renorm_axis(cw.z, dw.z, g.dz, g.idz);
+        }
+        // every stencil of this source lies within cells cw-2 .. cw+2 on each axis: it reaches
+        // the tile when that 5-cell window (folded to the tile origin) starts inside the tile or
+        // wraps around onto its start
+        const int l0x = fold(cw.x - 2 - ox, g.mx), l0y = fold(cw.y - 2 - oy, g.my), l0z = fold(cw.z - 2 - oz, g.mz);
+        const bool hitx = t.ntx == 1 || l0x < t.tx || l0x + 4 >= g.mx;
+        const bool hity = t.nty == 1 || l0y < t.ty || l0y + 4 >= g.my;
+        const bool hitz = t.ntz == 1 || l0z < t.tz || l0z + 4 >= g.mz;
+        if (hitx && hity && hitz) {
+          SrcWeights w;
+          const float3 fr = make_float3(dw.x * g.idx, dw.y * g.idy, dw.z * g.idz);
+          source_weights<NPTS>(cw, fr, w, G.mode == SRC_DRIFT);
+          if (G.mode == SRC_RFD && G.rfd_quirk) {
             // nodes from the displaced position, weights at the undisplaced one
-            int cc[3] = {c.x, c.y, c.z};
-            float ff[3] = {d.x * g.idx, d.y * g.idy, d.z * g.idz};
+            const int cc[3] = {c.x, c.y, c.z};
+            const float ff[3] = {d.x * g.idx, d.y * g.idy, d.z * g.idz};
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
               axis_weights_at<NPTS>(w.cen[ax].base, cc[ax], ff[ax], 0, w.cen[ax]);
               axis_weights_at<NPTS>(w.stg[ax].base, cc[ax], ff[ax], 1, w.stg[ax]);
             }
           }
+          const int slot = atomicAdd(&ncand, 1);
+          Cand& C = cand[slot];
+#pragma unroll
+          for (int ax = 0; ax < 3; ++ax) {
+            C.base[0][ax] = w.cen[ax].base;
+            C.base[1][ax] = w.stg[ax].base;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              C.w[0][ax][q] = w.cen[ax].w[q];
+              C.w[1][ax][q] = w.stg[ax].w[q];
+            }
+          }
+          C.val[0] = val.x * scale;
+          C.val[1] = val.y * scale;
+          C.val[2] = val.z * scale;
         }
-        accumulate_source<NPTS>(acc, g, t, ox, oy, oz, w, val, scale, G.mode == SRC_DRIFT);
       }
+      __syncthreads();
+      // accumulate: one work item per (candidate, component, z-row, y-row) line of NPTS nodes
+      const int nc = ncand;
+      const int nlines = nc * 3 * NPTS * NPTS;
+      for (int li = threadIdx.x; li < nlines; li += blockDim.x) {
+        const int cidx = li / (3 * NPTS * NPTS);
+        const int r = li - cidx * (3 * NPTS * NPTS);
+        const int comp = r / (NPTS * NPTS);
+        const int r2 = r - comp * NPTS * NPTS;
+        const int kz = r2 / NPTS, ky = r2 - kz * NPTS;
+        const Cand& C = cand[cidx];
+        const float vc = C.val[comp];
+        if (vc == 0.0f) continue;
+        const int sx = comp == 0 ? 1 : 0, sy = comp == 1 ? 1 : 0, sz = comp == 2 ? 1 : 0;
+        const int lz = fold(C.base[sz][2] + kz - oz, g.mz);
+        if (lz >= t.tz) continue;
+        const int ly = fold(C.base[sy][1] + ky - oy, g.my);
+        if (ly >= t.ty) continue;
+        const float wyz = C.w[sy][1][ky] * C.w[sz][2][kz] * vc;
+        const int bx = C.base[sx][0];
+#pragma unroll
+        for (int kx = 0; kx < NPTS; ++kx) {
+          const int lx = fold(bx + kx - ox, g.mx);
+          if (lx < t.tx) {
+            const long long q = __float2ll_rn(C.w[sx][0][kx] * wyz);
+            atomicAdd(&acc[comp * kTileMax + lx + t.tx * (ly + t.ty * lz)], (unsigned long long)q);
+          }
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
     const float inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
     for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
       int comp = e / ncell, l = e - comp * ncell;
@@ -349,14 +419,14 @@ FHD_DEV float interp_comp(const float* __restrict__ f, const GridDev& g, const A
                           const AxisWeights& wy, const AxisWeights& wz) {
   float s = 0.0f;
   for (int kz = 0; kz < NPTS; ++kz) {
-    int z = wrapi(wz.base + kz, g.mz);
+    int z = fold(wz.base + kz, g.mz);
     float sy = 0.0f;
     for (int ky = 0; ky < NPTS; ++ky) {
-      int y = wrapi(wy.base + ky, g.my);
+      int y = fold(wy.base + ky, g.my);
       const float* row = f + (size_t)g.mx * (y + (size_t)g.my * z);
       float sx = 0.0f;
 #pragma unroll
-      for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[wrapi(wx.base + kx, g.mx)];
+      for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[fold(wx.base + kx, g.mx)];
       sy += wy.w[ky] * sx;
     }
     s += wz.w[kz] * sy;
@@ -445,9 +515,10 @@ void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const in
                     int np, hipStream_t s) {
   if (np > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, g, t, cell, slot, start, tmp, np);
 }
-void launch_tile_sort(const Tiles& t, const int* start, const int* tmp, const int4* cell, const float4* off, int* sid,
-                      int4* scell, float4* soff, hipStream_t s) {
-  hipLaunchKernelGGL(k_tile_sort, dim3(nblk(t.n, 4)), dim3(256), 0, s, t.n, start, tmp, cell, off, sid, scell, soff);
+void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
+                      const float4* off, int* sid, int4* scell, float4* soff, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_sort, dim3(nblk(t.n, 4)), dim3(256), 0, s, g, t.n, start, tmp, cell, off, sid, scell,
+                     soff);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(a.np, 256)), dim3(256), 0, s, a);

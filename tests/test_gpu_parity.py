@@ -123,12 +123,16 @@ def test_stokes_wca_forces_match():
     p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 24), np=200, kernel_points=4, temperature=1e-3,
                     dt=0.05, seed=23, initial_positions=0, **WCA)
     _announce("stokes_wca_forces", p)
-    # pairs at moderate overlap only: a jittered lattice
+    # 100 pairs at separations 0.9..1.12 sigma, pair centres on a coarse lattice
     g = F.Simulation(p)
     o = O.Oracle(p)
-    base = O.initial_positions(make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 24), np=200,
-                                           initial_positions=1)).reshape(3, -1)
-    q = (base + rng.uniform(-0.3, 0.3, base.shape)).ravel()
+    centres = np.array([[x, y, z] for x in (-6, -2, 2, 6) for y in (-6, -2, 2, 6)
+                        for z in (-10, -6, -2, 2, 6, 10)], dtype=float)[:100]
+    d = rng.standard_normal((100, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d *= rng.uniform(0.9, 1.12, (100, 1)) / 2
+    pts = np.concatenate([centres + d, centres - d])  # (200, 3)
+    q = pts.T.ravel()
     g.set_particles(q)
     o.set_particles(q)
     g.step(1)
