@@ -34,15 +34,15 @@ FHD_DEV u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 }
 
 // Box-Muller (1958) on 24-bit uniforms u1 in (0,1], u2 in [0,1): z0 = r cos(2 pi u2),
-// z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).  sincospi keeps the angle exact in turns.
+// z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).  The hardware transcendentals take the angle in
+// turns (v_sin_f32 / v_cos_f32 compute sin(2 pi x)) and log in base 2 (v_log_f32), so no range
+// reduction is needed: u2 already is the angle in turns.
 FHD_DEV float2 box_muller(uint32_t a, uint32_t b) {
   const float inv24 = 1.0f / 16777216.0f;
-  float u1 = (float)((a >> 8) + 1u) * inv24;
-  float u2 = (float)(b >> 8) * inv24;
-  float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincospif(2.0f * u2, &s, &c);
-  return make_float2(r * c, r * s);
+  const float u1 = (float)((a >> 8) + 1u) * inv24;
+  const float u2 = (float)(b >> 8) * inv24;
+  const float r = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2(u1)
+  return make_float2(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2));
 }
 
 struct Key { uint32_t k0, k1; uint32_t step_lo, step_hi; };

@@ -102,12 +102,27 @@ FHD_DEV void nb_range(int nt, int& lo, int& hi) {
 }
 
 // WCA: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for r < rc.
+FHD_DEV void wca_partial(const WcaArgs& a, int s, int q, float& fx, float& fy, float& fz);
+
+// Four lanes per particle: lane q takes every fourth candidate of each neighbouring tile and
+// the partial forces are combined by a fixed xor-shuffle tree, so the sum order is fixed.
 __global__ __launch_bounds__(256) void k_wca(WcaArgs a) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.np) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid >> 2, q = gid & 3;
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  if (s < a.np) wca_partial(a, s, q, fx, fy, fz);
+  fx += __shfl_xor(fx, 1);
+  fy += __shfl_xor(fy, 1);
+  fz += __shfl_xor(fz, 1);
+  fx += __shfl_xor(fx, 2);
+  fy += __shfl_xor(fy, 2);
+  fz += __shfl_xor(fz, 2);
+  if (s < a.np && q == 0) a.F[a.set.id[s]] = make_float4(fx, fy, fz, 0.0f);
+}
+
+FHD_DEV void wca_partial(const WcaArgs& a, int s, int q, float& fx, float& fy, float& fz) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
-  const int id = a.set.id[s];
   const int4 ci = a.set.cell[s];
   const float4 di = a.set.off[s];
   const int tx = ci.x / t.tx, ty = ci.y / t.ty, tz = ci.z / t.tz;
@@ -133,13 +148,12 @@ __global__ __launch_bounds__(256) void k_wca(WcaArgs a) {
     if (lo >= rc) loz = 0;
     if ((float)t.tz * g.dz - lo >= rc) hiz = 0;
   }
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
   for (int dz = loz; dz <= hiz; ++dz)
     for (int dy = loy; dy <= hiy; ++dy)
       for (int dx = lox; dx <= hix; ++dx) {
         int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
         int j0 = a.set.start[nt], j1 = a.set.start[nt + 1];
-        for (int j = j0; j < j1; ++j) {
+        for (int j = j0 + q; j < j1; j += 4) {
           if (j == s) continue;
           int4 cj = a.set.cell[j];
           float4 dj = a.set.off[j];
@@ -160,7 +174,6 @@ __global__ __launch_bounds__(256) void k_wca(WcaArgs a) {
           }
         }
       }
-  a.F[id] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // Weights of one axis at an explicit node base (used for the reference-quirk RFD variant):
@@ -215,14 +228,39 @@ FHD_DEV void rfd_normal(const Key& key, int id, float n[4]) {
   draw_normals4(key, (uint32_t)id, FHD_STREAM_RFD, 0, n);
 }
 
+// Candidate item of a spreading workgroup: group (1 bit), sub-source (1 bit), sorted slot.
+FHD_DEV int pack_item(int gi, int sub, int sidx) { return (gi << 30) | (sub << 29) | sidx; }
+
+// Flattened source item e of group gi -> sorted slot (binary search over the neighbour-tile
+// prefix sums held in LDS).
+FHD_DEV int item_slot(const int* pref, const int* st, int n, int src) {
+  int lo = 0, hi = n;  // find k with pref[k] <= src < pref[k+1]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pref[mid] <= src) lo = mid; else hi = mid;
+  }
+  return st[lo] + (src - pref[lo]);
+}
+
+// Can a source in (wrapped) cell c reach the tile [o, o+T) along an axis of m cells?  Every
+// stencil of a source (including the RFD displacement, < h/2) lies in cells c-3 .. c+3.
+FHD_DEV bool reaches(int c, int o, int T, int m, int ntiles) {
+  if (ntiles == 1) return true;
+  const int l0 = fold(c - 3 - o, m);
+  return l0 < T || l0 + 6 >= m;
+}
+
+constexpr int kItemCap = 1024;   // compacted candidate items per round
+
 template <int NPTS>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   __shared__ unsigned long long acc[3 * kTileMax];
-  __shared__ int nb_start[2][27], nb_count[2][27], nb_pref[2][28];
+  __shared__ int nb_start[2][27], nb_pref[2][28];
   __shared__ int n_nb[2];
   __shared__ unsigned int maxbits;
   __shared__ Cand cand[kCandCap];
-  __shared__ int ncand;
+  __shared__ int items[kItemCap];
+  __shared__ int nitem;
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tile = blockIdx.x;
@@ -244,101 +282,102 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
           for (int dx = lox; dx <= hix; ++dx) {
             int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
             int s0 = a.grp[gi].set.start[nt];
-            int cnt = a.grp[gi].set.start[nt + 1] - s0;
             nb_start[gi][k] = s0;
-            nb_count[gi][k] = cnt;
-            nb_pref[gi][k + 1] = nb_pref[gi][k] + cnt;
+            nb_pref[gi][k + 1] = nb_pref[gi][k] + (a.grp[gi].set.start[nt + 1] - s0);
             ++k;
           }
       n_nb[gi] = k;
     }
   }
   __syncthreads();
-  // pass 1: largest source magnitude (sets the fixed-point scale)
-  float lmax = 0.0f;
-  int nsrc_total = 0;
+  int nsrc[2] = {0, 0}, nsub[2] = {1, 1};
   for (int gi = 0; gi < a.ngroups; ++gi) {
+    nsub[gi] = a.grp[gi].mode == SRC_RFD ? 2 : 1;
+    nsrc[gi] = nb_pref[gi][n_nb[gi]];
+  }
+  const int nall = nsrc[0] + nsrc[1];
+  // pass 1: cull on the sorted cell only and take the largest candidate magnitude (fixes the
+  // fixed-point scale); RFD values are bounded by |z| <= 5.8 for 24-bit Box-Muller normals
+  float lmax = 0.0f;
+  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
+    const int gi = e < nsrc[0] ? 0 : 1;
+    const int src = gi == 0 ? e : e - nsrc[0];
     const SpreadGroup& G = a.grp[gi];
-    const int tot = nb_pref[gi][n_nb[gi]];
-    nsrc_total += tot * (G.mode == SRC_RFD ? 2 : 1);
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      int k = 0;
-      while (nb_pref[gi][k + 1] <= e) ++k;
-      int sidx = nb_start[gi][k] + (e - nb_pref[gi][k]);
-      float m;
-      if (G.mode == SRC_VALUE) {
-        float4 v = G.val[G.set.id[sidx]];
-        m = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) * fabsf(G.coef);
-      } else if (G.mode == SRC_RFD) {
-        float n[4];
-        rfd_normal(a.key, G.set.id[sidx], n);
-        m = fmaxf(fabsf(n[0]), fmaxf(fabsf(n[1]), fabsf(n[2]))) * fabsf(G.coef);
-      } else {
-        m = 2.0f * fabsf(G.coef) * fmaxf(g.idx, fmaxf(g.idy, g.idz));
-      }
-      if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
-      lmax = fmaxf(lmax, m);
+    const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
+    const int4 c = G.set.cell[sidx];
+    if (!(reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
+          reaches(c.z, oz, t.tz, g.mz, t.ntz)))
+      continue;
+    float m;
+    if (G.mode == SRC_VALUE) {
+      const float4 v = G.val[G.set.id[sidx]];
+      m = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) * fabsf(G.coef);
+    } else if (G.mode == SRC_RFD) {
+      m = 5.8f * fabsf(G.coef);
+    } else {
+      m = 2.0f * fabsf(G.coef) * fmaxf(g.idx, fmaxf(g.idy, g.idz));
     }
+    if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
+    lmax = fmaxf(lmax, m);
   }
   atomicMax(&maxbits, __float_as_uint(lmax));
   __syncthreads();
   const float M = __uint_as_float(maxbits);
+  float inv = 0.0f;
   if (M > 0.0f) {
     int e2;
-    frexpf(M * (float)(8 * nsrc_total + 1), &e2);
+    frexpf(M * (float)(8 * nall * 2 + 1), &e2);
     e2 = max(e2, -60);
     const float scale = ldexpf(1.0f, 62 - e2);
-    // Flattened source items over all groups: (group, neighbour-tile slot, sub-source).
-    int items[2];
-    for (int gi = 0; gi < 2; ++gi)
-      items[gi] = gi < a.ngroups ? nb_pref[gi][n_nb[gi]] * (a.grp[gi].mode == SRC_RFD ? 2 : 1) : 0;
-    const int nitems = items[0] + items[1];
-    for (int cursor = 0; cursor < nitems; cursor += blockDim.x) {
-      if (threadIdx.x == 0) ncand = 0;
+    inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
+    // pass 2: rounds of kItemCap scanned sources -> compacted candidates -> accumulate
+    for (int r0 = 0; r0 < nall; r0 += kItemCap / 2) {
+      if (threadIdx.x == 0) nitem = 0;
       __syncthreads();
-      // stage: every source whose stencil can reach this tile becomes a candidate
-      const int e = cursor + threadIdx.x;
-      if (e < nitems) {
-        const int gi = e < items[0] ? 0 : 1;
-        const int el = gi == 0 ? e : e - items[0];
+      const int r1 = min(nall, r0 + kItemCap / 2);  // <= 2 items per source
+      for (int e = r0 + threadIdx.x; e < r1; e += blockDim.x) {
+        const int gi = e < nsrc[0] ? 0 : 1;
+        const int src = gi == 0 ? e : e - nsrc[0];
         const SpreadGroup& G = a.grp[gi];
-        const int nsub = G.mode == SRC_RFD ? 2 : 1;
-        const int src = el / nsub, sub = el - src * nsub;
-        int k = 0;
-        while (nb_pref[gi][k + 1] <= src) ++k;
-        const int sidx = nb_start[gi][k] + (src - nb_pref[gi][k]);
-        const int id = G.set.id[sidx];
-        int4 c = G.set.cell[sidx];
-        float4 d = G.set.off[sidx];
-        float3 val;
-        float nrm[4] = {0, 0, 0, 0};
-        int4 cw = c;  // cell the stencil is built on
-        float3 dw = make_float3(d.x, d.y, d.z);
-        if (G.mode == SRC_VALUE) {
-          const float4 v = G.val[id];
-          val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
-        } else if (G.mode == SRC_DRIFT) {
-          val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
-        } else {
-          rfd_normal(a.key, id, nrm);
-          const float sg = sub == 0 ? 1.0f : -1.0f;
-          val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
-          dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
-          renorm_axis(cw.x, dw.x, g.dx, g.idx);
-          renorm_axis(cw.y, dw.y, g.dy, g.idy);
-          renorm_axis(cw.z, dw.z, g.dz, g.idz);
+        const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
+        const int4 c = G.set.cell[sidx];
+        if (reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
+            reaches(c.z, oz, t.tz, g.mz, t.ntz)) {
+          const int k = atomicAdd(&nitem, nsub[gi]);
+          for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
         }
-        // every stencil of this source lies within cells cw-2 .. cw+2 on each axis: it reaches
-        // the tile when that 5-cell window (folded to the tile origin) starts inside the tile or
-        // wraps around onto its start
-        const int l0x = fold(cw.x - 2 - ox, g.mx), l0y = fold(cw.y - 2 - oy, g.my), l0z = fold(cw.z - 2 - oz, g.mz);
-        const bool hitx = t.ntx == 1 || l0x < t.tx || l0x + 4 >= g.mx;
-        const bool hity = t.nty == 1 || l0y < t.ty || l0y + 4 >= g.my;
-        const bool hitz = t.ntz == 1 || l0z < t.tz || l0z + 4 >= g.mz;
-        if (hitx && hity && hitz) {
+      }
+      __syncthreads();
+      const int ni = nitem;
+      for (int c0 = 0; c0 < ni; c0 += kCandCap) {
+        const int nc = min(kCandCap, ni - c0);
+        if ((int)threadIdx.x < nc) {
+          const int it = items[c0 + threadIdx.x];
+          const int gi = it >> 30, sub = (it >> 29) & 1, sidx = it & ((1 << 29) - 1);
+          const SpreadGroup& G = a.grp[gi];
+          const int id = G.set.id[sidx];
+          const int4 c = G.set.cell[sidx];
+          const float4 d = G.set.off[sidx];
+          float3 val;
+          int4 cw = c;
+          float3 dw = make_float3(d.x, d.y, d.z);
+          if (G.mode == SRC_VALUE) {
+            const float4 v = G.val[id];
+            val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
+          } else if (G.mode == SRC_DRIFT) {
+            val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
+          } else {
+            float nrm[4];
+            rfd_normal(a.key, id, nrm);
+            const float sg = sub == 0 ? 1.0f : -1.0f;
+            val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
+            dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
+            renorm_axis(cw.x, dw.x, g.dx, g.idx);
+            renorm_axis(cw.y, dw.y, g.dy, g.idy);
+            renorm_axis(cw.z, dw.z, g.dz, g.idz);
+          }
           SrcWeights w;
-          const float3 fr = make_float3(dw.x * g.idx, dw.y * g.idy, dw.z * g.idz);
-          source_weights<NPTS>(cw, fr, w, G.mode == SRC_DRIFT);
+          source_weights<NPTS>(cw, make_float3(dw.x * g.idx, dw.y * g.idy, dw.z * g.idz), w, G.mode == SRC_DRIFT);
           if (G.mode == SRC_RFD && G.rfd_quirk) {
             // nodes from the displaced position, weights at the undisplaced one
             const int cc[3] = {c.x, c.y, c.z};
@@ -349,8 +388,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
               axis_weights_at<NPTS>(w.stg[ax].base, cc[ax], ff[ax], 1, w.stg[ax]);
             }
           }
-          const int slot = atomicAdd(&ncand, 1);
-          Cand& C = cand[slot];
+          Cand& C = cand[threadIdx.x];
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
             C.base[0][ax] = w.cen[ax].base;
@@ -365,94 +403,103 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
           C.val[1] = val.y * scale;
           C.val[2] = val.z * scale;
         }
-      }
-      __syncthreads();
-      // accumulate: one work item per (candidate, component, z-row, y-row) line of NPTS nodes
-      const int nc = ncand;
-      const int nlines = nc * 3 * NPTS * NPTS;
-      for (int li = threadIdx.x; li < nlines; li += blockDim.x) {
-        const int cidx = li / (3 * NPTS * NPTS);
-        const int r = li - cidx * (3 * NPTS * NPTS);
-        const int comp = r / (NPTS * NPTS);
-        const int r2 = r - comp * NPTS * NPTS;
-        const int kz = r2 / NPTS, ky = r2 - kz * NPTS;
-        const Cand& C = cand[cidx];
-        const float vc = C.val[comp];
-        if (vc == 0.0f) continue;
-        const int sx = comp == 0 ? 1 : 0, sy = comp == 1 ? 1 : 0, sz = comp == 2 ? 1 : 0;
-        const int lz = fold(C.base[sz][2] + kz - oz, g.mz);
-        if (lz >= t.tz) continue;
-        const int ly = fold(C.base[sy][1] + ky - oy, g.my);
-        if (ly >= t.ty) continue;
-        const float wyz = C.w[sy][1][ky] * C.w[sz][2][kz] * vc;
-        const int bx = C.base[sx][0];
+        __syncthreads();
+        // one work item per (candidate, component, z-row): NPTS x NPTS nodes
+        const int nwork = nc * 3 * NPTS;
+        for (int li = threadIdx.x; li < nwork; li += blockDim.x) {
+          const int cidx = li / (3 * NPTS);
+          const int r = li - cidx * (3 * NPTS);
+          const int comp = r / NPTS, kz = r - comp * NPTS;
+          const Cand& C = cand[cidx];
+          const float vc = C.val[comp];
+          if (vc == 0.0f) continue;
+          const int sx = comp == 0, sy = comp == 1, sz = comp == 2;
+          const int lz = fold(C.base[sz][2] + kz - oz, g.mz);
+          if (lz >= t.tz) continue;
+          const float wz = C.w[sz][2][kz] * vc;
+          const int by = C.base[sy][1], bx = C.base[sx][0];
+          int lxs[NPTS];
 #pragma unroll
-        for (int kx = 0; kx < NPTS; ++kx) {
-          const int lx = fold(bx + kx - ox, g.mx);
-          if (lx < t.tx) {
-            const long long q = __float2ll_rn(C.w[sx][0][kx] * wyz);
-            atomicAdd(&acc[comp * kTileMax + lx + t.tx * (ly + t.ty * lz)], (unsigned long long)q);
+          for (int kx = 0; kx < NPTS; ++kx) lxs[kx] = fold(bx + kx - ox, g.mx);
+#pragma unroll
+          for (int ky = 0; ky < NPTS; ++ky) {
+            const int ly = fold(by + ky - oy, g.my);
+            if (ly >= t.ty) continue;
+            const float wyz = C.w[sy][1][ky] * wz;
+            unsigned long long* row = &acc[comp * kTileMax + t.tx * (ly + t.ty * lz)];
+#pragma unroll
+            for (int kx = 0; kx < NPTS; ++kx)
+              if (lxs[kx] < t.tx) {
+                const long long q = __float2ll_rn(C.w[sx][0][kx] * wyz);
+                atomicAdd(&row[lxs[kx]], (unsigned long long)q);
+              }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
-    const float inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
-    for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
-      int comp = e / ncell, l = e - comp * ncell;
-      int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
-      size_t gi = (size_t)comp * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-      a.out[gi] = (float)(long long)acc[comp * kTileMax + l] * inv;
-    }
-  } else {
-    for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
-      int comp = e / ncell, l = e - comp * ncell;
-      int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
-      size_t gi = (size_t)comp * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-      a.out[gi] = 0.0f;
-    }
+  }
+  for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
+    const int comp = e / ncell, l = e - comp * ncell;
+    const int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
+    const size_t gi = (size_t)comp * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
+    a.out[gi] = (float)(long long)acc[comp * kTileMax + l] * inv;
   }
 }
 
 template <int NPTS>
-FHD_DEV float interp_comp(const float* __restrict__ f, const GridDev& g, const AxisWeights& wx,
-                          const AxisWeights& wy, const AxisWeights& wz) {
-  float s = 0.0f;
-  for (int kz = 0; kz < NPTS; ++kz) {
-    int z = fold(wz.base + kz, g.mz);
-    float sy = 0.0f;
-    for (int ky = 0; ky < NPTS; ++ky) {
-      int y = fold(wy.base + ky, g.my);
-      const float* row = f + (size_t)g.mx * (y + (size_t)g.my * z);
-      float sx = 0.0f;
+// Sum over the NPTS x NPTS nodes of one z-row (kz) of a particle's stencil.
+template <int NPTS>
+FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const AxisWeights& wx,
+                         const AxisWeights& wy, const AxisWeights& wz, int kz) {
+  const int z = fold(wz.base + kz, g.mz);
+  int xs[NPTS];
 #pragma unroll
-      for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[fold(wx.base + kx, g.mx)];
-      sy += wy.w[ky] * sx;
-    }
-    s += wz.w[kz] * sy;
+  for (int kx = 0; kx < NPTS; ++kx) xs[kx] = fold(wx.base + kx, g.mx);
+  float sy = 0.0f;
+#pragma unroll
+  for (int ky = 0; ky < NPTS; ++ky) {
+    const int y = fold(wy.base + ky, g.my);
+    const float* row = f + (size_t)g.mx * (y + (size_t)g.my * z);
+    float sx = 0.0f;
+#pragma unroll
+    for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[xs[kx]];
+    sy += wy.w[ky] * sx;
   }
-  return s;
+  return wz.w[kz] * sy;
 }
 
+// Four lanes per particle: lane q sums z-row q of the stencil; the rows are combined by a fixed
+// xor-shuffle tree, then lane 0 applies the position update.
 template <int NPTS>
 __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.np) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid >> 2, q = gid & 3;
   const GridDev& g = a.g;
-  const int id = a.eval.id[s];
-  const int4 c = a.eval.cell[s];
-  const float4 d = a.eval.off[s];
-  SrcWeights w;
-  source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
   float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
+  if (s < a.np && q < NPTS) {
+    const int4 c = a.eval.cell[s];
+    const float4 d = a.eval.off[s];
+    SrcWeights w;
+    source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
 #pragma unroll
-  for (int comp = 0; comp < 3; ++comp) {
-    const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
-    const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
-    const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
-    if (a.f1) u1[comp] = interp_comp<NPTS>(a.f1 + (size_t)comp * g.N, g, wx, wy, wz);
-    if (a.f2) u2[comp] = interp_comp<NPTS>(a.f2 + (size_t)comp * g.N, g, wx, wy, wz);
+    for (int comp = 0; comp < 3; ++comp) {
+      const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
+      const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
+      const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
+      if (a.f1) u1[comp] = interp_row<NPTS>(a.f1 + (size_t)comp * g.N, g, wx, wy, wz, q);
+      if (a.f2) u2[comp] = interp_row<NPTS>(a.f2 + (size_t)comp * g.N, g, wx, wy, wz, q);
+    }
   }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    u1[k] += __shfl_xor(u1[k], 1);
+    u2[k] += __shfl_xor(u2[k], 1);
+    u1[k] += __shfl_xor(u1[k], 2);
+    u2[k] += __shfl_xor(u2[k], 2);
+  }
+  if (s >= a.np || q != 0) return;
+  const int id = a.eval.id[s];
   float disp[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) disp[k] = a.a1 * u1[k] + a.a2 * u2[k];
@@ -521,7 +568,7 @@ void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const 
                      soff);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
-  if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(a.np, 256)), dim3(256), 0, s, a);
+  if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
 }
 void launch_spread(const SpreadArgs& a, hipStream_t s) {
   if (a.kernel_points == 4) hipLaunchKernelGGL(k_spread<4>, dim3(a.t.n), dim3(256), 0, s, a);
@@ -529,8 +576,8 @@ void launch_spread(const SpreadArgs& a, hipStream_t s) {
 }
 void launch_interp(const InterpArgs& a, hipStream_t s) {
   if (a.np <= 0) return;
-  if (a.kernel_points == 4) hipLaunchKernelGGL(k_interp<4>, dim3(nblk(a.np, 256)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_interp<3>, dim3(nblk(a.np, 256)), dim3(256), 0, s, a);
+  if (a.kernel_points == 4) hipLaunchKernelGGL(k_interp<4>, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_interp<3>, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
 }
 void launch_zero_f4(float4* p, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero_f4, dim3(nblk(n, 256)), dim3(256), 0, s, p, n);
