@@ -268,26 +268,28 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
   const int ncell = t.tx * t.ty * t.tz;
   for (int e = threadIdx.x; e < 3 * kTileMax; e += blockDim.x) acc[e] = 0ull;
-  if (threadIdx.x == 0) {
-    maxbits = 0u;
-    int lox, hix, loy, hiy, loz, hiz;
-    nb_range(t.ntx, lox, hix);
-    nb_range(t.nty, loy, hiy);
-    nb_range(t.ntz, loz, hiz);
-    for (int gi = 0; gi < a.ngroups; ++gi) {
-      int k = 0;
-      nb_pref[gi][0] = 0;
-      for (int dz = loz; dz <= hiz; ++dz)
-        for (int dy = loy; dy <= hiy; ++dy)
-          for (int dx = lox; dx <= hix; ++dx) {
-            int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
-            int s0 = a.grp[gi].set.start[nt];
-            nb_start[gi][k] = s0;
-            nb_pref[gi][k + 1] = nb_pref[gi][k] + (a.grp[gi].set.start[nt + 1] - s0);
-            ++k;
-          }
-      n_nb[gi] = k;
-    }
+  // neighbour-tile table: one thread per (group, neighbour) loads its slot range in parallel
+  int lox, hix, loy, hiy, loz, hiz;
+  nb_range(t.ntx, lox, hix);
+  nb_range(t.nty, loy, hiy);
+  nb_range(t.ntz, loz, hiz);
+  const int nx = hix - lox + 1, ny = hiy - loy + 1, nz = hiz - loz + 1;
+  const int nnb = nx * ny * nz;
+  if ((int)threadIdx.x < a.ngroups * nnb) {
+    const int gi = threadIdx.x / nnb, k = threadIdx.x - gi * nnb;
+    const int dx = lox + k % nx, dy = loy + (k / nx) % ny, dz = loz + k / (nx * ny);
+    const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
+    const int s0 = a.grp[gi].set.start[nt];
+    nb_start[gi][k] = s0;
+    nb_pref[gi][k + 1] = a.grp[gi].set.start[nt + 1] - s0;  // count, prefix-summed below
+  }
+  if (threadIdx.x == 0) maxbits = 0u;
+  __syncthreads();
+  if ((int)threadIdx.x < a.ngroups) {
+    const int gi = threadIdx.x;
+    nb_pref[gi][0] = 0;
+    for (int k = 0; k < nnb; ++k) nb_pref[gi][k + 1] += nb_pref[gi][k];
+    n_nb[gi] = nnb;
   }
   __syncthreads();
   int nsrc[2] = {0, 0}, nsub[2] = {1, 1};
@@ -447,7 +449,6 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   }
 }
 
-template <int NPTS>
 // Sum over the NPTS x NPTS nodes of one z-row (kz) of a particle's stencil.
 template <int NPTS>
 FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const AxisWeights& wx,
