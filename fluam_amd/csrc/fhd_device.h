@@ -124,6 +124,10 @@ struct AxisWeights {
 // for a particle in cell c at offset frac = d/h in [-1/2, 1/2).  Returns nodes for the 3-point
 // (npts = 3) or 4-point (npts = 4) kernel.  Same stencil choice as the FP64 restatement: the
 // 3-point stencil is centred on the nearest node, the 4-point one on the two nearest.
+// The weight sets use one square root each: with f the offset from the stencil's reference
+// node, the 3-point weights follow from phi3(f) and the moment conditions sum w = 1,
+// sum (f - k) w_k = 0; the 4-point weights are Peskin's one-root form (src/deltaGPU.cu:44-53).
+// They equal phi3 / phi4 at the respective distances.
 template <int NPTS>
 FHD_DEV void axis_weights(int c, float frac, int stag, AxisWeights& aw) {
   // t = position in node units relative to node index 0 of this staggering.
@@ -135,9 +139,10 @@ FHD_DEV void axis_weights(int c, float frac, int stag, AxisWeights& aw) {
     else if (frac >= 0.0f) { j0 = c; f = frac - 0.5f; }
     else { j0 = c - 1; f = frac + 0.5f; }
     aw.base = j0 - 1;
-    aw.w[0] = phi3(f + 1.0f);
-    aw.w[1] = phi3(f);
-    aw.w[2] = phi3(f - 1.0f);
+    const float w0 = (1.0f + sqrtf(fmaxf(1.0f - 3.0f * f * f, 0.0f))) * (1.0f / 3.0f);
+    aw.w[0] = 0.5f * (1.0f - w0 - f);  // node j0 - 1, distance f + 1
+    aw.w[1] = w0;
+    aw.w[2] = 0.5f * (1.0f - w0 + f);  // node j0 + 1, distance f - 1
     aw.w[3] = 0.0f;
   } else {
     int j0;
@@ -148,10 +153,11 @@ FHD_DEV void axis_weights(int c, float frac, int stag, AxisWeights& aw) {
       j0 = c - 1; f = frac + 0.5f;
     }
     aw.base = j0 - 1;
-    aw.w[0] = phi4(f + 1.0f);
-    aw.w[1] = phi4(f);
-    aw.w[2] = phi4(f - 1.0f);
-    aw.w[3] = phi4(f - 2.0f);
+    const float s = 0.125f * (3.0f - 2.0f * f + sqrtf(fmaxf(1.0f + 4.0f * f * (1.0f - f), 0.0f)));
+    aw.w[0] = 0.75f - 0.5f * f - s;  // distance f + 1
+    aw.w[1] = s;                      // distance f
+    aw.w[2] = 0.5f * f - 0.25f + s;  // distance f - 1
+    aw.w[3] = 0.5f - s;               // distance f - 2
   }
 }
 
@@ -165,9 +171,11 @@ FHD_DEV void axis_dweights(int c, float frac, int stag, AxisWeights& aw) {
     else if (frac >= 0.0f) { j0 = c; f = frac - 0.5f; }
     else { j0 = c - 1; f = frac + 0.5f; }
     aw.base = j0 - 1;
-    aw.w[0] = dphi3(f + 1.0f);
-    aw.w[1] = dphi3(f);
-    aw.w[2] = dphi3(f - 1.0f);
+    // derivatives of the one-root forms in axis_weights with respect to f
+    const float d0 = -f * rsqrtf(fmaxf(1.0f - 3.0f * f * f, 1e-30f));
+    aw.w[0] = 0.5f * (-d0 - 1.0f);
+    aw.w[1] = d0;
+    aw.w[2] = 0.5f * (-d0 + 1.0f);
     aw.w[3] = 0.0f;
   } else {
     int j0;
@@ -178,10 +186,11 @@ FHD_DEV void axis_dweights(int c, float frac, int stag, AxisWeights& aw) {
       j0 = c - 1; f = frac + 0.5f;
     }
     aw.base = j0 - 1;
-    aw.w[0] = dphi4(f + 1.0f);
-    aw.w[1] = dphi4(f);
-    aw.w[2] = dphi4(f - 1.0f);
-    aw.w[3] = dphi4(f - 2.0f);
+    const float ds = 0.125f * (-2.0f + (2.0f - 4.0f * f) * rsqrtf(fmaxf(1.0f + 4.0f * f * (1.0f - f), 1e-30f)));
+    aw.w[0] = -0.5f - ds;
+    aw.w[1] = ds;
+    aw.w[2] = 0.5f + ds;
+    aw.w[3] = -ds;
   }
 }
 

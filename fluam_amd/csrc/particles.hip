@@ -205,10 +205,29 @@ FHD_DEV void source_weights(int4 c, float3 frac, SrcWeights& w, bool deriv) {
 constexpr int kTileMax = 512;  // cells per spreading tile (8 x 8 x 8)
 
 // Fold a node index offset into [0, m) (the offsets here stay within a few boxes of the tile).
+// Offsets passed here lie in (-m, 2m) whenever m >= 8 (wrapped cells, stencils within three
+// cells, tiles of at least four cells); smaller boxes take the loop.
 FHD_DEV int fold(int l, int m) {
-  while (l < 0) l += m;
-  while (l >= m) l -= m;
+  if (l < 0) l += m;
+  else if (l >= m) l -= m;
+  if (m < 8) {
+    while (l < 0) l += m;
+    while (l >= m) l -= m;
+  }
   return l;
+}
+
+// Fixed-point image of a float for the LDS accumulators: the value is m 2^e with an integer
+// mantissa m, shifted to an integer (truncating below 2^0); |x| must be below 2^62.  Exact and
+// order-independent when summed as 64-bit integers.
+FHD_DEV unsigned long long f2fixed(float x) {
+  const unsigned b = __float_as_uint(x);
+  const int be = (int)((b >> 23) & 255u);
+  if (be == 0) return 0ull;  // zero or denormal
+  const unsigned long long m = (unsigned long long)((b & 0x7FFFFFu) | 0x800000u);
+  const int e = be - 150;
+  unsigned long long v = e >= 0 ? (m << e) : (e > -24 ? (m >> -e) : 0ull);
+  return (b >> 31) ? (0ull - v) : v;
 }
 
 // Candidate source staged in LDS: per axis the centred and face weights with their node bases,
@@ -283,7 +302,10 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     nb_start[gi][k] = s0;
     nb_pref[gi][k + 1] = a.grp[gi].set.start[nt + 1] - s0;  // count, prefix-summed below
   }
-  if (threadIdx.x == 0) maxbits = 0u;
+  if (threadIdx.x == 0) {
+    maxbits = 0u;
+    nitem = 0;
+  }
   __syncthreads();
   if ((int)threadIdx.x < a.ngroups) {
     const int gi = threadIdx.x;
@@ -298,8 +320,9 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     nsrc[gi] = nb_pref[gi][n_nb[gi]];
   }
   const int nall = nsrc[0] + nsrc[1];
-  // pass 1: cull on the sorted cell only and take the largest candidate magnitude (fixes the
-  // fixed-point scale); RFD values are bounded by |z| <= 5.8 for 24-bit Box-Muller normals
+  // pass 1: cull on the sorted cell only, compact the candidates into the LDS item list (when
+  // they fit) and take the largest candidate magnitude (fixes the fixed-point scale); RFD values
+  // are bounded by |z| <= 5.8 for 24-bit Box-Muller normals
   float lmax = 0.0f;
   for (int e = threadIdx.x; e < nall; e += blockDim.x) {
     const int gi = e < nsrc[0] ? 0 : 1;
@@ -310,6 +333,9 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     if (!(reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
           reaches(c.z, oz, t.tz, g.mz, t.ntz)))
       continue;
+    const int k = atomicAdd(&nitem, nsub[gi]);
+    if (k + nsub[gi] <= kItemCap)
+      for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
     float m;
     if (G.mode == SRC_VALUE) {
       const float4 v = G.val[G.set.id[sidx]];
@@ -332,25 +358,31 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     e2 = max(e2, -60);
     const float scale = ldexpf(1.0f, 62 - e2);
     inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
-    // pass 2: rounds of kItemCap scanned sources -> compacted candidates -> accumulate
+    // pass 2: the compacted list from pass 1, or (when it overflowed) rounds of kItemCap/2
+    // rescanned sources -> candidates -> accumulate
+    const int nfirst = nitem;
+    const bool fits = nfirst <= kItemCap;
     for (int r0 = 0; r0 < nall; r0 += kItemCap / 2) {
-      if (threadIdx.x == 0) nitem = 0;
-      __syncthreads();
-      const int r1 = min(nall, r0 + kItemCap / 2);  // <= 2 items per source
-      for (int e = r0 + threadIdx.x; e < r1; e += blockDim.x) {
-        const int gi = e < nsrc[0] ? 0 : 1;
-        const int src = gi == 0 ? e : e - nsrc[0];
-        const SpreadGroup& G = a.grp[gi];
-        const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
-        const int4 c = G.set.cell[sidx];
-        if (reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
-            reaches(c.z, oz, t.tz, g.mz, t.ntz)) {
-          const int k = atomicAdd(&nitem, nsub[gi]);
-          for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
+      if (!fits) {
+        __syncthreads();
+        if (threadIdx.x == 0) nitem = 0;
+        __syncthreads();
+        const int r1 = min(nall, r0 + kItemCap / 2);  // <= 2 items per source
+        for (int e = r0 + threadIdx.x; e < r1; e += blockDim.x) {
+          const int gi = e < nsrc[0] ? 0 : 1;
+          const int src = gi == 0 ? e : e - nsrc[0];
+          const SpreadGroup& G = a.grp[gi];
+          const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
+          const int4 c = G.set.cell[sidx];
+          if (reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
+              reaches(c.z, oz, t.tz, g.mz, t.ntz)) {
+            const int k = atomicAdd(&nitem, nsub[gi]);
+            for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
-      const int ni = nitem;
+      const int ni = fits ? nfirst : nitem;
       for (int c0 = 0; c0 < ni; c0 += kCandCap) {
         const int nc = min(kCandCap, ni - c0);
         if ((int)threadIdx.x < nc) {
@@ -431,14 +463,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
             unsigned long long* row = &acc[comp * kTileMax + t.tx * (ly + t.ty * lz)];
 #pragma unroll
             for (int kx = 0; kx < NPTS; ++kx)
-              if (lxs[kx] < t.tx) {
-                const long long q = __float2ll_rn(C.w[sx][0][kx] * wyz);
-                atomicAdd(&row[lxs[kx]], (unsigned long long)q);
-              }
+              if (lxs[kx] < t.tx) atomicAdd(&row[lxs[kx]], f2fixed(C.w[sx][0][kx] * wyz));
           }
         }
         __syncthreads();
       }
+      if (fits) break;
     }
   }
   for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
@@ -525,8 +555,8 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
     disp[2] += a.aN * z[2];
   }
   if (a.u_out) {
-    const float* u = a.f2 ? u2 : u1;
-    a.u_out[id] = make_float4(u[0], u[1], u[2], 0.0f);
+    const bool two = a.f2 != nullptr;
+    a.u_out[id] = make_float4(two ? u2[0] : u1[0], two ? u2[1] : u1[1], two ? u2[2] : u1[2], 0.0f);
   }
   if (a.out_cell) {
     int4 bc = a.base_cell[id];
