@@ -497,7 +497,9 @@ FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const Ax
     for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[xs[kx]];
     sy += wy.w[ky] * sx;
   }
-  return wz.w[kz] * sy;
+  // kz is a lane index: select the weight instead of indexing (keeps wz in registers)
+  const float wk = kz == 0 ? wz.w[0] : kz == 1 ? wz.w[1] : kz == 2 ? wz.w[2] : wz.w[3];
+  return wk * sy;
 }
 
 // Four lanes per particle: lane q sums z-row q of the stencil; the rows are combined by a fixed
