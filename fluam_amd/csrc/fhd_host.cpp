@@ -103,6 +103,8 @@ struct fhd_ctx {
   int* sid[2] = {nullptr, nullptr};
   int4* scell[2] = {nullptr, nullptr};
   float4* soff[2] = {nullptr, nullptr};
+  SrcRec* rec[2] = {nullptr, nullptr};   // spreading source records per group (2 np each)
+  int4* cull[2] = {nullptr, nullptr};
   int* errd = nullptr;
   long long step = 0;
   bool projected = false;
@@ -232,7 +234,8 @@ void free_ctx(fhd_ctx* c) {
   void* ptrs[] = {c->v, c->v2, c->W, c->SF, c->adv[0], c->adv[1], c->div, c->spec, c->twx, c->twy, c->twz,
                   c->ex, c->ey, c->ez, c->gx, c->gy, c->gz, c->cell[0], c->cell[1], c->off[0], c->off[1],
                   c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
-                  c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd};
+                  c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
+                  c->cull[0], c->cull[1]};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -327,6 +330,17 @@ void wca(fhd_ctx* c, int k, float4* F) {
   launch_wca(a, c->s);
   mark(c, "wca");
 }
+// Attach the per-group record buffers, then prepare the source records and spread.
+void run_spread(fhd_ctx* c, SpreadArgs& sa) {
+  for (int k = 0; k < sa.ngroups; ++k) {
+    sa.grp[k].nsrc = c->np;
+    sa.grp[k].rec = c->rec[k];
+    sa.grp[k].cull = c->cull[k];
+  }
+  launch_spread(sa, c->s);
+  mark(c, "spread");
+}
+
 SpreadArgs spread_base(fhd_ctx* c) {
   SpreadArgs a;
   memset(&a, 0, sizeof(a));
@@ -402,8 +416,7 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[0].mode = SRC_VALUE;
     sa.grp[0].val = c->F[0];
     sa.grp[0].coef = 1.0f;
-    launch_spread(sa, c->s);
-    mark(c, "spread");
+    run_spread(c, sa);
   } else {
     launch_zero_f(c->SF, 3 * (size_t)c->g.N, c->s);
   }
@@ -444,8 +457,7 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[1].coef = (float)(p.temperature / p.rfd_delta);
     sa.grp[1].disp = (float)(0.5 * p.rfd_delta);
     sa.grp[1].rfd_quirk = p.rfd_mode == FHD_RFD_REFERENCE;
-    launch_spread(sa, c->s);
-    mark(c, "spread");
+    run_spread(c, sa);
   }
   w.noise = 2;
   w.f1 = (float)(sqrt(0.5) * f1);
@@ -514,8 +526,7 @@ void step_qnb(fhd_ctx* c) {
     sa.grp[1].set = sset(c, 1);
     sa.grp[1].mode = SRC_DRIFT;
     sa.grp[1].coef = (float)(p.temperature * ratio);
-    launch_spread(sa, c->s);
-    mark(c, "spread");
+    run_spread(c, sa);
     w.force = c->SF;
     w.force_coef = (float)(dt / rho);
     c->last_force_set = 1;
@@ -884,6 +895,8 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
         c->sid[k] = dalloc<int>(c->np);
         c->scell[k] = dalloc<int4>(c->np);
         c->soff[k] = dalloc<float4>(c->np);
+        c->rec[k] = dalloc<SrcRec>(2 * (size_t)c->np);
+        c->cull[k] = dalloc<int4>(2 * (size_t)c->np);
         HIPCHK(hipMemset(c->F[k], 0, sizeof(float4) * c->np));
       }
       c->u = dalloc<float4>(c->np);

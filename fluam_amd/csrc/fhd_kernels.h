@@ -59,13 +59,26 @@ struct SortedSet {
 
 // Source groups for the spreading kernel.
 enum { SRC_VALUE = 0, SRC_RFD = 1, SRC_DRIFT = 2 };
+// Stencil record of one spreading source (144 bytes): centred and face weights per axis, the
+// first node of each stencil, and the value.
+struct SrcRec {
+  float w[2][3][4];
+  float val[3];
+  float pad0;
+  int base[2][3];
+  int pad1[2];
+};
+
 struct SpreadGroup {
   SortedSet set;
+  int nsrc;             // sorted slots in the set (particles)
   int mode;
   const float4* val;    // SRC_VALUE: per-particle vector value (by id)
   float coef;           // multiplies the value (SRC_RFD: kT/delta; SRC_DRIFT: kT ratio)
   float disp;           // SRC_RFD: displacement magnitude delta/2
   int rfd_quirk;        // SRC_RFD: weights at the undisplaced position
+  SrcRec* rec;          // nsrc x (1 or 2) records, in sorted order
+  int4* cull;           // per record: lowest stencil node per axis, max |value| (float bits)
 };
 
 struct SpreadArgs {

@@ -247,11 +247,11 @@ FHD_DEV void rfd_normal(const Key& key, int id, float n[4]) {
   draw_normals4(key, (uint32_t)id, FHD_STREAM_RFD, 0, n);
 }
 
-// Candidate item of a spreading workgroup: group (1 bit), sub-source (1 bit), sorted slot.
-FHD_DEV int pack_item(int gi, int sub, int sidx) { return (gi << 30) | (sub << 29) | sidx; }
+// Candidate item of a spreading workgroup: group (1 bit) and source record index.
+FHD_DEV int pack_item(int gi, int rec) { return (gi << 30) | rec; }
 
-// Flattened source item e of group gi -> sorted slot (binary search over the neighbour-tile
-// prefix sums held in LDS).
+// Flattened source item -> record index (binary search over the neighbour-tile prefix sums
+// held in LDS).
 FHD_DEV int item_slot(const int* pref, const int* st, int n, int src) {
   int lo = 0, hi = n;  // find k with pref[k] <= src < pref[k+1]
   while (hi - lo > 1) {
@@ -261,16 +261,87 @@ FHD_DEV int item_slot(const int* pref, const int* st, int n, int src) {
   return st[lo] + (src - pref[lo]);
 }
 
-// Can a source in (wrapped) cell c reach the tile [o, o+T) along an axis of m cells?  Every
-// stencil of a source (including the RFD displacement, < h/2) lies in cells c-3 .. c+3.
-FHD_DEV bool reaches(int c, int o, int T, int m, int ntiles) {
+// Does a stencil whose nodes span [lo, lo + NPTS] (both staggerings) reach the tile [o, o+T)?
+template <int NPTS>
+FHD_DEV bool reaches(int lo, int o, int T, int m, int ntiles) {
   if (ntiles == 1) return true;
-  const int l0 = fold(c - 3 - o, m);
-  return l0 < T || l0 + 6 >= m;
+  const int l0 = fold(lo - o, m);
+  return l0 < T || l0 + NPTS >= m;
 }
 
-constexpr int kItemCap = 1024;   // compacted candidate items per round
+constexpr int kItemCap = 1024;   // compacted candidate records per round
 
+// One record per source (sorted slot x sub-source): weights, bases, value and the cull data.
+template <int NPTS>
+__global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int gi) {
+  const SpreadGroup& G = a.grp[gi];
+  const GridDev& g = a.g;
+  const int nsub = G.mode == SRC_RFD ? 2 : 1;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= G.nsrc * nsub) return;
+  const int sidx = r / nsub, sub = r - sidx * nsub;
+  const int id = G.set.id[sidx];
+  const int4 c = G.set.cell[sidx];
+  const float4 d = G.set.off[sidx];
+  float3 val;
+  int4 cw = c;
+  float3 dw = make_float3(d.x, d.y, d.z);
+  if (G.mode == SRC_VALUE) {
+    const float4 v = G.val[id];
+    val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
+  } else if (G.mode == SRC_DRIFT) {
+    val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
+  } else {
+    float nrm[4];
+    rfd_normal(a.key, id, nrm);
+    const float sg = sub == 0 ? 1.0f : -1.0f;
+    val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
+    dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
+    renorm_axis(cw.x, dw.x, g.dx, g.idx);
+    renorm_axis(cw.y, dw.y, g.dy, g.idy);
+    renorm_axis(cw.z, dw.z, g.dz, g.idz);
+  }
+  SrcWeights w;
+  source_weights<NPTS>(cw, make_float3(dw.x * g.idx, dw.y * g.idy, dw.z * g.idz), w, G.mode == SRC_DRIFT);
+  if (G.mode == SRC_RFD && G.rfd_quirk) {
+    // nodes from the displaced position, weights at the undisplaced one
+    const int cc[3] = {c.x, c.y, c.z};
+    const float ff[3] = {d.x * g.idx, d.y * g.idy, d.z * g.idz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      axis_weights_at<NPTS>(w.cen[ax].base, cc[ax], ff[ax], 0, w.cen[ax]);
+      axis_weights_at<NPTS>(w.stg[ax].base, cc[ax], ff[ax], 1, w.stg[ax]);
+    }
+  }
+  SrcRec rec;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    rec.base[0][ax] = w.cen[ax].base;
+    rec.base[1][ax] = w.stg[ax].base;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      rec.w[0][ax][q] = w.cen[ax].w[q];
+      rec.w[1][ax][q] = w.stg[ax].w[q];
+    }
+  }
+  rec.val[0] = val.x;
+  rec.val[1] = val.y;
+  rec.val[2] = val.z;
+  rec.pad0 = 0.0f;
+  rec.pad1[0] = rec.pad1[1] = 0;
+  const float4* src4 = reinterpret_cast<const float4*>(&rec);
+  float4* dst4 = reinterpret_cast<float4*>(G.rec + r);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(SrcRec) / 16); ++k) dst4[k] = src4[k];
+  float m = fmaxf(fabsf(val.x), fmaxf(fabsf(val.y), fabsf(val.z)));
+  if (G.mode == SRC_DRIFT) m *= 2.0f;  // derivative weights reach ~1.6
+  if (!(m <= 3.0e38f)) m = 3.0e38f;    // non-finite sources saturate
+  G.cull[r] = make_int4(min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
+                        min(w.cen[2].base, w.stg[2].base), (int)__float_as_uint(m));
+}
+
+// One workgroup per tile: cull the records of the sources in the neighbouring tiles, stage the
+// candidates in LDS and accumulate every in-tile contribution as a 64-bit fixed-point integer.
 template <int NPTS>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   __shared__ unsigned long long acc[3 * kTileMax];
@@ -287,7 +358,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
   const int ncell = t.tx * t.ty * t.tz;
   for (int e = threadIdx.x; e < 3 * kTileMax; e += blockDim.x) acc[e] = 0ull;
-  // neighbour-tile table: one thread per (group, neighbour) loads its slot range in parallel
+  // neighbour-tile table: one thread per (group, neighbour) loads its record range in parallel
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
   nb_range(t.nty, loy, hiy);
@@ -296,11 +367,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   const int nnb = nx * ny * nz;
   if ((int)threadIdx.x < a.ngroups * nnb) {
     const int gi = threadIdx.x / nnb, k = threadIdx.x - gi * nnb;
+    const int nsub = a.grp[gi].mode == SRC_RFD ? 2 : 1;
     const int dx = lox + k % nx, dy = loy + (k / nx) % ny, dz = loz + k / (nx * ny);
     const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
     const int s0 = a.grp[gi].set.start[nt];
-    nb_start[gi][k] = s0;
-    nb_pref[gi][k + 1] = a.grp[gi].set.start[nt + 1] - s0;  // count, prefix-summed below
+    nb_start[gi][k] = s0 * nsub;
+    nb_pref[gi][k + 1] = (a.grp[gi].set.start[nt + 1] - s0) * nsub;  // prefix-summed below
   }
   if (threadIdx.x == 0) {
     maxbits = 0u;
@@ -314,39 +386,21 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
     n_nb[gi] = nnb;
   }
   __syncthreads();
-  int nsrc[2] = {0, 0}, nsub[2] = {1, 1};
-  for (int gi = 0; gi < a.ngroups; ++gi) {
-    nsub[gi] = a.grp[gi].mode == SRC_RFD ? 2 : 1;
-    nsrc[gi] = nb_pref[gi][n_nb[gi]];
-  }
-  const int nall = nsrc[0] + nsrc[1];
-  // pass 1: cull on the sorted cell only, compact the candidates into the LDS item list (when
-  // they fit) and take the largest candidate magnitude (fixes the fixed-point scale); RFD values
-  // are bounded by |z| <= 5.8 for 24-bit Box-Muller normals
+  int nrec[2] = {0, 0};
+  for (int gi = 0; gi < a.ngroups; ++gi) nrec[gi] = nb_pref[gi][n_nb[gi]];
+  const int nall = nrec[0] + nrec[1];
+  // pass 1: cull, compact the candidates into the LDS list (when they fit), largest magnitude
   float lmax = 0.0f;
   for (int e = threadIdx.x; e < nall; e += blockDim.x) {
-    const int gi = e < nsrc[0] ? 0 : 1;
-    const int src = gi == 0 ? e : e - nsrc[0];
-    const SpreadGroup& G = a.grp[gi];
-    const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
-    const int4 c = G.set.cell[sidx];
-    if (!(reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
-          reaches(c.z, oz, t.tz, g.mz, t.ntz)))
+    const int gi = e < nrec[0] ? 0 : 1;
+    const int ri = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], gi == 0 ? e : e - nrec[0]);
+    const int4 cu = a.grp[gi].cull[ri];
+    if (!(reaches<NPTS>(cu.x, ox, t.tx, g.mx, t.ntx) && reaches<NPTS>(cu.y, oy, t.ty, g.my, t.nty) &&
+          reaches<NPTS>(cu.z, oz, t.tz, g.mz, t.ntz)))
       continue;
-    const int k = atomicAdd(&nitem, nsub[gi]);
-    if (k + nsub[gi] <= kItemCap)
-      for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
-    float m;
-    if (G.mode == SRC_VALUE) {
-      const float4 v = G.val[G.set.id[sidx]];
-      m = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z))) * fabsf(G.coef);
-    } else if (G.mode == SRC_RFD) {
-      m = 5.8f * fabsf(G.coef);
-    } else {
-      m = 2.0f * fabsf(G.coef) * fmaxf(g.idx, fmaxf(g.idy, g.idz));
-    }
-    if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
-    lmax = fmaxf(lmax, m);
+    const int k = atomicAdd(&nitem, 1);
+    if (k < kItemCap) items[k] = pack_item(gi, ri);
+    lmax = fmaxf(lmax, __int_as_float(cu.w));
   }
   atomicMax(&maxbits, __float_as_uint(lmax));
   __syncthreads();
@@ -354,31 +408,26 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
   float inv = 0.0f;
   if (M > 0.0f) {
     int e2;
-    frexpf(M * (float)(8 * nall * 2 + 1), &e2);
+    frexpf(M * (float)(8 * nall + 1), &e2);
     e2 = max(e2, -60);
     const float scale = ldexpf(1.0f, 62 - e2);
     inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
-    // pass 2: the compacted list from pass 1, or (when it overflowed) rounds of kItemCap/2
-    // rescanned sources -> candidates -> accumulate
+    // pass 2: the compacted list, or (when it overflowed) rounds of rescanned records
     const int nfirst = nitem;
     const bool fits = nfirst <= kItemCap;
-    for (int r0 = 0; r0 < nall; r0 += kItemCap / 2) {
+    for (int r0 = 0; r0 < nall; r0 += kItemCap) {
       if (!fits) {
         __syncthreads();
         if (threadIdx.x == 0) nitem = 0;
         __syncthreads();
-        const int r1 = min(nall, r0 + kItemCap / 2);  // <= 2 items per source
+        const int r1 = min(nall, r0 + kItemCap);
         for (int e = r0 + threadIdx.x; e < r1; e += blockDim.x) {
-          const int gi = e < nsrc[0] ? 0 : 1;
-          const int src = gi == 0 ? e : e - nsrc[0];
-          const SpreadGroup& G = a.grp[gi];
-          const int sidx = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], src);
-          const int4 c = G.set.cell[sidx];
-          if (reaches(c.x, ox, t.tx, g.mx, t.ntx) && reaches(c.y, oy, t.ty, g.my, t.nty) &&
-              reaches(c.z, oz, t.tz, g.mz, t.ntz)) {
-            const int k = atomicAdd(&nitem, nsub[gi]);
-            for (int sub = 0; sub < nsub[gi]; ++sub) items[k + sub] = pack_item(gi, sub, sidx);
-          }
+          const int gi = e < nrec[0] ? 0 : 1;
+          const int ri = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], gi == 0 ? e : e - nrec[0]);
+          const int4 cu = a.grp[gi].cull[ri];
+          if (reaches<NPTS>(cu.x, ox, t.tx, g.mx, t.ntx) && reaches<NPTS>(cu.y, oy, t.ty, g.my, t.nty) &&
+              reaches<NPTS>(cu.z, oz, t.tz, g.mz, t.ntz))
+            items[atomicAdd(&nitem, 1)] = pack_item(gi, ri);
         }
         __syncthreads();
       }
@@ -387,55 +436,22 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
         const int nc = min(kCandCap, ni - c0);
         if ((int)threadIdx.x < nc) {
           const int it = items[c0 + threadIdx.x];
-          const int gi = it >> 30, sub = (it >> 29) & 1, sidx = it & ((1 << 29) - 1);
-          const SpreadGroup& G = a.grp[gi];
-          const int id = G.set.id[sidx];
-          const int4 c = G.set.cell[sidx];
-          const float4 d = G.set.off[sidx];
-          float3 val;
-          int4 cw = c;
-          float3 dw = make_float3(d.x, d.y, d.z);
-          if (G.mode == SRC_VALUE) {
-            const float4 v = G.val[id];
-            val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
-          } else if (G.mode == SRC_DRIFT) {
-            val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
-          } else {
-            float nrm[4];
-            rfd_normal(a.key, id, nrm);
-            const float sg = sub == 0 ? 1.0f : -1.0f;
-            val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
-            dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
-            renorm_axis(cw.x, dw.x, g.dx, g.idx);
-            renorm_axis(cw.y, dw.y, g.dy, g.idy);
-            renorm_axis(cw.z, dw.z, g.dz, g.idz);
-          }
-          SrcWeights w;
-          source_weights<NPTS>(cw, make_float3(dw.x * g.idx, dw.y * g.idy, dw.z * g.idz), w, G.mode == SRC_DRIFT);
-          if (G.mode == SRC_RFD && G.rfd_quirk) {
-            // nodes from the displaced position, weights at the undisplaced one
-            const int cc[3] = {c.x, c.y, c.z};
-            const float ff[3] = {d.x * g.idx, d.y * g.idy, d.z * g.idz};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-              axis_weights_at<NPTS>(w.cen[ax].base, cc[ax], ff[ax], 0, w.cen[ax]);
-              axis_weights_at<NPTS>(w.stg[ax].base, cc[ax], ff[ax], 1, w.stg[ax]);
-            }
-          }
+          const int gi = it >> 30, ri = it & ((1 << 30) - 1);
+          const SrcRec* R = a.grp[gi].rec + ri;
           Cand& C = cand[threadIdx.x];
 #pragma unroll
           for (int ax = 0; ax < 3; ++ax) {
-            C.base[0][ax] = w.cen[ax].base;
-            C.base[1][ax] = w.stg[ax].base;
+            C.base[0][ax] = R->base[0][ax];
+            C.base[1][ax] = R->base[1][ax];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-              C.w[0][ax][q] = w.cen[ax].w[q];
-              C.w[1][ax][q] = w.stg[ax].w[q];
+              C.w[0][ax][q] = R->w[0][ax][q];
+              C.w[1][ax][q] = R->w[1][ax][q];
             }
           }
-          C.val[0] = val.x * scale;
-          C.val[1] = val.y * scale;
-          C.val[2] = val.z * scale;
+          C.val[0] = R->val[0] * scale;
+          C.val[1] = R->val[1] * scale;
+          C.val[2] = R->val[2] * scale;
         }
         __syncthreads();
         // one work item per (candidate, component, z-row): NPTS x NPTS nodes
@@ -604,6 +620,12 @@ void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
 }
 void launch_spread(const SpreadArgs& a, hipStream_t s) {
+  for (int gi = 0; gi < a.ngroups; ++gi) {
+    const int nrec = a.grp[gi].nsrc * (a.grp[gi].mode == SRC_RFD ? 2 : 1);
+    if (nrec <= 0) continue;
+    if (a.kernel_points == 4) hipLaunchKernelGGL(k_src_prep<4>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
+    else hipLaunchKernelGGL(k_src_prep<3>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
+  }
   if (a.kernel_points == 4) hipLaunchKernelGGL(k_spread<4>, dim3(a.t.n), dim3(256), 0, s, a);
   else hipLaunchKernelGGL(k_spread<3>, dim3(a.t.n), dim3(256), 0, s, a);
 }
