@@ -337,6 +337,8 @@ void run_spread(fhd_ctx* c, SpreadArgs& sa) {
     sa.grp[k].rec = c->rec[k];
     sa.grp[k].cull = c->cull[k];
   }
+  launch_src_prep(sa, c->s);
+  mark(c, "src_prep");
   launch_spread(sa, c->s);
   mark(c, "spread");
 }

@@ -59,14 +59,14 @@ struct SortedSet {
 
 // Source groups for the spreading kernel.
 enum { SRC_VALUE = 0, SRC_RFD = 1, SRC_DRIFT = 2 };
-// Stencil record of one spreading source (144 bytes): centred and face weights per axis, the
-// first node of each stencil, and the value.
+// Stencil record of one spreading source (128 bytes): per axis the centred weights and the face
+// weights multiplied by the value's component along that axis (component a is the only one
+// sampled on the a-faces, so each component's stencil is a product of one face row and two
+// centred rows), then the first node of each stencil.
 struct SrcRec {
-  float w[2][3][4];
-  float val[3];
-  float pad0;
+  float w[2][3][4];     // [0: centred, 1: face x value][axis][k]
   int base[2][3];
-  int pad1[2];
+  int pad[2];
 };
 
 struct SpreadGroup {
@@ -78,7 +78,7 @@ struct SpreadGroup {
   float disp;           // SRC_RFD: displacement magnitude delta/2
   int rfd_quirk;        // SRC_RFD: weights at the undisplaced position
   SrcRec* rec;          // nsrc x (1 or 2) records, in sorted order
-  int4* cull;           // per record: lowest stencil node per axis, max |value| (float bits)
+  int4* cull;           // per record: lowest stencil node per axis, bound on one contribution (float bits)
 };
 
 struct SpreadArgs {
@@ -144,7 +144,8 @@ void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const in
 void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
                       const float4* off, int* sid, int4* scell, float4* soff, hipStream_t s);
 void launch_wca(const WcaArgs& a, hipStream_t s);
-void launch_spread(const SpreadArgs& a, hipStream_t s);
+void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
+void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation
 void launch_interp(const InterpArgs& a, hipStream_t s);
 void launch_zero_f4(float4* p, int n, hipStream_t s);
 

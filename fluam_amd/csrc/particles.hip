@@ -6,10 +6,11 @@
 //  * WCA forces from the particles of the neighbouring tiles (fluam's cell-list pair loop,
 //    src/stokesLimitFunctions.cu:160-653, with the analytic potential instead of the float
 //    texture table of src/initForcesNonBonded.cu:21-66).
-//  * spreading S F: one workgroup per tile accumulates every contribution to its nodes in LDS
-//    as 64-bit fixed-point integers (scale chosen per tile from the largest source).  Integer
-//    addition is associative, so the result is bit-identical whatever order the contributions
-//    arrive in: deterministic without global float atomics.  Replaces the 27-slot scatter +
+//  * spreading S F: one workgroup per tile; each lane owns a column of the tile's nodes and
+//    sums, in registers, the contributions of the candidate sources (those of the neighbouring
+//    tiles whose stencils reach the tile), visited in a fixed order.  Every node's sum is
+//    therefore formed in the same order on every run: deterministic without atomics on the
+//    field.  Replaces the 27-slot scatter +
 //    per-cell gather of src/stokesLimitFunctions.cu:659-1158,1198-1681 and
 //    src/kernelSpreadParticles.cu:3012-4082.
 //  * interpolation J v and the particle position updates (src/stokesLimitFunctions.cu:
@@ -17,6 +18,8 @@
 //
 // Positions are (unwrapped integer cell, FP32 offset from the cell centre), so kernel weights
 // and pair distances are computed from small offsets and exact integer cell differences.
+#include <type_traits>
+
 #include "fhd_device.h"
 #include "fhd_kernels.h"
 
@@ -202,7 +205,8 @@ FHD_DEV void source_weights(int4 c, float3 frac, SrcWeights& w, bool deriv) {
   }
 }
 
-constexpr int kTileMax = 512;  // cells per spreading tile (8 x 8 x 8)
+constexpr int kTileEdge = 8;         // largest spreading tile edge (cells)
+constexpr int kSpreadThreads = 256;
 
 // Fold a node index offset into [0, m) (the offsets here stay within a few boxes of the tile).
 // Offsets passed here lie in (-m, 2m) whenever m >= 8 (wrapped cells, stencils within three
@@ -217,29 +221,6 @@ FHD_DEV int fold(int l, int m) {
   return l;
 }
 
-// Fixed-point image of a float for the LDS accumulators: the value is m 2^e with an integer
-// mantissa m, shifted to an integer (truncating below 2^0); |x| must be below 2^62.  Exact and
-// order-independent when summed as 64-bit integers.
-FHD_DEV unsigned long long f2fixed(float x) {
-  const unsigned b = __float_as_uint(x);
-  const int be = (int)((b >> 23) & 255u);
-  if (be == 0) return 0ull;  // zero or denormal
-  const unsigned long long m = (unsigned long long)((b & 0x7FFFFFu) | 0x800000u);
-  const int e = be - 150;
-  unsigned long long v = e >= 0 ? (m << e) : (e > -24 ? (m >> -e) : 0ull);
-  return (b >> 31) ? (0ull - v) : v;
-}
-
-// Candidate source staged in LDS: per axis the centred and face weights with their node bases,
-// and the value (already multiplied by the fixed-point scale).
-struct Cand {
-  float w[2][3][4];   // [0: centred, 1: face][axis][k]
-  int base[2][3];
-  float val[3];
-};
-
-constexpr int kCandCap = 256;
-
 FHD_DEV float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 
 // Position (cell, offset) of source `sub` of a particle, plus its value.
@@ -247,18 +228,15 @@ FHD_DEV void rfd_normal(const Key& key, int id, float n[4]) {
   draw_normals4(key, (uint32_t)id, FHD_STREAM_RFD, 0, n);
 }
 
-// Candidate item of a spreading workgroup: group (1 bit) and source record index.
-FHD_DEV int pack_item(int gi, int rec) { return (gi << 30) | rec; }
-
-// Flattened source item -> record index (binary search over the neighbour-tile prefix sums
-// held in LDS).
-FHD_DEV int item_slot(const int* pref, const int* st, int n, int src) {
-  int lo = 0, hi = n;  // find k with pref[k] <= src < pref[k+1]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (pref[mid] <= src) lo = mid; else hi = mid;
+// Inclusive prefix sum over the 64 lanes of a wave.
+FHD_DEV int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
   }
-  return st[lo] + (src - pref[lo]);
+  return v;
 }
 
 // Does a stencil whose nodes span [lo, lo + NPTS] (both staggerings) reach the tile [o, o+T)?
@@ -268,8 +246,6 @@ FHD_DEV bool reaches(int lo, int o, int T, int m, int ntiles) {
   const int l0 = fold(lo - o, m);
   return l0 < T || l0 + NPTS >= m;
 }
-
-constexpr int kItemCap = 1024;   // compacted candidate records per round
 
 // One record per source (sorted slot x sub-source): weights, bases, value and the cull data.
 template <int NPTS>
@@ -313,6 +289,7 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int gi) {
       axis_weights_at<NPTS>(w.stg[ax].base, cc[ax], ff[ax], 1, w.stg[ax]);
     }
   }
+  const float vv[3] = {val.x, val.y, val.z};
   SrcRec rec;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
@@ -321,178 +298,249 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int gi) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       rec.w[0][ax][q] = w.cen[ax].w[q];
-      rec.w[1][ax][q] = w.stg[ax].w[q];
+      rec.w[1][ax][q] = w.stg[ax].w[q] * vv[ax];
     }
   }
-  rec.val[0] = val.x;
-  rec.val[1] = val.y;
-  rec.val[2] = val.z;
-  rec.pad0 = 0.0f;
-  rec.pad1[0] = rec.pad1[1] = 0;
+  rec.pad[0] = rec.pad[1] = 0;
   const float4* src4 = reinterpret_cast<const float4*>(&rec);
   float4* dst4 = reinterpret_cast<float4*>(G.rec + r);
 #pragma unroll
   for (int k = 0; k < (int)(sizeof(SrcRec) / 16); ++k) dst4[k] = src4[k];
-  float m = fmaxf(fabsf(val.x), fmaxf(fabsf(val.y), fabsf(val.z)));
-  if (G.mode == SRC_DRIFT) m *= 2.0f;  // derivative weights reach ~1.6
-  if (!(m <= 3.0e38f)) m = 3.0e38f;    // non-finite sources saturate
+  // bound on one contribution: centred weights are at most 1, so the largest face weight x value
+  float m = 0.0f;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(rec.w[1][ax][q]));
+  if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
   G.cull[r] = make_int4(min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
                         min(w.cen[2].base, w.stg[2].base), (int)__float_as_uint(m));
 }
 
-// One workgroup per tile: cull the records of the sources in the neighbouring tiles, stage the
-// candidates in LDS and accumulate every in-tile contribution as a 64-bit fixed-point integer.
-template <int NPTS>
-__global__ __launch_bounds__(256) void k_spread(SpreadArgs a) {
-  __shared__ unsigned long long acc[3 * kTileMax];
-  __shared__ int nb_start[2][27], nb_pref[2][28];
-  __shared__ int n_nb[2];
-  __shared__ unsigned int maxbits;
-  __shared__ Cand cand[kCandCap];
+// Fixed-point image of one contribution (|x| <= 2^30 after scaling): nearest integer, as a
+// 64-bit two's-complement addend.  Integer sums are exact, so the accumulated field does not
+// depend on the order in which the contributions arrive.
+FHD_DEV unsigned long long fixed64(float x) {
+  const int q = __float2int_rn(x);
+  return (unsigned long long)(long long)q;
+}
+
+// Candidate-record reference: group in bit 30, record index below.
+constexpr int kItemCap = 1024;
+
+// Contributions of one source record to component C of the tile accumulator.
+// FAST: the accumulator is the tile plus a halo of NPTS-1 nodes on each side (edge
+// 8 + 2(NPTS-1)), addressed from the stencil's first node, so every node offset is a constant.
+// Otherwise (small boxes): the accumulator is the tile itself and each node is folded
+// periodically and tested.
+template <int NPTS, int C, bool FAST>
+FHD_DEV void add_source(unsigned long long* acc, const SrcRec& r, int ox, int oy, int oz, const GridDev& g,
+                        const Tiles& t, float scale) {
+  constexpr int H = NPTS - 1;
+  constexpr int S = kTileEdge + 2 * H;
+  const int bx = r.base[C == 0][0], by = r.base[C == 1][1], bz = r.base[C == 2][2];
+  const float* wx = r.w[C == 0][0];
+  const float* wy = r.w[C == 1][1];
+  const float* wz = r.w[C == 2][2];
+  if (FAST) {
+    // first stencil node relative to the tile, folded to [-H, m - H)
+    int d[3] = {bx - ox, by - oy, bz - oz};
+    const int m[3] = {g.mx, g.my, g.mz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      if (d[ax] < -H) d[ax] += m[ax];
+      else if (d[ax] >= m[ax] - H) d[ax] -= m[ax];
+    }
+    if (d[0] >= kTileEdge || d[1] >= kTileEdge || d[2] >= kTileEdge) return;  // misses in this staggering
+    unsigned long long* p = acc + ((d[2] + H) * S + (d[1] + H)) * S + (d[0] + H);
+#pragma unroll
+    for (int kz = 0; kz < NPTS; ++kz) {
+      const float wzs = wz[kz] * scale;
+#pragma unroll
+      for (int ky = 0; ky < NPTS; ++ky) {
+        const float wyz = wy[ky] * wzs;
+#pragma unroll
+        for (int kx = 0; kx < NPTS; ++kx) atomicAdd(&p[(kz * S + ky) * S + kx], fixed64(wx[kx] * wyz));
+      }
+    }
+  } else {
+    int lx[NPTS], ly[NPTS];
+#pragma unroll
+    for (int k = 0; k < NPTS; ++k) {
+      lx[k] = fold(bx + k - ox, g.mx);
+      ly[k] = fold(by + k - oy, g.my);
+    }
+#pragma unroll
+    for (int kz = 0; kz < NPTS; ++kz) {
+      const int lz = fold(bz + kz - oz, g.mz);
+      if (lz >= t.tz) continue;
+      const float wzs = wz[kz] * scale;
+#pragma unroll
+      for (int ky = 0; ky < NPTS; ++ky) {
+        if (ly[ky] >= t.ty) continue;
+        const float wyz = wy[ky] * wzs;
+#pragma unroll
+        for (int kx = 0; kx < NPTS; ++kx)
+          if (lx[kx] < t.tx)
+            atomicAdd(&acc[(lz * t.ty + ly[ky]) * t.tx + lx[kx]], fixed64(wx[kx] * wyz));
+      }
+    }
+  }
+}
+
+FHD_DEV void load_rec(const SpreadArgs& a, int ref, SrcRec& r) {
+  const int gi = ref >> 30, ri = ref & ((1 << 30) - 1);
+  const float4* src = reinterpret_cast<const float4*>(a.grp[gi].rec + ri);
+  float4* dst = reinterpret_cast<float4*>(&r);
+#pragma unroll
+  for (int q = 0; q < (int)(sizeof(SrcRec) / 16); ++q) dst[q] = src[q];
+}
+
+// Spreading, one workgroup per tile.  The records of the sources in the neighbouring tiles are
+// culled against the tile into a candidate list; each lane then adds all NPTS^3 contributions
+// of one candidate, one velocity component at a time, into a 64-bit fixed-point LDS
+// accumulator (scale 2^(30 - e) per tile, from the largest face weight x value of its
+// candidates).  Integer addition is associative, so the result is bit-identical whatever order
+// the lanes' atomics land in.  FAST: tiles of 8 x 8 x 8 cells and at least two tiles per axis.
+template <int NPTS, bool FAST>
+__global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
+  constexpr int H = NPTS - 1;
+  constexpr int S = kTileEdge + 2 * H;
+  constexpr int NACC = FAST ? S * S * S : kTileEdge * kTileEdge * kTileEdge;
+  __shared__ unsigned long long acc[NACC];
+  __shared__ int nb_start[64], nb_pref[65];
   __shared__ int items[kItemCap];
   __shared__ int nitem;
+  __shared__ unsigned int maxbits;
   const GridDev& g = a.g;
   const Tiles& t = a.t;
+  const int tid = threadIdx.x;
   const int tile = blockIdx.x;
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
-  const int ncell = t.tx * t.ty * t.tz;
-  for (int e = threadIdx.x; e < 3 * kTileMax; e += blockDim.x) acc[e] = 0ull;
-  // neighbour-tile table: one thread per (group, neighbour) loads its record range in parallel
+  // neighbour-tile table: lane k of wave 0 loads the record range of (group, neighbour) k
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
   nb_range(t.nty, loy, hiy);
   nb_range(t.ntz, loz, hiz);
   const int nx = hix - lox + 1, ny = hiy - loy + 1, nz = hiz - loz + 1;
   const int nnb = nx * ny * nz;
-  if ((int)threadIdx.x < a.ngroups * nnb) {
-    const int gi = threadIdx.x / nnb, k = threadIdx.x - gi * nnb;
-    const int nsub = a.grp[gi].mode == SRC_RFD ? 2 : 1;
-    const int dx = lox + k % nx, dy = loy + (k / nx) % ny, dz = loz + k / (nx * ny);
-    const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
-    const int s0 = a.grp[gi].set.start[nt];
-    nb_start[gi][k] = s0 * nsub;
-    nb_pref[gi][k + 1] = (a.grp[gi].set.start[nt + 1] - s0) * nsub;  // prefix-summed below
-  }
-  if (threadIdx.x == 0) {
-    maxbits = 0u;
-    nitem = 0;
+  const int ntab = a.ngroups * nnb;  // <= 54
+  if (tid < 64) {
+    int cnt = 0;
+    if (tid < ntab) {
+      const int gi = tid / nnb, k = tid - gi * nnb;
+      const int nsub = a.grp[gi].mode == SRC_RFD ? 2 : 1;
+      const int dx = lox + k % nx, dy = loy + (k / nx) % ny, dz = loz + k / (nx * ny);
+      const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
+      const int s0 = a.grp[gi].set.start[nt];
+      nb_start[tid] = s0 * nsub;
+      cnt = (a.grp[gi].set.start[nt + 1] - s0) * nsub;
+    }
+    const int incl = wave_incl_scan(cnt);
+    nb_pref[tid + 1] = incl;
+    if (tid == 0) {
+      nb_pref[0] = 0;
+      nitem = 0;
+      maxbits = 0u;
+    }
   }
   __syncthreads();
-  if ((int)threadIdx.x < a.ngroups) {
-    const int gi = threadIdx.x;
-    nb_pref[gi][0] = 0;
-    for (int k = 0; k < nnb; ++k) nb_pref[gi][k + 1] += nb_pref[gi][k];
-    n_nb[gi] = nnb;
-  }
-  __syncthreads();
-  int nrec[2] = {0, 0};
-  for (int gi = 0; gi < a.ngroups; ++gi) nrec[gi] = nb_pref[gi][n_nb[gi]];
-  const int nall = nrec[0] + nrec[1];
-  // pass 1: cull, compact the candidates into the LDS list (when they fit), largest magnitude
-  float lmax = 0.0f;
-  for (int e = threadIdx.x; e < nall; e += blockDim.x) {
-    const int gi = e < nrec[0] ? 0 : 1;
-    const int ri = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], gi == 0 ? e : e - nrec[0]);
+  const int nall = nb_pref[ntab];
+  // record index e -> reference; returns -1 when the record's stencils miss the tile
+  auto cull = [&](int e, float& mx) -> int {
+    int lo = 0, hi = ntab;  // nb_pref[lo] <= e < nb_pref[lo + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (nb_pref[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int gi = lo >= nnb;
+    const int ri = nb_start[lo] + (e - nb_pref[lo]);
     const int4 cu = a.grp[gi].cull[ri];
     if (!(reaches<NPTS>(cu.x, ox, t.tx, g.mx, t.ntx) && reaches<NPTS>(cu.y, oy, t.ty, g.my, t.nty) &&
           reaches<NPTS>(cu.z, oz, t.tz, g.mz, t.ntz)))
-      continue;
-    const int k = atomicAdd(&nitem, 1);
-    if (k < kItemCap) items[k] = pack_item(gi, ri);
-    lmax = fmaxf(lmax, __int_as_float(cu.w));
+      return -1;
+    mx = fmaxf(mx, __int_as_float(cu.w));
+    return (gi << 30) | ri;
+  };
+  // pass 1: cull every record, keep the first kItemCap candidates, largest magnitude
+  float lmax = 0.0f;
+  for (int e = tid; e < nall; e += kSpreadThreads) {
+    const int ref = cull(e, lmax);
+    if (ref >= 0) {
+      const int k = atomicAdd(&nitem, 1);
+      if (k < kItemCap) items[k] = ref;
+    }
   }
   atomicMax(&maxbits, __float_as_uint(lmax));
   __syncthreads();
+  const int ntot = nitem;
+  const bool fits = ntot <= kItemCap;
   const float M = __uint_as_float(maxbits);
-  float inv = 0.0f;
-  if (M > 0.0f) {
-    int e2;
-    frexpf(M * (float)(8 * nall + 1), &e2);
-    e2 = max(e2, -60);
-    const float scale = ldexpf(1.0f, 62 - e2);
-    inv = ldexpf(1.0f, e2 - 62) * a.out_coef;
-    // pass 2: the compacted list, or (when it overflowed) rounds of rescanned records
-    const int nfirst = nitem;
-    const bool fits = nfirst <= kItemCap;
-    for (int r0 = 0; r0 < nall; r0 += kItemCap) {
-      if (!fits) {
-        __syncthreads();
-        if (threadIdx.x == 0) nitem = 0;
-        __syncthreads();
-        const int r1 = min(nall, r0 + kItemCap);
-        for (int e = r0 + threadIdx.x; e < r1; e += blockDim.x) {
-          const int gi = e < nrec[0] ? 0 : 1;
-          const int ri = item_slot(nb_pref[gi], nb_start[gi], n_nb[gi], gi == 0 ? e : e - nrec[0]);
-          const int4 cu = a.grp[gi].cull[ri];
-          if (reaches<NPTS>(cu.x, ox, t.tx, g.mx, t.ntx) && reaches<NPTS>(cu.y, oy, t.ty, g.my, t.nty) &&
-              reaches<NPTS>(cu.z, oz, t.tz, g.mz, t.ntz))
-            items[atomicAdd(&nitem, 1)] = pack_item(gi, ri);
-        }
-        __syncthreads();
+  int e2 = 0;
+  if (M > 0.0f) frexpf(M, &e2);  // M < 2^e2
+  e2 = max(e2, -90);
+  const float scale = ldexpf(1.0f, 30 - e2);
+  const float inv = ldexpf(1.0f, e2 - 30) * a.out_coef;
+  // the first candidate of each lane stays in registers across the three components
+  SrcRec r0;
+  const bool have0 = fits && tid < ntot;
+  if (have0) load_rec(a, items[tid], r0);
+
+  auto add = [&](auto cc, const SrcRec& r) {
+    constexpr int C = decltype(cc)::value;
+    add_source<NPTS, C, FAST>(acc, r, ox, oy, oz, g, t, scale);
+  };
+  auto component = [&](auto cc) {
+    constexpr int C = decltype(cc)::value;
+    for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
+    __syncthreads();
+    if (fits) {
+      if (have0) add(cc, r0);
+      for (int c = tid + kSpreadThreads; c < ntot; c += kSpreadThreads) {
+        SrcRec r;
+        load_rec(a, items[c], r);
+        add(cc, r);
       }
-      const int ni = fits ? nfirst : nitem;
-      for (int c0 = 0; c0 < ni; c0 += kCandCap) {
-        const int nc = min(kCandCap, ni - c0);
-        if ((int)threadIdx.x < nc) {
-          const int it = items[c0 + threadIdx.x];
-          const int gi = it >> 30, ri = it & ((1 << 30) - 1);
-          const SrcRec* R = a.grp[gi].rec + ri;
-          Cand& C = cand[threadIdx.x];
-#pragma unroll
-          for (int ax = 0; ax < 3; ++ax) {
-            C.base[0][ax] = R->base[0][ax];
-            C.base[1][ax] = R->base[1][ax];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              C.w[0][ax][q] = R->w[0][ax][q];
-              C.w[1][ax][q] = R->w[1][ax][q];
-            }
-          }
-          C.val[0] = R->val[0] * scale;
-          C.val[1] = R->val[1] * scale;
-          C.val[2] = R->val[2] * scale;
+    } else {
+      // more candidates than the list holds: rounds over disjoint record ranges, re-culled
+      for (int b0 = 0; b0 < nall; b0 += kItemCap) {
+        __syncthreads();
+        if (tid == 0) nitem = 0;
+        __syncthreads();
+        float dummy = 0.0f;
+        for (int e = b0 + tid; e < min(nall, b0 + kItemCap); e += kSpreadThreads) {
+          const int ref = cull(e, dummy);
+          if (ref >= 0) items[atomicAdd(&nitem, 1)] = ref;
         }
         __syncthreads();
-        // one work item per (candidate, component, z-row): NPTS x NPTS nodes
-        const int nwork = nc * 3 * NPTS;
-        for (int li = threadIdx.x; li < nwork; li += blockDim.x) {
-          const int cidx = li / (3 * NPTS);
-          const int r = li - cidx * (3 * NPTS);
-          const int comp = r / NPTS, kz = r - comp * NPTS;
-          const Cand& C = cand[cidx];
-          const float vc = C.val[comp];
-          if (vc == 0.0f) continue;
-          const int sx = comp == 0, sy = comp == 1, sz = comp == 2;
-          const int lz = fold(C.base[sz][2] + kz - oz, g.mz);
-          if (lz >= t.tz) continue;
-          const float wz = C.w[sz][2][kz] * vc;
-          const int by = C.base[sy][1], bx = C.base[sx][0];
-          int lxs[NPTS];
-#pragma unroll
-          for (int kx = 0; kx < NPTS; ++kx) lxs[kx] = fold(bx + kx - ox, g.mx);
-#pragma unroll
-          for (int ky = 0; ky < NPTS; ++ky) {
-            const int ly = fold(by + ky - oy, g.my);
-            if (ly >= t.ty) continue;
-            const float wyz = C.w[sy][1][ky] * wz;
-            unsigned long long* row = &acc[comp * kTileMax + t.tx * (ly + t.ty * lz)];
-#pragma unroll
-            for (int kx = 0; kx < NPTS; ++kx)
-              if (lxs[kx] < t.tx) atomicAdd(&row[lxs[kx]], f2fixed(C.w[sx][0][kx] * wyz));
-          }
+        for (int c = tid; c < nitem; c += kSpreadThreads) {
+          SrcRec r;
+          load_rec(a, items[c], r);
+          add(cc, r);
         }
-        __syncthreads();
       }
-      if (fits) break;
     }
-  }
-  for (int e = threadIdx.x; e < 3 * ncell; e += blockDim.x) {
-    const int comp = e / ncell, l = e - comp * ncell;
-    const int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
-    const size_t gi = (size_t)comp * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-    a.out[gi] = (float)(long long)acc[comp * kTileMax + l] * inv;
-  }
+    __syncthreads();
+    // interior nodes -> field
+    const int ncell = t.tx * t.ty * t.tz;
+    for (int l = tid; l < ncell; l += kSpreadThreads) {
+      int lx, ly, lz, ai;
+      if (FAST) {
+        lx = l & 7; ly = (l >> 3) & 7; lz = l >> 6;
+        ai = ((lz + H) * S + (ly + H)) * S + (lx + H);
+      } else {
+        lx = l % t.tx; ly = (l / t.tx) % t.ty; lz = l / (t.tx * t.ty);
+        ai = l;
+      }
+      const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
+      a.out[gi] = (float)(long long)acc[ai] * inv;
+    }
+    __syncthreads();
+  };
+  component(std::integral_constant<int, 0>());
+  component(std::integral_constant<int, 1>());
+  component(std::integral_constant<int, 2>());
 }
 
 // Sum over the NPTS x NPTS nodes of one z-row (kz) of a particle's stencil.
@@ -619,15 +667,25 @@ void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const 
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
 }
-void launch_spread(const SpreadArgs& a, hipStream_t s) {
+void launch_src_prep(const SpreadArgs& a, hipStream_t s) {
   for (int gi = 0; gi < a.ngroups; ++gi) {
     const int nrec = a.grp[gi].nsrc * (a.grp[gi].mode == SRC_RFD ? 2 : 1);
     if (nrec <= 0) continue;
     if (a.kernel_points == 4) hipLaunchKernelGGL(k_src_prep<4>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
     else hipLaunchKernelGGL(k_src_prep<3>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
   }
-  if (a.kernel_points == 4) hipLaunchKernelGGL(k_spread<4>, dim3(a.t.n), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_spread<3>, dim3(a.t.n), dim3(256), 0, s, a);
+}
+void launch_spread(const SpreadArgs& a, hipStream_t s) {
+  const bool fast = a.t.tx == kTileEdge && a.t.ty == kTileEdge && a.t.tz == kTileEdge && a.t.ntx >= 2 &&
+                    a.t.nty >= 2 && a.t.ntz >= 2;
+  const dim3 grid(a.t.n), block(kSpreadThreads);
+  if (a.kernel_points == 4) {
+    if (fast) hipLaunchKernelGGL((k_spread<4, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_spread<4, false>), grid, block, 0, s, a);
+  } else {
+    if (fast) hipLaunchKernelGGL((k_spread<3, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((k_spread<3, false>), grid, block, 0, s, a);
+  }
 }
 void launch_interp(const InterpArgs& a, hipStream_t s) {
   if (a.np <= 0) return;
