@@ -18,8 +18,6 @@
 //
 // Positions are (unwrapped integer cell, FP32 offset from the cell centre), so kernel weights
 // and pair distances are computed from small offsets and exact integer cell differences.
-#include <type_traits>
-
 #include "fhd_device.h"
 #include "fhd_kernels.h"
 
@@ -313,8 +311,9 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int gi) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(rec.w[1][ax][q]));
   if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
-  G.cull[r] = make_int4(min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
-                        min(w.cen[2].base, w.stg[2].base), (int)__float_as_uint(m));
+  const int lo[3] = {min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
+                     min(w.cen[2].base, w.stg[2].base)};
+  G.cull[r] = make_int4(lo[0], lo[1], lo[2], (int)__float_as_uint(m));
 }
 
 // Fixed-point image of one contribution (|x| <= 2^30 after scaling): nearest integer, as a
@@ -327,86 +326,95 @@ FHD_DEV unsigned long long fixed64(float x) {
 
 // Candidate-record reference: group in bit 30, record index below.
 constexpr int kItemCap = 1024;
+// Accumulator z-row stride (64-bit words): the 8 x 8 plane plus 2, so the rows that the four
+// lanes of one candidate update start in different LDS banks.
+constexpr int kRowStride = kTileEdge * kTileEdge + 2;
 
-// Contributions of one source record to component C of the tile accumulator.
-// FAST: the accumulator is the tile plus a halo of NPTS-1 nodes on each side (edge
-// 8 + 2(NPTS-1)), addressed from the stencil's first node, so every node offset is a constant.
-// Otherwise (small boxes): the accumulator is the tile itself and each node is folded
-// periodically and tested.
-template <int NPTS, int C, bool FAST>
-FHD_DEV void add_source(unsigned long long* acc, const SrcRec& r, int ox, int oy, int oz, const GridDev& g,
-                        const Tiles& t, float scale) {
+// Contributions of stencil row kz (nodes base_z + kz) of one source record to the three
+// components of the tile accumulator acc[component][z row][y][x] (x, y: the 8 x 8 tile columns).
+// FAST: the z rows carry a halo of NPTS-1 rows on each side, so the row index needs no test;
+// columns outside the tile are skipped.  Otherwise (small boxes) every node is folded
+// periodically and tested against the tile.
+template <int NPTS, bool FAST>
+FHD_DEV void add_source_row(unsigned long long* acc, const SrcRec* __restrict__ R, int kz, int ox, int oy, int oz,
+                            const GridDev& g, const Tiles& t, float scale) {
   constexpr int H = NPTS - 1;
-  constexpr int S = kTileEdge + 2 * H;
-  const int bx = r.base[C == 0][0], by = r.base[C == 1][1], bz = r.base[C == 2][2];
-  const float* wx = r.w[C == 0][0];
-  const float* wy = r.w[C == 1][1];
-  const float* wz = r.w[C == 2][2];
-  if (FAST) {
-    // first stencil node relative to the tile, folded to [-H, m - H)
-    int d[3] = {bx - ox, by - oy, bz - oz};
-    const int m[3] = {g.mx, g.my, g.mz};
+  constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
+  constexpr int PL = kRowStride;
+  const float4 cx = *reinterpret_cast<const float4*>(R->w[0][0]);
+  const float4 cy = *reinterpret_cast<const float4*>(R->w[0][1]);
+  const float4 fx = *reinterpret_cast<const float4*>(R->w[1][0]);
+  const float4 fy = *reinterpret_cast<const float4*>(R->w[1][1]);
+  const float cz = R->w[0][2][kz], fz = R->w[1][2][kz];
+  const int4 b0 = *reinterpret_cast<const int4*>(&R->base[0][0]);  // cen x, y, z, face x
+  const int2 b1 = *reinterpret_cast<const int2*>(&R->base[1][1]);  // face y, z
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-      if (d[ax] < -H) d[ax] += m[ax];
-      else if (d[ax] >= m[ax] - H) d[ax] -= m[ax];
-    }
-    if (d[0] >= kTileEdge || d[1] >= kTileEdge || d[2] >= kTileEdge) return;  // misses in this staggering
-    unsigned long long* p = acc + ((d[2] + H) * S + (d[1] + H)) * S + (d[0] + H);
-#pragma unroll
-    for (int kz = 0; kz < NPTS; ++kz) {
-      const float wzs = wz[kz] * scale;
-#pragma unroll
-      for (int ky = 0; ky < NPTS; ++ky) {
-        const float wyz = wy[ky] * wzs;
-#pragma unroll
-        for (int kx = 0; kx < NPTS; ++kx) atomicAdd(&p[(kz * S + ky) * S + kx], fixed64(wx[kx] * wyz));
-      }
-    }
-  } else {
-    int lx[NPTS], ly[NPTS];
-#pragma unroll
-    for (int k = 0; k < NPTS; ++k) {
-      lx[k] = fold(bx + k - ox, g.mx);
-      ly[k] = fold(by + k - oy, g.my);
-    }
-#pragma unroll
-    for (int kz = 0; kz < NPTS; ++kz) {
-      const int lz = fold(bz + kz - oz, g.mz);
-      if (lz >= t.tz) continue;
-      const float wzs = wz[kz] * scale;
+  for (int C = 0; C < 3; ++C) {
+    const int bx = C == 0 ? b0.w : b0.x, by = C == 1 ? b1.x : b0.y, bz = C == 2 ? b1.y : b0.z;
+    const float4 w4x = C == 0 ? fx : cx, w4y = C == 1 ? fy : cy;
+    const float wx[4] = {w4x.x, w4x.y, w4x.z, w4x.w}, wy[4] = {w4y.x, w4y.y, w4y.z, w4y.w};
+    const float wzs = (C == 2 ? fz : cz) * scale;
+    if (FAST) {
+      // first stencil node relative to the tile, folded to [-H, m - H)
+      int dx = bx - ox, dy = by - oy, dz = bz - oz;
+      dx += dx < -H ? g.mx : (dx >= g.mx - H ? -g.mx : 0);
+      dy += dy < -H ? g.my : (dy >= g.my - H ? -g.my : 0);
+      dz += dz < -H ? g.mz : (dz >= g.mz - H ? -g.mz : 0);
+      if (dx >= kTileEdge || dy >= kTileEdge || dz >= kTileEdge) continue;  // misses in this staggering
+      unsigned long long* p = acc + (C * ZS + dz + kz + H) * PL + dy * kTileEdge + dx;
 #pragma unroll
       for (int ky = 0; ky < NPTS; ++ky) {
-        if (ly[ky] >= t.ty) continue;
+        if ((unsigned)(dy + ky) >= (unsigned)kTileEdge) continue;
         const float wyz = wy[ky] * wzs;
 #pragma unroll
         for (int kx = 0; kx < NPTS; ++kx)
-          if (lx[kx] < t.tx)
-            atomicAdd(&acc[(lz * t.ty + ly[ky]) * t.tx + lx[kx]], fixed64(wx[kx] * wyz));
+          if ((unsigned)(dx + kx) < (unsigned)kTileEdge)
+            atomicAdd(&p[ky * kTileEdge + kx], fixed64(wx[kx] * wyz));
+      }
+    } else {
+      const int lz = fold(bz + kz - oz, g.mz);
+      if (lz >= t.tz) continue;
+      unsigned long long* row = acc + (C * ZS + lz) * PL;
+#pragma unroll
+      for (int ky = 0; ky < NPTS; ++ky) {
+        const int ly = fold(by + ky - oy, g.my);
+        if (ly >= t.ty) continue;
+        const float wyz = wy[ky] * wzs;
+#pragma unroll
+        for (int kx = 0; kx < NPTS; ++kx) {
+          const int lx = fold(bx + kx - ox, g.mx);
+          if (lx < t.tx) atomicAdd(&row[ly * kTileEdge + lx], fixed64(wx[kx] * wyz));
+        }
       }
     }
   }
 }
 
-FHD_DEV void load_rec(const SpreadArgs& a, int ref, SrcRec& r) {
-  const int gi = ref >> 30, ri = ref & ((1 << 30) - 1);
-  const float4* src = reinterpret_cast<const float4*>(a.grp[gi].rec + ri);
-  float4* dst = reinterpret_cast<float4*>(&r);
-#pragma unroll
-  for (int q = 0; q < (int)(sizeof(SrcRec) / 16); ++q) dst[q] = src[q];
+FHD_DEV const SrcRec* rec_of(const SpreadArgs& a, int ref) {
+  return a.grp[ref >> 30].rec + (ref & ((1 << 30) - 1));
 }
 
 // Spreading, one workgroup per tile.  The records of the sources in the neighbouring tiles are
-// culled against the tile into a candidate list; each lane then adds all NPTS^3 contributions
-// of one candidate, one velocity component at a time, into a 64-bit fixed-point LDS
+// culled against the tile into a candidate list; four lanes take each candidate, one stencil
+// z-row each, and add its contributions to the three components into a 64-bit fixed-point LDS
 // accumulator (scale 2^(30 - e) per tile, from the largest face weight x value of its
 // candidates).  Integer addition is associative, so the result is bit-identical whatever order
 // the lanes' atomics land in.  FAST: tiles of 8 x 8 x 8 cells and at least two tiles per axis.
+// Tile of workgroup b.  Workgroups are dealt round-robin to the eight XCDs (each with its own
+// L2); giving every XCD a contiguous block of tiles, x fastest, keeps the neighbouring tiles
+// that share candidate records and output cache lines on the same L2.
+FHD_DEV int tile_of_block(int b, int ntiles) {
+  constexpr int kXcd = 8;
+  if (ntiles % kXcd) return b;
+  return (b % kXcd) * (ntiles / kXcd) + b / kXcd;
+}
+
 template <int NPTS, bool FAST>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   constexpr int H = NPTS - 1;
-  constexpr int S = kTileEdge + 2 * H;
-  constexpr int NACC = FAST ? S * S * S : kTileEdge * kTileEdge * kTileEdge;
+  constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
+  constexpr int NACC = 3 * ZS * kRowStride;
+  constexpr int LPC = 4;  // lanes per candidate (one stencil z-row each)
   __shared__ unsigned long long acc[NACC];
   __shared__ int nb_start[64], nb_pref[65];
   __shared__ int items[kItemCap];
@@ -415,9 +423,10 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x;
-  const int tile = blockIdx.x;
+  const int tile = tile_of_block(blockIdx.x, t.n);
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
+  for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
   // neighbour-tile table: lane k of wave 0 loads the record range of (group, neighbour) k
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
@@ -447,7 +456,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   }
   __syncthreads();
   const int nall = nb_pref[ntab];
-  // record index e -> reference; returns -1 when the record's stencils miss the tile
+  // record index e -> reference; -1 when the record's stencils miss the tile
   auto cull = [&](int e, float& mx) -> int {
     int lo = 0, hi = ntab;  // nb_pref[lo] <= e < nb_pref[lo + 1]
     while (hi - lo > 1) {
@@ -475,72 +484,44 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   atomicMax(&maxbits, __float_as_uint(lmax));
   __syncthreads();
   const int ntot = nitem;
-  const bool fits = ntot <= kItemCap;
   const float M = __uint_as_float(maxbits);
   int e2 = 0;
   if (M > 0.0f) frexpf(M, &e2);  // M < 2^e2
   e2 = max(e2, -90);
   const float scale = ldexpf(1.0f, 30 - e2);
   const float inv = ldexpf(1.0f, e2 - 30) * a.out_coef;
-  // the first candidate of each lane stays in registers across the three components
-  SrcRec r0;
-  const bool have0 = fits && tid < ntot;
-  if (have0) load_rec(a, items[tid], r0);
-
-  auto add = [&](auto cc, const SrcRec& r) {
-    constexpr int C = decltype(cc)::value;
-    add_source<NPTS, C, FAST>(acc, r, ox, oy, oz, g, t, scale);
-  };
-  auto component = [&](auto cc) {
-    constexpr int C = decltype(cc)::value;
-    for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
-    __syncthreads();
-    if (fits) {
-      if (have0) add(cc, r0);
-      for (int c = tid + kSpreadThreads; c < ntot; c += kSpreadThreads) {
-        SrcRec r;
-        load_rec(a, items[c], r);
-        add(cc, r);
+  const int kz = tid % LPC;
+  if (ntot <= kItemCap) {
+    if (kz < NPTS)
+      for (int c = tid / LPC; c < ntot; c += kSpreadThreads / LPC)
+        add_source_row<NPTS, FAST>(acc, rec_of(a, items[c]), kz, ox, oy, oz, g, t, scale);
+  } else {
+    // more candidates than the list holds: rounds over disjoint record ranges, re-culled
+    for (int b0 = 0; b0 < nall; b0 += kItemCap) {
+      __syncthreads();
+      if (tid == 0) nitem = 0;
+      __syncthreads();
+      float dummy = 0.0f;
+      for (int e = b0 + tid; e < min(nall, b0 + kItemCap); e += kSpreadThreads) {
+        const int ref = cull(e, dummy);
+        if (ref >= 0) items[atomicAdd(&nitem, 1)] = ref;
       }
-    } else {
-      // more candidates than the list holds: rounds over disjoint record ranges, re-culled
-      for (int b0 = 0; b0 < nall; b0 += kItemCap) {
-        __syncthreads();
-        if (tid == 0) nitem = 0;
-        __syncthreads();
-        float dummy = 0.0f;
-        for (int e = b0 + tid; e < min(nall, b0 + kItemCap); e += kSpreadThreads) {
-          const int ref = cull(e, dummy);
-          if (ref >= 0) items[atomicAdd(&nitem, 1)] = ref;
-        }
-        __syncthreads();
-        for (int c = tid; c < nitem; c += kSpreadThreads) {
-          SrcRec r;
-          load_rec(a, items[c], r);
-          add(cc, r);
-        }
-      }
+      __syncthreads();
+      if (kz < NPTS)
+        for (int c = tid / LPC; c < nitem; c += kSpreadThreads / LPC)
+          add_source_row<NPTS, FAST>(acc, rec_of(a, items[c]), kz, ox, oy, oz, g, t, scale);
     }
-    __syncthreads();
-    // interior nodes -> field
-    const int ncell = t.tx * t.ty * t.tz;
-    for (int l = tid; l < ncell; l += kSpreadThreads) {
-      int lx, ly, lz, ai;
-      if (FAST) {
-        lx = l & 7; ly = (l >> 3) & 7; lz = l >> 6;
-        ai = ((lz + H) * S + (ly + H)) * S + (lx + H);
-      } else {
-        lx = l % t.tx; ly = (l / t.tx) % t.ty; lz = l / (t.tx * t.ty);
-        ai = l;
-      }
-      const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-      a.out[gi] = (float)(long long)acc[ai] * inv;
-    }
-    __syncthreads();
-  };
-  component(std::integral_constant<int, 0>());
-  component(std::integral_constant<int, 1>());
-  component(std::integral_constant<int, 2>());
+  }
+  __syncthreads();
+  // interior nodes -> field
+  const int ncell = t.tx * t.ty * t.tz;
+  for (int e = tid; e < 3 * ncell; e += kSpreadThreads) {
+    const int C = e / ncell, l = e - C * ncell;
+    const int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
+    const int ai = (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
+    const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
+    a.out[gi] = (float)(long long)acc[ai] * inv;
+  }
 }
 
 // Sum over the NPTS x NPTS nodes of one z-row (kz) of a particle's stencil.

@@ -157,6 +157,30 @@ def test_stokes_extra_mobility():
     _particle_parity(p, 20)
 
 
+def test_stokes_crowded_tile_overflow():
+    """1,500 particles inside one 8^3-cell tile: the tile's candidate list overflows its capacity
+    (1,024), so spreading takes the re-culling path for that tile and its neighbours."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=1500, kernel_points=4, nonbonded=0,
+                    temperature=1e-3, dt=0.05, seed=27, initial_positions=0, **WCA)
+    _announce("stokes_crowded_tile", p)
+    rng = np.random.default_rng(27)
+    pts = rng.uniform(-7.9, -0.1, (1500, 3))  # cells 8..15 of the box [-16, 16)
+    q = pts.T.ravel()
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.set_particles(q)
+    o.set_particles(q)
+    q0 = o.particles()
+    g.step(5)
+    g.sync()
+    o.step(5)
+    qg, qo = g.particles(), o.particles()
+    assert np.isfinite(qg).all()
+    assert rel(qg, qo) < 1e-5
+    assert rel(qg - q0, qo - q0) < 1e-3
+    assert rel(g.velocity(), o.velocity()) < 1e-4
+
+
 def test_stokes_deterministic_replay():
     p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=2000, kernel_points=4, temperature=1e-3,
                     dt=0.05, seed=26, initial_positions=0, **WCA)
