@@ -42,6 +42,19 @@ def kernel_bytes(name, mx, my, mz, npart):
     return None
 
 
+def hbm_traffic(name):
+    """Measured HBM bytes per launch of a kernel group, from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by its plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("groups", {}).get(name)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(args):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -124,7 +137,7 @@ def main():
                                "4-point Peskin kernel, WCA sigma=1 rc=2^(1/6) eps=kT, kT=1e-3, eta=1, dt=0.05",
                    "cells": [mx, my, mz], "particles": npart},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic(dom),
                      "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},
         "step_roofline": {"algorithmic_bytes": algorithmic_bytes(N, npart), "achieved_GBs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
