@@ -3,16 +3,14 @@
 //  * binning: particles are bucketed by an 8x8x8-cell tile of the fluid grid (counting sort),
 //    then each bucket is ordered by particle index, so every later pass visits particles in a
 //    fixed order whatever order the atomics placed them in.
-//  * WCA forces from the particles of the neighbouring tiles (fluam's cell-list pair loop,
-//    src/stokesLimitFunctions.cu:160-653, with the analytic potential instead of the float
-//    texture table of src/initForcesNonBonded.cu:21-66).
-//  * spreading S F: one workgroup per tile; each lane owns a column of the tile's nodes and
-//    sums, in registers, the contributions of the candidate sources (those of the neighbouring
-//    tiles whose stencils reach the tile), visited in a fixed order.  Every node's sum is
-//    therefore formed in the same order on every run: deterministic without atomics on the
-//    field.  Replaces the 27-slot scatter +
-//    per-cell gather of src/stokesLimitFunctions.cu:659-1158,1198-1681 and
-//    src/kernelSpreadParticles.cu:3012-4082.
+//  * WCA forces, one workgroup per tile, from the particles of the neighbouring tiles staged
+//    in LDS (fluam's cell-list pair loop, src/stokesLimitFunctions.cu:160-653, with the
+//    analytic potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).
+//  * spreading S F: one workgroup per tile adds the contributions of every source whose
+//    stencil reaches the tile into a 64-bit fixed-point LDS accumulator.  Integer addition is
+//    associative, so the field is bit-identical whatever order the contributions arrive in:
+//    deterministic without float atomics.  Replaces the 27-slot scatter + per-cell gather of
+//    src/stokesLimitFunctions.cu:659-1158,1198-1681 and src/kernelSpreadParticles.cu:3012-4082.
 //  * interpolation J v and the particle position updates (src/stokesLimitFunctions.cu:
 //    2479-2892,3513-3894; src/quasiNeutrallyBuoyantFunctions.cu:2448-2853,4481-4862).
 //
@@ -102,79 +100,158 @@ FHD_DEV void nb_range(int nt, int& lo, int& hi) {
   hi = nt >= 2 ? 1 : 0;
 }
 
-// WCA: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for r < rc.
-FHD_DEV void wca_partial(const WcaArgs& a, int s, int q, float& fx, float& fy, float& fz);
-
-// Four lanes per particle: lane q takes every fourth candidate of each neighbouring tile and
-// the partial forces are combined by a fixed xor-shuffle tree, so the sum order is fixed.
-__global__ __launch_bounds__(256) void k_wca(WcaArgs a) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gid >> 2, q = gid & 3;
-  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-  if (s < a.np) wca_partial(a, s, q, fx, fy, fz);
-  fx += __shfl_xor(fx, 1);
-  fy += __shfl_xor(fy, 1);
-  fz += __shfl_xor(fz, 1);
-  fx += __shfl_xor(fx, 2);
-  fy += __shfl_xor(fy, 2);
-  fz += __shfl_xor(fz, 2);
-  if (s < a.np && q == 0) a.F[a.set.id[s]] = make_float4(fx, fy, fz, 0.0f);
+// Work item (tile, or block of sorted particles) of workgroup b out of n.  Workgroups are dealt
+// round-robin to the eight XCDs, each with its own L2; giving every XCD a contiguous range of
+// items keeps neighbouring tiles, which share particles and field cache lines, on one L2.
+FHD_DEV int xcd_item(int b, int n) {
+  constexpr int kXcd = 8;
+  if (n % kXcd) return b;
+  return (b % kXcd) * (n / kXcd) + b / kXcd;
 }
 
-FHD_DEV void wca_partial(const WcaArgs& a, int s, int q, float& fx, float& fy, float& fz) {
+// Inclusive prefix sum over the 64 lanes of a wave.
+FHD_DEV int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+constexpr int kWcaThreads = 256;
+constexpr int kWcaSlices = 16;  // lanes sharing one particle's candidates
+
+// WCA, one workgroup per tile: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for
+// r < rc (fluam's cell-list pair loop, src/stokesLimitFunctions.cu:160-653, with the analytic
+// potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).  The
+// particles of the neighbouring tiles that lie within rc of the tile are staged in LDS in
+// sorted order (a deterministic compaction); 16 lanes share each of the tile's particles,
+// lane l taking candidates l, l + 16, ..., and the partial forces are combined by a fixed
+// xor-shuffle tree, so every sum is formed in the same order on every run.
+__global__ __launch_bounds__(kWcaThreads) void k_wca(WcaArgs a) {
+  __shared__ int4 ccell[kWcaThreads];   // w: sorted slot
+  __shared__ float4 coff[kWcaThreads];
+  __shared__ int nb_start[32], nb_pref[33];
+  __shared__ int wsum[kWcaThreads / 64];
   const GridDev& g = a.g;
   const Tiles& t = a.t;
-  const int4 ci = a.set.cell[s];
-  const float4 di = a.set.off[s];
-  const int tx = ci.x / t.tx, ty = ci.y / t.ty, tz = ci.z / t.tz;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tile = xcd_item(blockIdx.x, t.n);
+  const int s0 = a.set.start[tile], nown = a.set.start[tile + 1] - s0;
+  if (nown == 0) return;  // uniform: no barrier is skipped by part of the group
+  const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
+  const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
   nb_range(t.nty, loy, hiy);
   nb_range(t.ntz, loz, hiz);
-  // with three or more tiles along an axis, visit a neighbouring tile only when the particle
-  // lies within the cutoff of that face
-  const float rc = sqrtf(a.rc2);
-  if (t.ntx >= 3) {
-    const float lo = ((float)(ci.x - tx * t.tx) + 0.5f) * g.dx + di.x;
-    if (lo >= rc) lox = 0;
-    if ((float)t.tx * g.dx - lo >= rc) hix = 0;
+  const int nx = hix - lox + 1, ny = hiy - loy + 1, nz = hiz - loz + 1;
+  const int nnb = nx * ny * nz;  // <= 27
+  if (tid < 64) {
+    int cnt = 0;
+    if (tid < nnb) {
+      const int dx = lox + tid % nx, dy = loy + (tid / nx) % ny, dz = loz + tid / (nx * ny);
+      const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
+      const int j0 = a.set.start[nt];
+      nb_start[tid] = j0;
+      cnt = a.set.start[nt + 1] - j0;
+    }
+    const int incl = wave_incl_scan(cnt);
+    if (tid < 32) nb_pref[tid + 1] = incl;
+    if (tid == 0) nb_pref[0] = 0;
   }
-  if (t.nty >= 3) {
-    const float lo = ((float)(ci.y - ty * t.ty) + 0.5f) * g.dy + di.y;
-    if (lo >= rc) loy = 0;
-    if ((float)t.ty * g.dy - lo >= rc) hiy = 0;
-  }
-  if (t.ntz >= 3) {
-    const float lo = ((float)(ci.z - tz * t.tz) + 0.5f) * g.dz + di.z;
-    if (lo >= rc) loz = 0;
-    if ((float)t.tz * g.dz - lo >= rc) hiz = 0;
-  }
-  for (int dz = loz; dz <= hiz; ++dz)
-    for (int dy = loy; dy <= hiy; ++dy)
-      for (int dx = lox; dx <= hix; ++dx) {
-        int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
-        int j0 = a.set.start[nt], j1 = a.set.start[nt + 1];
-        for (int j = j0 + q; j < j1; j += 4) {
-          if (j == s) continue;
-          int4 cj = a.set.cell[j];
-          float4 dj = a.set.off[j];
-          float rx = (float)min_image(ci.x - cj.x, g.mx) * g.dx + (di.x - dj.x);
-          float ry = (float)min_image(ci.y - cj.y, g.my) * g.dy + (di.y - dj.y);
-          float rz = (float)min_image(ci.z - cj.z, g.mz) * g.dz + (di.z - dj.z);
+  __syncthreads();
+  const int nall = nb_pref[nnb];
+  // conservative cull radius (the pair test below is exact)
+  const float rcc = sqrtf(a.rc2) * 1.001f + 1e-6f;
+  const float Tx = t.tx * g.dx, Ty = t.ty * g.dy, Tz = t.tz * g.dz;
+  for (int i0 = 0; i0 < nown; i0 += kWcaThreads / kWcaSlices) {
+    const int il = tid / kWcaSlices, sl = tid % kWcaSlices;
+    const int si = s0 + i0 + il;
+    const bool own = i0 + il < nown;
+    int4 ci = make_int4(0, 0, 0, 0);
+    float4 di = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (own) {
+      ci = a.set.cell[si];
+      di = a.set.off[si];
+    }
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int c0 = 0; c0 < nall; c0 += kWcaThreads) {
+      // stage candidates c0 + tid that lie within rc of the tile, keeping their order
+      const int e = c0 + tid;
+      bool near = false;
+      int4 cj;
+      float4 dj;
+      if (e < nall) {
+        int lo = 0, hi = nnb;  // nb_pref[lo] <= e < nb_pref[lo + 1]
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (nb_pref[mid] <= e) lo = mid; else hi = mid;
+        }
+        const int j = nb_start[lo] + (e - nb_pref[lo]);
+        cj = a.set.cell[j];
+        dj = a.set.off[j];
+        cj.w = j;
+        // position relative to the tile centre (nearest image: the box spans at least three
+        // tiles, so m >= T + 2 rc), then the distance outside the tile along each axis
+        const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
+        const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
+        const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
+        const float ex = fmaxf(0.0f, fabsf(px) - 0.5f * Tx);
+        const float ey = fmaxf(0.0f, fabsf(py) - 0.5f * Ty);
+        const float ez = fmaxf(0.0f, fabsf(pz) - 0.5f * Tz);
+        // small boxes: min_image folds the tile onto itself, keep every candidate
+        near = (t.ntx < 3 || t.nty < 3 || t.ntz < 3) || (ex * ex + ey * ey + ez * ez <= rcc * rcc);
+      }
+      const unsigned long long bal = __ballot(near);
+      const int before_w = __popcll(bal & ((1ull << lane) - 1ull));
+      __syncthreads();  // previous chunk's candidates consumed
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int base = 0, nk = 0;
+#pragma unroll
+      for (int w = 0; w < kWcaThreads / 64; ++w) {
+        base += w < wv ? wsum[w] : 0;
+        nk += wsum[w];
+      }
+      if (near) {
+        ccell[base + before_w] = cj;
+        coff[base + before_w] = dj;
+      }
+      __syncthreads();
+      if (own) {
+        for (int k = sl; k < nk; k += kWcaSlices) {
+          const int4 cc = ccell[k];
+          if (cc.w == si) continue;
+          const float4 dd = coff[k];
+          float rx = (float)min_image(ci.x - cc.x, g.mx) * g.dx + (di.x - dd.x);
+          float ry = (float)min_image(ci.y - cc.y, g.my) * g.dy + (di.y - dd.y);
+          float rz = (float)min_image(ci.z - cc.z, g.mz) * g.dz + (di.z - dd.z);
           float r2 = rx * rx + ry * ry + rz * rz;
           if (r2 < a.rc2 && r2 > 0.0f) {
             // separations below sigma/100 are evaluated at sigma/100, so the force stays finite
             // (fluam's tabulated force is bounded near r = 0 too, src/initForcesNonBonded.cu:59)
             r2 = fmaxf(r2, 1.0e-4f * a.sig2);
-            float s2 = a.sig2 / r2;
-            float s6 = s2 * s2 * s2;
-            float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
+            const float s2 = a.sig2 / r2;
+            const float s6 = s2 * s2 * s2;
+            const float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
             fx += f * rx;
             fy += f * ry;
             fz += f * rz;
           }
         }
       }
+    }
+#pragma unroll
+    for (int o = 1; o < kWcaSlices; o <<= 1) {
+      fx += __shfl_xor(fx, o);
+      fy += __shfl_xor(fy, o);
+      fz += __shfl_xor(fz, o);
+    }
+    if (own && sl == 0) a.F[a.set.id[si]] = make_float4(fx, fy, fz, 0.0f);
+  }
 }
 
 // Weights of one axis at an explicit node base (used for the reference-quirk RFD variant):
@@ -224,17 +301,6 @@ FHD_DEV float3 f3(float4 a) { return make_float3(a.x, a.y, a.z); }
 // Position (cell, offset) of source `sub` of a particle, plus its value.
 FHD_DEV void rfd_normal(const Key& key, int id, float n[4]) {
   draw_normals4(key, (uint32_t)id, FHD_STREAM_RFD, 0, n);
-}
-
-// Inclusive prefix sum over the 64 lanes of a wave.
-FHD_DEV int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
-  return v;
 }
 
 // Does a stencil whose nodes span [lo, lo + NPTS] (both staggerings) reach the tile [o, o+T)?
@@ -400,15 +466,6 @@ FHD_DEV const SrcRec* rec_of(const SpreadArgs& a, int ref) {
 // accumulator (scale 2^(30 - e) per tile, from the largest face weight x value of its
 // candidates).  Integer addition is associative, so the result is bit-identical whatever order
 // the lanes' atomics land in.  FAST: tiles of 8 x 8 x 8 cells and at least two tiles per axis.
-// Tile of workgroup b.  Workgroups are dealt round-robin to the eight XCDs (each with its own
-// L2); giving every XCD a contiguous block of tiles, x fastest, keeps the neighbouring tiles
-// that share candidate records and output cache lines on the same L2.
-FHD_DEV int tile_of_block(int b, int ntiles) {
-  constexpr int kXcd = 8;
-  if (ntiles % kXcd) return b;
-  return (b % kXcd) * (ntiles / kXcd) + b / kXcd;
-}
-
 template <int NPTS, bool FAST>
 __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   constexpr int H = NPTS - 1;
@@ -423,7 +480,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x;
-  const int tile = tile_of_block(blockIdx.x, t.n);
+  const int tile = xcd_item(blockIdx.x, t.n);
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
   for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
@@ -551,7 +608,7 @@ FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const Ax
 // xor-shuffle tree, then lane 0 applies the position update.
 template <int NPTS>
 __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int s = gid >> 2, q = gid & 3;
   const GridDev& g = a.g;
   float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
@@ -646,7 +703,7 @@ void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const 
                      soff);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
-  if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
+  if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(a.t.n), dim3(kWcaThreads), 0, s, a);
 }
 void launch_src_prep(const SpreadArgs& a, hipStream_t s) {
   for (int gi = 0; gi < a.ngroups; ++gi) {
