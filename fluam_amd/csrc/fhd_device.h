@@ -12,7 +12,8 @@ enum { FHD_STREAM_FLUID = 1, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4, FHD_STREA
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products map to
-// v_mad_u64_u32, so a round costs two wide multiplies and two 3-input xors.
+// v_mad_u64_u32, which runs at the full VALU rate on gfx950, so a round costs two wide
+// multiplies and four xors (gfx9 has no three-input xor).
 // ---------------------------------------------------------------------------------------
 struct u32x4 { uint32_t x, y, z, w; };
 
@@ -36,12 +37,13 @@ FHD_DEV u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 // Box-Muller (1958) on 24-bit uniforms u1 in (0,1], u2 in [0,1): z0 = r cos(2 pi u2),
 // z1 = r sin(2 pi u2), r = sqrt(-2 ln u1).  The hardware transcendentals take the angle in
 // turns (v_sin_f32 / v_cos_f32 compute sin(2 pi x)) and log in base 2 (v_log_f32), so no range
-// reduction is needed: u2 already is the angle in turns.
+// reduction is needed: u2 already is the angle in turns.  The argument of the square root lies
+// in [0, 33.3] (never denormal), so the hardware v_sqrt_f32 (1 ulp) needs no scaling.
 FHD_DEV float2 box_muller(uint32_t a, uint32_t b) {
   const float inv24 = 1.0f / 16777216.0f;
   const float u1 = (float)((a >> 8) + 1u) * inv24;
   const float u2 = (float)(b >> 8) * inv24;
-  const float r = sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2(u1)
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2(u1)
   return make_float2(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2));
 }
 

@@ -21,7 +21,8 @@ constexpr int PLANE = PX * PY;
 // stress slot components: 0 sXX, 1 XY, 2 XZ, 3 sYY, 4 YZ, 5 sZZ
 FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz) {
   const GridDev& g = a.g;
-  const int z = wrapi(zz, g.mz);
+  // zz in [-1, mz], x0 + lx in [-1, mx], y0 + ly in [-1, my]: one conditional wrap each
+  const int z = zz < 0 ? zz + g.mz : (zz >= g.mz ? zz - g.mz : zz);
   const int nown = TX * TY, ntot = nown + 2 * TX + 2 * TY;
   for (int e = threadIdx.x; e < ntot; e += blockDim.x) {
     int lx, ly;
@@ -33,7 +34,9 @@ FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz)
       else if (r < 2 * TY + TX) { lx = r - 2 * TY; ly = -1; }
       else { lx = r - 2 * TY - TX; ly = TY; }
     }
-    int x = wrapi(x0 + lx, g.mx), y = wrapi(y0 + ly, g.my);
+    int x = x0 + lx, y = y0 + ly;
+    x = x < 0 ? x + g.mx : (x >= g.mx ? x - g.mx : x);
+    y = y < 0 ? y + g.my : (y >= g.my ? y - g.my : y);
     uint32_t cell = (uint32_t)(x + g.mx * (y + g.my * z));
     float n[12];
     draw_normals4(a.key, cell, FHD_STREAM_FLUID, 0, &n[0]);
