@@ -8,6 +8,10 @@
 #include "fhd_kernels.h"
 #include "fft_common.h"
 
+#ifndef FFTB_TILE
+#define FFTB_TILE 8
+#endif
+
 namespace fhd {
 
 // One radix-R Stockham stage, lines l = 0..lines-1 at base l * LS, element stride ES.
@@ -172,9 +176,9 @@ __global__ __launch_bounds__(NT) void k_fftC(FftArgs a, const float2* __restrict
 // ---------------- pass B: FFT along z, operator, inverse FFT along z ----------------
 // Workgroup (tile i, ky): kx columns i*T .. i*T+T-1, plus column ntile*T + i when P is not a
 // multiple of T (the Nyquist column for P = mx/2 + 1).  LDS lines: (c, t) at (c*(T+1)+t)*(MZ+1).
-template <int MZ, int NT>
+template <int MZ, int NT, int TM>
 __global__ __launch_bounds__(NT) void k_fftB(FftArgs a, float2* __restrict__ spec, int T) {
-  constexpr int TM = 17;  // T <= 16 plus one extra column
+  // TM: room for T <= TM - 1 columns plus one extra column
   constexpr int LS = MZ + 1;
   __shared__ float2 buf[3 * TM * LS];
   __shared__ float2 twz[MZ], ezs[MZ];
@@ -276,18 +280,20 @@ bool fast_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const flo
   return fast_AC(a, 1, nullptr, const_cast<float2*>(spec), out, aux, alpha, beta, s);
 }
 
-// Largest column-tile width T <= 16 whose leftover columns (P mod T) fit one per tile.
-static int column_tile(int P) {
-  for (int T = 16; T > 1; --T)
+// Largest column-tile width T <= tmax whose leftover columns (P mod T) fit one per tile.
+static int column_tile(int P, int tmax) {
+  for (int T = tmax; T > 1; --T)
     if (T <= P && P % T <= P / T) return T;
   return 1;
 }
 
+constexpr int kFftBTile = FFTB_TILE;  // kx columns per z-pass workgroup (8 or 16)
+
 template <int MZ>
 static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
   const int P = a.g.P;
-  const int T = column_tile(P);
-  hipLaunchKernelGGL((k_fftB<MZ, 256>), dim3(P / T, a.g.my), dim3(256), 0, s, a, spec, T);
+  const int T = column_tile(P, kFftBTile);
+  hipLaunchKernelGGL((k_fftB<MZ, 256, kFftBTile + 1>), dim3(P / T, a.g.my), dim3(256), 0, s, a, spec, T);
   return true;
 }
 

@@ -297,14 +297,15 @@ FluidWArgs base_w(fhd_ctx* c) {
 }
 
 // ---------------- particles ----------------
+// Counting sort of position set k by tile, then each tile ordered by particle index.
 void sort_set(fhd_ctx* c, int k) {
   launch_bin(c->g, c->t, c->cell[k], c->count, c->slot, c->np, c->s);
-  launch_scan(c->t, c->count, c->start[k], c->s);
-  launch_scatter(c->g, c->t, c->cell[k], c->slot, c->start[k], c->tmp, c->np, c->s);
+  launch_scan_scatter(c->g, c->t, c->cell[k], c->slot, c->count, c->start[k], c->tmp, c->np, c->s);
   launch_tile_sort(c->g, c->t, c->start[k], c->tmp, c->cell[k], c->off[k], c->sid[k], c->scell[k], c->soff[k],
-                   c->s);
+                   c->count, c->s);
   mark(c, "sort");
 }
+
 SortedSet sset(fhd_ctx* c, int k) {
   SortedSet s;
   s.cell = c->scell[k];

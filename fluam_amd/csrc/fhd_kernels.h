@@ -141,8 +141,10 @@ void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, 
 void launch_scan(const Tiles& t, int* count, int* start, hipStream_t s);
 void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, const int* start,
                     int* tmp, int np, hipStream_t s);
+void launch_scan_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, int* count, int* start,
+                         int* tmp, int np, hipStream_t s);
 void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
-                      const float4* off, int* sid, int4* scell, float4* soff, hipStream_t s);
+                      const float4* off, int* sid, int4* scell, float4* soff, int* count, hipStream_t s);
 void launch_wca(const WcaArgs& a, hipStream_t s);
 void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
 void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation

@@ -21,6 +21,8 @@
 
 namespace fhd {
 
+FHD_DEV int wave_incl_scan(int v);
+
 FHD_DEV int tile_of(const GridDev& g, const Tiles& t, int4 c) {
   int x = wrapi(c.x, g.mx) / t.tx, y = wrapi(c.y, g.my) / t.ty, z = wrapi(c.z, g.mz) / t.tz;
   return x + t.ntx * (y + t.nty * z);
@@ -66,17 +68,51 @@ __global__ void k_scatter(GridDev g, Tiles t, const int4* __restrict__ cell, con
   tmp[start[tile_of(g, t, cell[i])] + slot[i]] = i;
 }
 
+// Scan + scatter in one launch for up to kScanTilesMax tiles: every workgroup forms the whole
+// exclusive prefix of the tile counts (a few KB, read from L2) in LDS, workgroup 0 stores it as
+// the tile starts, and each thread places its particle.  The counts are cleared later, by the
+// per-tile ordering pass.
+constexpr int kScanTilesMax = 8192;
+__global__ __launch_bounds__(256) void k_scan_scatter(GridDev g, Tiles t, const int4* __restrict__ cell,
+                                                      const int* __restrict__ slot, const int* __restrict__ count,
+                                                      int* __restrict__ start, int* __restrict__ tmp, int np) {
+  __shared__ int pre[kScanTilesMax + 1];
+  __shared__ int wsum[4];
+  const int n = t.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ch = (n + 255) / 256;
+  const int lo = min(n, tid * ch), hi = min(n, lo + ch);
+  int sum = 0;
+  for (int k = lo; k < hi; ++k) sum += count[k];
+  const int incl = wave_incl_scan(sum);
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  int run = incl - sum;
+  for (int w = 0; w < wv; ++w) run += wsum[w];
+  for (int k = lo; k < hi; ++k) {
+    pre[k] = run;
+    run += count[k];
+  }
+  if (tid == 255) pre[n] = run;
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int k = tid; k <= n; k += 256) start[k] = pre[k];
+  const int i = blockIdx.x * blockDim.x + tid;
+  if (i < np) tmp[pre[tile_of(g, t, cell[i])] + slot[i]] = i;
+}
+
 // One wavefront per tile: rank each particle by its index within the tile, then gather.  The
 // sorted copies carry the cell wrapped into the box (0 <= c < m), which is all the spatial
 // passes need; the unwrapped cell stays in the per-particle arrays.
 __global__ __launch_bounds__(256) void k_tile_sort(GridDev g, int ntiles, const int* __restrict__ start,
                                                    const int* __restrict__ tmp, const int4* __restrict__ cell,
                                                    const float4* __restrict__ off, int* __restrict__ sid,
-                                                   int4* __restrict__ scell, float4* __restrict__ soff) {
+                                                   int4* __restrict__ scell, float4* __restrict__ soff,
+                                                   int* __restrict__ count) {
   const int lane = threadIdx.x & 63;
   const int tile = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (tile >= ntiles) return;
   const int s0 = start[tile], n = start[tile + 1] - s0;
+  if (lane == 0) count[tile] = 0;  // ready for the next binning
   for (int e = lane; e < n; e += 64) {
     int id = tmp[s0 + e];
     int rank = 0;
@@ -312,13 +348,16 @@ FHD_DEV bool reaches(int lo, int o, int T, int m, int ntiles) {
 }
 
 // One record per source (sorted slot x sub-source): weights, bases, value and the cull data.
+// One launch covers every group: threads [0, n0) build group 0's records, the rest group 1's.
 template <int NPTS>
-__global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int gi) {
+__global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= ntot) return;
+  const int gi = r < n0 ? 0 : 1;
+  if (gi) r -= n0;
   const SpreadGroup& G = a.grp[gi];
   const GridDev& g = a.g;
   const int nsub = G.mode == SRC_RFD ? 2 : 1;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= G.nsrc * nsub) return;
   const int sidx = r / nsub, sub = r - sidx * nsub;
   const int id = G.set.id[sidx];
   const int4 c = G.set.cell[sidx];
@@ -697,21 +736,32 @@ void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const in
                     int np, hipStream_t s) {
   if (np > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, g, t, cell, slot, start, tmp, np);
 }
+void launch_scan_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, int* count, int* start,
+                         int* tmp, int np, hipStream_t s) {
+  if (t.n <= kScanTilesMax) {
+    hipLaunchKernelGGL(k_scan_scatter, dim3(nblk(np > 0 ? np : 1, 256)), dim3(256), 0, s, g, t, cell, slot, count,
+                       start, tmp, np);
+  } else {
+    launch_scan(t, count, start, s);
+    launch_scatter(g, t, cell, slot, start, tmp, np, s);
+  }
+}
 void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
-                      const float4* off, int* sid, int4* scell, float4* soff, hipStream_t s) {
+                      const float4* off, int* sid, int4* scell, float4* soff, int* count, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_sort, dim3(nblk(t.n, 4)), dim3(256), 0, s, g, t.n, start, tmp, cell, off, sid, scell,
-                     soff);
+                     soff, count);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(a.t.n), dim3(kWcaThreads), 0, s, a);
 }
 void launch_src_prep(const SpreadArgs& a, hipStream_t s) {
-  for (int gi = 0; gi < a.ngroups; ++gi) {
-    const int nrec = a.grp[gi].nsrc * (a.grp[gi].mode == SRC_RFD ? 2 : 1);
-    if (nrec <= 0) continue;
-    if (a.kernel_points == 4) hipLaunchKernelGGL(k_src_prep<4>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
-    else hipLaunchKernelGGL(k_src_prep<3>, dim3(nblk(nrec, 256)), dim3(256), 0, s, a, gi);
-  }
+  int nrec[2] = {0, 0};
+  for (int gi = 0; gi < a.ngroups; ++gi) nrec[gi] = a.grp[gi].nsrc * (a.grp[gi].mode == SRC_RFD ? 2 : 1);
+  const int ntot = nrec[0] + nrec[1];
+  if (ntot <= 0) return;
+  if (a.kernel_points == 4)
+    hipLaunchKernelGGL(k_src_prep<4>, dim3(nblk(ntot, 256)), dim3(256), 0, s, a, nrec[0], ntot);
+  else hipLaunchKernelGGL(k_src_prep<3>, dim3(nblk(ntot, 256)), dim3(256), 0, s, a, nrec[0], ntot);
 }
 void launch_spread(const SpreadArgs& a, hipStream_t s) {
   const bool fast = a.t.tx == kTileEdge && a.t.ty == kTileEdge && a.t.tz == kTileEdge && a.t.ntx >= 2 &&
