@@ -78,7 +78,7 @@ struct SpreadGroup {
   float disp;           // SRC_RFD: displacement magnitude delta/2
   int rfd_quirk;        // SRC_RFD: weights at the undisplaced position
   SrcRec* rec;          // nsrc x (1 or 2) records, in sorted order
-  int4* cull;           // per record: lowest stencil node per axis, bound on one contribution (float bits)
+  int4* cull;           // per record: x = tile reach mask (3 bits per axis), w = bound on one contribution (float bits)
 };
 
 struct SpreadArgs {

@@ -416,16 +416,29 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot
 #pragma unroll
     for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(rec.w[1][ax][q]));
   if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
+  // reach mask: per axis, bit e + 1 set when the stencils (nodes lo .. lo + NPTS, both
+  // staggerings) reach the tile at offset e in {-1, 0, 1} from the source's sorting tile
   const int lo[3] = {min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
                      min(w.cen[2].base, w.stg[2].base)};
-  G.cull[r] = make_int4(lo[0], lo[1], lo[2], (int)__float_as_uint(m));
+  const Tiles& t = a.t;
+  const int own[3] = {c.x / t.tx, c.y / t.ty, c.z / t.tz};  // sorted copies hold wrapped cells
+  const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz}, MM[3] = {g.mx, g.my, g.mz};
+  int mask = 0;
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+    for (int e = -1; e <= 1; ++e)
+      if (reaches<NPTS>(lo[ax], wrapi(own[ax] + e, NT[ax]) * T[ax], T[ax], MM[ax], NT[ax]))
+        mask |= 1 << (3 * ax + e + 1);
+  G.cull[r] = make_int4(mask, 0, 0, (int)__float_as_uint(m));
 }
 
-// Fixed-point image of one contribution (|x| <= 2^30 after scaling): nearest integer, as a
-// 64-bit two's-complement addend.  Integer sums are exact, so the accumulated field does not
+// Fixed-point image of one contribution (|x| <= 2^30 after scaling), as a 64-bit two's-complement
+// addend.  Integer sums are exact, so the accumulated field does not
 // depend on the order in which the contributions arrive.
 FHD_DEV unsigned long long fixed64(float x) {
-  const int q = __float2int_rn(x);
+  // truncation toward zero: at the 2^30 scale its bias (< 1 unit) is far below FP32 rounding
+  const int q = (int)x;
   return (unsigned long long)(long long)q;
 }
 
@@ -434,6 +447,9 @@ constexpr int kItemCap = 1024;
 // Accumulator z-row stride (64-bit words): the 8 x 8 plane plus 2, so the rows that the four
 // lanes of one candidate update start in different LDS banks.
 constexpr int kRowStride = kTileEdge * kTileEdge + 2;
+// Guard words before and after the accumulator: a stencil's zero-weight columns address up to
+// 3 (x) + 3 x 8 (y) words before its row and 2 x 8 + 2 + ... words past the last row.
+constexpr int kAccPad = 32;
 
 // Contributions of stencil row kz (nodes base_z + kz) of one source record to the three
 // components of the tile accumulator acc[component][z row][y][x] (x, y: the 8 x 8 tile columns).
@@ -466,15 +482,20 @@ FHD_DEV void add_source_row(unsigned long long* acc, const SrcRec* __restrict__ 
       dy += dy < -H ? g.my : (dy >= g.my - H ? -g.my : 0);
       dz += dz < -H ? g.mz : (dz >= g.mz - H ? -g.mz : 0);
       if (dx >= kTileEdge || dy >= kTileEdge || dz >= kTileEdge) continue;  // misses in this staggering
-      unsigned long long* p = acc + (C * ZS + dz + kz + H) * PL + dy * kTileEdge + dx;
+      // columns outside the tile get zero weight: their (zero) additions land on other nodes of
+      // the padded accumulator and change nothing, so no node needs a test
+      float zx[4], zy[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        zx[k] = (unsigned)(dx + k) < (unsigned)kTileEdge ? wx[k] : 0.0f;
+        zy[k] = (unsigned)(dy + k) < (unsigned)kTileEdge ? wy[k] : 0.0f;
+      }
+      unsigned long long* p = acc + kAccPad + (C * ZS + dz + kz + H) * PL + dy * kTileEdge + dx;
 #pragma unroll
       for (int ky = 0; ky < NPTS; ++ky) {
-        if ((unsigned)(dy + ky) >= (unsigned)kTileEdge) continue;
-        const float wyz = wy[ky] * wzs;
+        const float wyz = zy[ky] * wzs;
 #pragma unroll
-        for (int kx = 0; kx < NPTS; ++kx)
-          if ((unsigned)(dx + kx) < (unsigned)kTileEdge)
-            atomicAdd(&p[ky * kTileEdge + kx], fixed64(wx[kx] * wyz));
+        for (int kx = 0; kx < NPTS; ++kx) atomicAdd(&p[ky * kTileEdge + kx], fixed64(zx[kx] * wyz));
       }
     } else {
       const int lz = fold(bz + kz - oz, g.mz);
@@ -511,8 +532,9 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
   constexpr int NACC = 3 * ZS * kRowStride;
   constexpr int LPC = 4;  // lanes per candidate (one stencil z-row each)
-  __shared__ unsigned long long acc[NACC];
-  __shared__ int nb_start[64], nb_pref[65];
+  __shared__ unsigned long long acc_buf[NACC + 2 * kAccPad];
+  unsigned long long* acc = acc_buf;  // the node arrays start at acc + kAccPad (FAST) or acc
+  __shared__ int nb_start[64], nb_pref[65], nb_sel[64];
   __shared__ int items[kItemCap];
   __shared__ int nitem;
   __shared__ unsigned int maxbits;
@@ -522,7 +544,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   const int tile = xcd_item(blockIdx.x, t.n);
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
-  for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
+  for (int e = tid; e < NACC + 2 * kAccPad; e += kSpreadThreads) acc[e] = 0ull;
   // neighbour-tile table: lane k of wave 0 loads the record range of (group, neighbour) k
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
@@ -541,6 +563,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
       const int s0 = a.grp[gi].set.start[nt];
       nb_start[tid] = s0 * nsub;
       cnt = (a.grp[gi].set.start[nt + 1] - s0) * nsub;
+      // this tile sits at offset -d from the source tile: the reach-mask bits to test
+      nb_sel[tid] = (1 << (1 - dx)) | (1 << (4 - dy)) | (1 << (7 - dz));
     }
     const int incl = wave_incl_scan(cnt);
     nb_pref[tid + 1] = incl;
@@ -562,9 +586,8 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
     const int gi = lo >= nnb;
     const int ri = nb_start[lo] + (e - nb_pref[lo]);
     const int4 cu = a.grp[gi].cull[ri];
-    if (!(reaches<NPTS>(cu.x, ox, t.tx, g.mx, t.ntx) && reaches<NPTS>(cu.y, oy, t.ty, g.my, t.nty) &&
-          reaches<NPTS>(cu.z, oz, t.tz, g.mz, t.ntz)))
-      return -1;
+    const int sel = nb_sel[lo];
+    if ((cu.x & sel) != sel) return -1;
     mx = fmaxf(mx, __int_as_float(cu.w));
     return (gi << 30) | ri;
   };
@@ -614,7 +637,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   for (int e = tid; e < 3 * ncell; e += kSpreadThreads) {
     const int C = e / ncell, l = e - C * ncell;
     const int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
-    const int ai = (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
+    const int ai = (FAST ? kAccPad : 0) + (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
     const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
     a.out[gi] = (float)(long long)acc[ai] * inv;
   }
