@@ -578,11 +578,12 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   const int nall = nb_pref[ntab];
   // record index e -> reference; -1 when the record's stencils miss the tile
   auto cull = [&](int e, float& mx) -> int {
-    int lo = 0, hi = ntab;  // nb_pref[lo] <= e < nb_pref[lo + 1]
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (nb_pref[mid] <= e) lo = mid; else hi = mid;
-    }
+    // largest lo with nb_pref[lo] <= e: fixed-step search (no divergent loop); entries past
+    // ntab hold nall, which no record index reaches
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+      if (nb_pref[lo + step] <= e) lo += step;
     const int gi = lo >= nnb;
     const int ri = nb_start[lo] + (e - nb_pref[lo]);
     const int4 cu = a.grp[gi].cull[ri];
@@ -591,12 +592,20 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
     mx = fmaxf(mx, __int_as_float(cu.w));
     return (gi << 30) | ri;
   };
-  // pass 1: cull every record, keep the first kItemCap candidates, largest magnitude
+  // pass 1: cull every record, keep the first kItemCap candidates, largest magnitude; each wave
+  // appends its survivors with one LDS atomic
   float lmax = 0.0f;
-  for (int e = tid; e < nall; e += kSpreadThreads) {
-    const int ref = cull(e, lmax);
+  const int lane = tid & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int e0 = 0; e0 < nall; e0 += kSpreadThreads) {
+    const int e = e0 + tid;
+    const int ref = e < nall ? cull(e, lmax) : -1;
+    const unsigned long long bal = __ballot(ref >= 0);
+    int base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&nitem, __popcll(bal));
+    base = __shfl(base, 0);
     if (ref >= 0) {
-      const int k = atomicAdd(&nitem, 1);
+      const int k = base + __popcll(bal & below);
       if (k < kItemCap) items[k] = ref;
     }
   }
@@ -632,11 +641,22 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
     }
   }
   __syncthreads();
-  // interior nodes -> field
+  // interior nodes -> field (8 x 8 x 8 tiles: index arithmetic by shifts)
   const int ncell = t.tx * t.ty * t.tz;
   for (int e = tid; e < 3 * ncell; e += kSpreadThreads) {
-    const int C = e / ncell, l = e - C * ncell;
-    const int lx = l % t.tx, ly = (l / t.tx) % t.ty, lz = l / (t.tx * t.ty);
+    int C, lx, ly, lz;
+    if (FAST) {
+      C = e >> 9;
+      lx = e & 7;
+      ly = (e >> 3) & 7;
+      lz = (e >> 6) & 7;
+    } else {
+      C = e / ncell;
+      const int l = e - C * ncell;
+      lx = l % t.tx;
+      ly = (l / t.tx) % t.ty;
+      lz = l / (t.tx * t.ty);
+    }
     const int ai = (FAST ? kAccPad : 0) + (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
     const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
     a.out[gi] = (float)(long long)acc[ai] * inv;
