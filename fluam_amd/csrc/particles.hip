@@ -221,11 +221,11 @@ __global__ __launch_bounds__(kWcaThreads) void k_wca(WcaArgs a) {
       int4 cj;
       float4 dj;
       if (e < nall) {
-        int lo = 0, hi = nnb;  // nb_pref[lo] <= e < nb_pref[lo + 1]
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (nb_pref[mid] <= e) lo = mid; else hi = mid;
-        }
+        // largest lo with nb_pref[lo] <= e (entries past nnb hold nall)
+        int lo = 0;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1)
+          if (nb_pref[lo + step] <= e) lo += step;
         const int j = nb_start[lo] + (e - nb_pref[lo]);
         cj = a.set.cell[j];
         dj = a.set.off[j];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot
   const SpreadGroup& G = a.grp[gi];
   const GridDev& g = a.g;
   const int nsub = G.mode == SRC_RFD ? 2 : 1;
-  const int sidx = r / nsub, sub = r - sidx * nsub;
+  const int sidx = nsub == 2 ? r >> 1 : r, sub = r - sidx * nsub;
   const int id = G.set.id[sidx];
   const int4 c = G.set.cell[sidx];
   const float4 d = G.set.off[sidx];
@@ -421,15 +421,19 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot
   const int lo[3] = {min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
                      min(w.cen[2].base, w.stg[2].base)};
   const Tiles& t = a.t;
-  const int own[3] = {c.x / t.tx, c.y / t.ty, c.z / t.tz};  // sorted copies hold wrapped cells
   const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz}, MM[3] = {g.mx, g.my, g.mz};
+  const int cc[3] = {c.x, c.y, c.z};  // sorted copies hold wrapped cells
   int mask = 0;
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax)
+  for (int ax = 0; ax < 3; ++ax) {
+    const int own = T[ax] == 8 ? cc[ax] >> 3 : cc[ax] / T[ax];
 #pragma unroll
-    for (int e = -1; e <= 1; ++e)
-      if (reaches<NPTS>(lo[ax], wrapi(own[ax] + e, NT[ax]) * T[ax], T[ax], MM[ax], NT[ax]))
-        mask |= 1 << (3 * ax + e + 1);
+    for (int e = -1; e <= 1; ++e) {
+      int tt = own + e;  // in [-1, NT]
+      tt += tt < 0 ? NT[ax] : (tt >= NT[ax] ? -NT[ax] : 0);
+      if (reaches<NPTS>(lo[ax], tt * T[ax], T[ax], MM[ax], NT[ax])) mask |= 1 << (3 * ax + e + 1);
+    }
+  }
   G.cull[r] = make_int4(mask, 0, 0, (int)__float_as_uint(m));
 }
 
