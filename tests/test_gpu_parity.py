@@ -77,6 +77,38 @@ def test_incompressible_noncubic_with_background():
     assert g.max_divergence() < 1e-6
 
 
+def test_incompressible_structure_factor_gpu():
+    """Equal-time structure factor over an ensemble: S_a(k) = <|v_hat_a(k)|^2> / N with v_hat the
+    unnormalised DFT; the discrete theory for Crank-Nicolson with fluam's noise gives
+    sum_a S_a(k) = trace P(k) kT / (rho dV) = 2 kT / (rho dV) for every k != 0 (at any dt for
+    the linear part; at kT = 1e-3 advection is negligible).  Four replicas (own seeds), 32^3,
+    120 snapshots each: the mode average must match to 1%, and every |k| shell from 2 up to 5%
+    (shell 1 holds 18 slowly decorrelating modes, too few for a 5% statistical bound)."""
+    m, kT, rho = 32, 1e-3, 1.0
+    N = m ** 3
+    acc = np.zeros((m, m, m))
+    ns = 0
+    for seed in (41, 42, 43, 44):
+        p = make_params(SCHEME_INCOMPRESSIBLE, cells=(m, m, m), temperature=kT, dt=0.5, seed=seed)
+        _announce(f"structure_factor_seed{seed}", p)
+        g = F.Simulation(p)
+        g.step(100)
+        for _ in range(120):
+            g.step(5)
+            v = g.velocity().reshape(3, m, m, m)
+            acc += sum(np.abs(np.fft.fftn(v[a])) ** 2 for a in range(3)) / N
+            ns += 1
+    S = acc / ns / (2 * kT / rho)
+    S[0, 0, 0] = np.nan
+    assert abs(np.nanmean(S) - 1.0) < 0.01
+    k = np.fft.fftfreq(m) * m
+    kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2)
+    shells = np.round(kk).astype(int)
+    for sh in range(2, m // 2):
+        sel = (shells == sh) & np.isfinite(S)
+        assert abs(S[sel].mean() - 1.0) < 0.05, (sh, S[sel].mean())
+
+
 def test_incompressible_deterministic_replay():
     p = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=5)
     a = F.Simulation(p)
