@@ -58,6 +58,7 @@ def lib():
         getattr(L, n).argtypes = [vp, dp]
     L.fhd_step.argtypes = [vp, C.c_int]
     L.fhd_sync.argtypes = [vp]
+    L.fhd_set_graph.argtypes = [vp, C.c_int]
     L.fhd_get_step.argtypes = [vp]
     L.fhd_get_step.restype = C.c_longlong
     L.fhd_time_steps.argtypes = [vp, C.c_int, dp]
@@ -147,6 +148,10 @@ class Simulation:
 
     def step(self, n=1):
         self._check(lib().fhd_step(self.h, int(n)))
+
+    def set_graph(self, on=True):
+        """Replay the steady-state step from captured HIP graphs (identical results)."""
+        self._check(lib().fhd_set_graph(self.h, 1 if on else 0))
 
     def sync(self):
         self._check(lib().fhd_sync(self.h))

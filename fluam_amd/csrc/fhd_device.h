@@ -47,7 +47,23 @@ FHD_DEV float2 box_muller(uint32_t a, uint32_t b) {
   return make_float2(r * __builtin_amdgcn_cosf(u2), r * __builtin_amdgcn_sinf(u2));
 }
 
-struct Key { uint32_t k0, k1; uint32_t step_lo, step_hi; };
+// Noise key: seed words and step counter.  When `step` is non-null (steps replayed from a
+// captured graph) the counter is read from device memory at kernel entry (resolve_key).
+struct Key {
+  uint32_t k0, k1;
+  uint32_t step_lo, step_hi;
+  const unsigned long long* step;
+};
+
+FHD_DEV Key resolve_key(Key k) {
+  if (k.step) {
+    const unsigned long long s = *k.step;
+    k.step_lo = (uint32_t)s;
+    k.step_hi = (uint32_t)(s >> 32);
+    k.step = nullptr;
+  }
+  return k;
+}
 
 FHD_DEV u32x4 draw_words(const Key& key, uint32_t index, uint32_t stream, uint32_t sub) {
   u32x4 c = {index, (stream << 4) | sub, key.step_lo, key.step_hi};

@@ -109,10 +109,21 @@ struct fhd_ctx {
   long long step = 0;
   bool projected = false;
   bool qnb_started = false;
+  bool qnb_steady = false;   // the current QNB step is an Adams-Bashforth one
   int adv_cur = 0;
   int last_force_set = 0;
   std::vector<std::pair<std::string, hipEvent_t>> marks;  // per-kernel profiling
   bool profiling = false;
+  // graph replay of the steady-state step: one captured graph per step phase (QNB alternates
+  // its velocity / advection-history buffers, so it has two phases)
+  bool graph_mode = false;
+  bool capturing = false;
+  bool graphs_built = false;
+  bool dev_step_valid = false;           // step_dev == step
+  unsigned long long* step_dev = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int gperiod = 0, gphase = 0;
+  bool own_stream = true;
 };
 
 namespace {
@@ -131,6 +142,8 @@ Key make_key(const fhd_ctx* c) {
   k.k1 = (uint32_t)(c->p.seed >> 32);
   k.step_lo = (uint32_t)(uint64_t)c->step;
   k.step_hi = (uint32_t)((uint64_t)c->step >> 32);
+  // a step being captured for replay reads the counter from the device at run time
+  k.step = c->capturing ? c->step_dev : nullptr;
   return k;
 }
 
@@ -239,7 +252,10 @@ void free_ctx(fhd_ctx* c) {
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
-  if (c->s) (void)hipStreamDestroy(c->s);
+  for (auto& e : c->gexec)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (c->step_dev) (void)hipFree(c->step_dev);
+  if (c->s && c->own_stream) (void)hipStreamDestroy(c->s);
 }
 
 // Host position <-> (cell, offset).
@@ -491,6 +507,14 @@ void step_stokes(fhd_ctx* c) {
   }
 }
 
+// Host-side state a QNB step leaves behind (also applied after each replay of a captured step):
+// the new velocity becomes v, and in the Adams-Bashforth regime the history buffers alternate.
+void qnb_after_step(fhd_ctx* c) {
+  if (c->qnb_steady) c->adv_cur = 1 - c->adv_cur;
+  c->qnb_steady = c->qnb_started;
+  std::swap(c->v, c->v2);
+}
+
 // Quasi-neutrally-buoyant particles, excess mass 0: src/runSchemeQuasiNeutrallyBuoyant.cu:178-677
 // with the first-step bootstrap of src/firstStepQuasiNeutrallyBuoyant.cu:35-1152.
 void step_qnb(fhd_ctx* c) {
@@ -558,7 +582,6 @@ void step_qnb(fhd_ctx* c) {
     w.adv_store = c->adv[1 - c->adv_cur];
     launch_fluid_w(w, c->s);
     mark(c, "fluid_w");
-    c->adv_cur = 1 - c->adv_cur;
   }
   fft_forward(c, c->W);
   fft_solve(c, 0, cn);
@@ -578,17 +601,84 @@ void step_qnb(fhd_ctx* c) {
     launch_interp(ia, c->s);
     mark(c, "interp");
   }
-  std::swap(c->v, c->v2);
+  qnb_after_step(c);
 }
 
-void do_step(fhd_ctx* c) {
+void scheme_step(fhd_ctx* c) {
   switch (c->p.scheme) {
     case FHD_SCHEME_INCOMPRESSIBLE: step_incompressible(c); break;
     case FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT: step_qnb(c); break;
     case FHD_SCHEME_STOKES_LIMIT: step_stokes(c); break;
     default: throw std::runtime_error("unknown scheme");
   }
+}
+
+// Eager step: every kernel launched with the step counter as an argument.
+void do_step(fhd_ctx* c) {
+  scheme_step(c);
   c->step++;
+  c->dev_step_valid = false;
+  if (c->graphs_built) c->gphase = (c->gphase + 1) % c->gperiod;
+}
+
+// The step sequence repeats (with period 1, or 2 for QNB) once the start-up steps are done.
+bool steady(const fhd_ctx* c) {
+  switch (c->p.scheme) {
+    case FHD_SCHEME_INCOMPRESSIBLE: return c->projected;
+    case FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT: return c->projected && c->qnb_steady;
+    default: return true;
+  }
+}
+
+// Capture one graph per phase of the steady-state step.  Capturing runs the host logic of the
+// step (which advances host-side state exactly as an eager step would) and records the launches;
+// the step counter comes from device memory, advanced by the last node of each graph.
+void build_graphs(fhd_ctx* c) {
+  if (!c->step_dev) c->step_dev = dalloc<unsigned long long>(1);
+  c->gperiod = c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT ? 2 : 1;
+  for (int ph = 0; ph < c->gperiod; ++ph) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    try {
+      scheme_step(c);
+      launch_advance_step(c->step_dev, c->s);
+    } catch (...) {
+      c->capturing = false;
+      (void)hipStreamEndCapture(c->s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    c->capturing = false;
+    HIPCHK(hipStreamEndCapture(c->s, &g));
+    HIPCHK(hipGraphInstantiate(&c->gexec[ph], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  // after gperiod captured steps the host state is back where it started
+  c->gphase = 0;
+  c->graphs_built = true;
+}
+
+void graph_step(fhd_ctx* c) {
+  if (!c->dev_step_valid) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    const unsigned long long v = (unsigned long long)c->step;
+    HIPCHK(hipMemcpy(c->step_dev, &v, sizeof(v), hipMemcpyHostToDevice));
+    c->dev_step_valid = true;
+  }
+  HIPCHK(hipGraphLaunch(c->gexec[c->gphase], c->s));
+  if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) qnb_after_step(c);
+  c->gphase = (c->gphase + 1) % c->gperiod;
+  c->step++;
+}
+
+void advance(fhd_ctx* c) {
+  if (c->graph_mode && c->profiling == false && steady(c)) {
+    if (!c->graphs_built) build_graphs(c);
+    graph_step(c);
+  } else {
+    do_step(c);
+  }
 }
 
 }  // namespace
@@ -1039,9 +1129,14 @@ int fhd_get_forces(fhd_ctx* c, double* f) { return get_f4(c, c ? c->F[c->last_fo
 
 int fhd_step(fhd_ctx* c, int nsteps) {
   CTX_TRY(c, {
-    for (int k = 0; k < nsteps; ++k) do_step(c);
+    for (int k = 0; k < nsteps; ++k) advance(c);
     HIPCHK(hipGetLastError());
   });
+  return 1;
+}
+
+int fhd_set_graph(fhd_ctx* c, int on) {
+  CTX_TRY(c, { c->graph_mode = on != 0; });
   return 1;
 }
 
@@ -1063,7 +1158,7 @@ int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms) {
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
     HIPCHK(hipEventRecord(a, c->s));
-    for (int k = 0; k < nsteps; ++k) do_step(c);
+    for (int k = 0; k < nsteps; ++k) advance(c);
     HIPCHK(hipEventRecord(b, c->s));
     HIPCHK(hipEventSynchronize(b));
     float t = 0;
@@ -1207,6 +1302,7 @@ int fhd_normals(unsigned long long seed, long long step, int stream, int sub, in
     k.k1 = (uint32_t)(seed >> 32);
     k.step_lo = (uint32_t)(uint64_t)step;
     k.step_hi = (uint32_t)((uint64_t)step >> 32);
+    k.step = nullptr;
     float* d = dalloc<float>(4 * (size_t)n);
     launch_normals(k, stream, sub, n, d, 0);
     std::vector<float> h(4 * (size_t)n);

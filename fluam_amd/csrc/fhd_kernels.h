@@ -135,6 +135,7 @@ void launch_fluid_w(const FluidWArgs& a, hipStream_t s);
 void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t s);
 void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipStream_t s);
 void launch_zero_f(float* p, size_t n, hipStream_t s);
+void launch_advance_step(unsigned long long* step, hipStream_t s);
 
 void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, int* slot, int np,
                 hipStream_t s);

@@ -19,7 +19,7 @@ constexpr int PX = TX + 2, PY = TY + 2;
 constexpr int PLANE = PX * PY;
 
 // stress slot components: 0 sXX, 1 XY, 2 XZ, 3 sYY, 4 YZ, 5 sZZ
-FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz) {
+FHD_DEV void gen_plane(const FluidWArgs& a, const Key& key, float* slot, int x0, int y0, int zz) {
   const GridDev& g = a.g;
   // zz in [-1, mz], x0 + lx in [-1, mx], y0 + ly in [-1, my]: one conditional wrap each
   const int z = zz < 0 ? zz + g.mz : (zz >= g.mz ? zz - g.mz : zz);
@@ -39,11 +39,11 @@ FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz)
     y = y < 0 ? y + g.my : (y >= g.my ? y - g.my : y);
     uint32_t cell = (uint32_t)(x + g.mx * (y + g.my * z));
     float n[12];
-    draw_normals4(a.key, cell, FHD_STREAM_FLUID, 0, &n[0]);
-    draw_normals4(a.key, cell, FHD_STREAM_FLUID, 1, &n[4]);
+    draw_normals4(key, cell, FHD_STREAM_FLUID, 0, &n[0]);
+    draw_normals4(key, cell, FHD_STREAM_FLUID, 1, &n[4]);
     float s[6];
     if (a.noise == 2) {
-      draw_normals4(a.key, cell, FHD_STREAM_FLUID, 2, &n[8]);
+      draw_normals4(key, cell, FHD_STREAM_FLUID, 2, &n[8]);
 #pragma unroll
       for (int c = 0; c < 6; ++c) s[c] = n[c] + n[6 + c];
     } else {
@@ -65,6 +65,7 @@ FHD_DEV void gen_plane(const FluidWArgs& a, float* slot, int x0, int y0, int zz)
 __global__ __launch_bounds__(256) void k_fluid_w(FluidWArgs a) {
   __shared__ float nbuf[3][6 * PLANE];
   const GridDev& g = a.g;
+  const Key key = resolve_key(a.key);
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
   const int zs = blockIdx.z * a.zchunk;
   const int ze = min(zs + a.zchunk, g.mz);
@@ -73,13 +74,13 @@ __global__ __launch_bounds__(256) void k_fluid_w(FluidWArgs a) {
   const bool own = (x < g.mx) && (y < g.my);
   const int N = g.N;
   if (a.noise) {
-    gen_plane(a, nbuf[(zs - 1 + 3) % 3], x0, y0, zs - 1);
-    gen_plane(a, nbuf[zs % 3], x0, y0, zs);
+    gen_plane(a, key, nbuf[(zs - 1 + 3) % 3], x0, y0, zs - 1);
+    gen_plane(a, key, nbuf[zs % 3], x0, y0, zs);
   }
   const int xm = wrapi(x - 1, g.mx), xp = wrapi(x + 1, g.mx);
   const int ym = wrapi(y - 1, g.my), yp = wrapi(y + 1, g.my);
   for (int z = zs; z < ze; ++z) {
-    if (a.noise) gen_plane(a, nbuf[(z + 1) % 3], x0, y0, z + 1);
+    if (a.noise) gen_plane(a, key, nbuf[(z + 1) % 3], x0, y0, z + 1);
     __syncthreads();
     if (own) {
       const int zm = wrapi(z - 1, g.mz), zp = wrapi(z + 1, g.mz);
@@ -200,6 +201,14 @@ void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipS
 
 void launch_zero_f(float* p, size_t n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+// Advances the device-side step counter that graph-replayed steps key their noise with.
+__global__ void k_advance_step(unsigned long long* step) {
+  if (threadIdx.x == 0) *step += 1ull;
+}
+void launch_advance_step(unsigned long long* step, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, step);
 }
 
 void launch_fluid_w(const FluidWArgs& a, hipStream_t s) {

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot
     val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
   } else {
     float nrm[4];
-    rfd_normal(a.key, id, nrm);
+    rfd_normal(resolve_key(a.key), id, nrm);
     const float sg = sub == 0 ? 1.0f : -1.0f;
     val = make_float3(sg * G.coef * nrm[0], sg * G.coef * nrm[1], sg * G.coef * nrm[2]);
     dw = make_float3(d.x + sg * G.disp * nrm[0], d.y + sg * G.disp * nrm[1], d.z + sg * G.disp * nrm[2]);
@@ -731,11 +731,12 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
     disp[2] += a.aF * F.z;
   }
   if (a.nnoise > 0) {
+    const Key key = resolve_key(a.key);
     float n[4];
-    draw_normals4(a.key, (uint32_t)id, FHD_STREAM_PART1, 0, n);
+    draw_normals4(key, (uint32_t)id, FHD_STREAM_PART1, 0, n);
     float z[3] = {n[0], n[1], n[2]};
     if (a.nnoise > 1) {
-      draw_normals4(a.key, (uint32_t)id, FHD_STREAM_PART2, 0, n);
+      draw_normals4(key, (uint32_t)id, FHD_STREAM_PART2, 0, n);
       z[0] += n[0];
       z[1] += n[1];
       z[2] += n[2];

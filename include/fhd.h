@@ -105,6 +105,9 @@ int fhd_get_particle_velocities(fhd_ctx* c, double* u);  /* J v at the last step
 int fhd_get_forces(fhd_ctx* c, double* f);              /* last particle forces, 3*np */
 int fhd_step(fhd_ctx* c, int nsteps);                    /* enqueue nsteps steps (async) */
 int fhd_sync(fhd_ctx* c);                                /* wait, then check device error flags */
+/* Replay the steady-state step from captured HIP graphs (on != 0) instead of launching each
+ * kernel.  No fluam counterpart (fluam launches every kernel each step); results are identical. */
+int fhd_set_graph(fhd_ctx* c, int on);
 long long fhd_get_step(const fhd_ctx* c);
 /* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
  * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names

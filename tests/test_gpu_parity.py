@@ -260,3 +260,36 @@ def test_stokes_config3_full_size_steps():
     # checked relative to the velocity scale (FP32 round-off level)
     v = g.velocity()
     assert g.max_divergence() * 1.0 / np.abs(v).max() < 1e-6
+
+
+# ---------------- graph replay ----------------
+@pytest.mark.parametrize("scheme,np_", [(SCHEME_INCOMPRESSIBLE, 0), (SCHEME_STOKES_LIMIT, 300),
+                                        (SCHEME_QUASI_NEUTRALLY_BUOYANT, 200)])
+def test_graph_replay_matches_eager(scheme, np_):
+    """Steps replayed from captured graphs (noise step counter read from device memory) give the
+    same bits as eager launches; replays are checked one by one, with host work and syncs in
+    between, and across a switch back to eager and into graphs again."""
+    kw = dict(cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=61)
+    if np_:
+        kw.update(np=np_, kernel_points=3, initial_positions=1, **WCA)
+    p = make_params(scheme, **kw)
+    _announce(f"graph_replay_{scheme}", p)
+    e = F.Simulation(p)
+    g = F.Simulation(p)
+    g.set_graph(True)
+    for k in range(7):
+        e.step(1)
+        g.step(1)
+        g.sync()
+        assert np.array_equal(e.velocity(), g.velocity()), k
+        if np_:
+            assert np.array_equal(e.particles(), g.particles()), k
+    g.set_graph(False)
+    e.step(3)
+    g.step(3)
+    g.set_graph(True)
+    e.step(4)
+    g.step(4)
+    assert np.array_equal(e.velocity(), g.velocity())
+    if np_:
+        assert np.array_equal(e.particles(), g.particles())
