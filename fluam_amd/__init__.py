@@ -16,7 +16,7 @@ from ._abi import (FhdParams, make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_N
 __all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "get_params", "main",
            "scheme_incompressible", "scheme_quasi_neutrally_buoyant", "scheme_stokes_limit",
            "SCHEME_INCOMPRESSIBLE", "SCHEME_QUASI_NEUTRALLY_BUOYANT", "SCHEME_STOKES_LIMIT",
-           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH"]
+           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libfhd.so")
@@ -59,6 +59,22 @@ def lib():
     L.fhd_step.argtypes = [vp, C.c_int]
     L.fhd_sync.argtypes = [vp]
     L.fhd_set_graph.argtypes = [vp, C.c_int]
+    L.fhd_bytes_needed.argtypes = [C.POINTER(FhdParams)]
+    L.fhd_bytes_needed.restype = C.c_longlong
+    L.fhd_record.argtypes = [vp]
+    L.fhd_query.argtypes = [vp]
+    L.fhd_batch_create.argtypes = [C.POINTER(FhdParams), C.c_int]
+    L.fhd_batch_create.restype = vp
+    L.fhd_batch_destroy.argtypes = [vp]
+    L.fhd_batch_destroy.restype = None
+    L.fhd_batch_size.argtypes = [vp]
+    L.fhd_batch_replica.argtypes = [vp, C.c_int]
+    L.fhd_batch_replica.restype = vp
+    L.fhd_batch_error.argtypes = [vp]
+    L.fhd_batch_error.restype = C.c_char_p
+    L.fhd_batch_step.argtypes = [vp, C.c_int]
+    L.fhd_batch_sync.argtypes = [vp]
+    L.fhd_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.fhd_get_step.argtypes = [vp]
     L.fhd_get_step.restype = C.c_longlong
     L.fhd_time_steps.argtypes = [vp, C.c_int, dp]
@@ -124,17 +140,28 @@ def main(path="data.main"):
 class Simulation:
     """One re-entrant step context (all fields and particles resident on the GPU)."""
 
-    def __init__(self, params: FhdParams):
+    def __init__(self, params: FhdParams, handle=None):
         self.p = params.copy()
-        self.h = lib().fhd_create(C.byref(self.p))
+        self._owned = handle is None
+        self.h = lib().fhd_create(C.byref(self.p)) if handle is None else handle
         if not self.h:
             raise FhdError(_err())
         self.N = self.p.mx * self.p.my * self.p.mz
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and self._owned:
             lib().fhd_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def record(self):
+        """Mark the work enqueued so far; done() reports when it has completed."""
+        self._check(lib().fhd_record(self.h))
+
+    def done(self):
+        r = lib().fhd_query(self.h)
+        if r < 0:
+            raise FhdError(_err(self.h))
+        return r == 1
 
     def __del__(self):
         try:
@@ -237,3 +264,51 @@ def normals(seed, step, stream, sub, n):
     if not lib().fhd_normals(seed, step, stream, sub, n, _p(out)):
         raise FhdError(_err())
     return out.reshape(n, 4)
+
+
+def bytes_needed(params: FhdParams) -> int:
+    """Device bytes a Simulation with these parameters allocates."""
+    p = params.copy()
+    return int(lib().fhd_bytes_needed(C.byref(p)))
+
+
+class Batch:
+    """Independent replicas stepped together: each on its own stream with graph-replayed steps,
+    so their kernels overlap on the GPU.  Each replica's results equal those of a Simulation
+    with the same parameters run alone."""
+
+    def __init__(self, params_list):
+        n = len(params_list)
+        arr = (FhdParams * n)(*[p.copy() for p in params_list])
+        self.h = lib().fhd_batch_create(arr, n)
+        if not self.h:
+            raise FhdError(_err())
+        self.replicas = [Simulation(params_list[i], handle=lib().fhd_batch_replica(self.h, i)) for i in range(n)]
+
+    def _check(self, ok):
+        if not ok:
+            raise FhdError(lib().fhd_batch_error(self.h).decode())
+
+    def step(self, n=1):
+        self._check(lib().fhd_batch_step(self.h, n))
+
+    def sync(self):
+        self._check(lib().fhd_batch_sync(self.h))
+
+    def time_steps(self, n):
+        ms = C.c_double(0.0)
+        self._check(lib().fhd_batch_time_steps(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for r in self.replicas:
+                r.h = None
+            lib().fhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,9 +8,6 @@
 #include "fhd_kernels.h"
 #include "fft_common.h"
 
-#ifndef FFTB_TILE
-#define FFTB_TILE 8
-#endif
 
 namespace fhd {
 
@@ -287,13 +284,17 @@ static int column_tile(int P, int tmax) {
   return 1;
 }
 
-constexpr int kFftBTile = FFTB_TILE;  // kx columns per z-pass workgroup (8 or 16)
-
+// Column-tile width of the z pass: 4, 8 or 16 (chosen per grid shape by the context's
+// schedule), with one extra LDS column for the leftover Nyquist column.
 template <int MZ>
 static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
   const int P = a.g.P;
-  const int T = column_tile(P, kFftBTile);
-  hipLaunchKernelGGL((k_fftB<MZ, 256, kFftBTile + 1>), dim3(P / T, a.g.my), dim3(256), 0, s, a, spec, T);
+  const int bt = a.btile == 4 || a.btile == 16 ? a.btile : 8;
+  const int T = column_tile(P, bt);
+  const dim3 grid(P / T, a.g.my);
+  if (bt == 4) hipLaunchKernelGGL((k_fftB<MZ, 256, 5>), grid, dim3(256), 0, s, a, spec, T);
+  else if (bt == 8) hipLaunchKernelGGL((k_fftB<MZ, 256, 9>), grid, dim3(256), 0, s, a, spec, T);
+  else hipLaunchKernelGGL((k_fftB<MZ, 256, 17>), grid, dim3(256), 0, s, a, spec, T);
   return true;
 }
 

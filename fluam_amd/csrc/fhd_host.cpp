@@ -13,6 +13,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <array>
+#include <functional>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <fstream>
 #include <sstream>
@@ -124,6 +128,8 @@ struct fhd_ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int gperiod = 0, gphase = 0;
   bool own_stream = true;
+  int zchunk = 8;                        // W kernel: z planes per workgroup (schedule)
+  hipEvent_t done_ev = nullptr;          // fhd_record / fhd_query
 };
 
 namespace {
@@ -234,6 +240,7 @@ void build_tables(fhd_ctx* c) {
   make_plan(my, &a.plan_y, &a.nst_y);
   make_plan(mz, &a.plan_z, &a.nst_z);
   a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
+  a.btile = 8;
   a.zmode = 0;
   const char* gen = getenv("FHD_FFT_GENERIC");  // 1: use the runtime-size FFT kernels everywhere
   a.generic = (gen && gen[0] == '1') ? 1 : 0;
@@ -255,6 +262,7 @@ void free_ctx(fhd_ctx* c) {
   for (auto& e : c->gexec)
     if (e) (void)hipGraphExecDestroy(e);
   if (c->step_dev) (void)hipFree(c->step_dev);
+  if (c->done_ev) (void)hipEventDestroy(c->done_ev);
   if (c->s && c->own_stream) (void)hipStreamDestroy(c->s);
 }
 
@@ -308,8 +316,69 @@ FluidWArgs base_w(fhd_ctx* c) {
   w.g = c->g;
   w.key = make_key(c);
   w.W = c->W;
-  w.zchunk = 8;
+  w.zchunk = c->zchunk;
   return w;
+}
+
+// ---------------- schedule autotuner ----------------
+// The z-pass column tile (4, 8 or 16 columns) and the W kernel's z-chunk (4, 8 or 16 planes)
+// are timed once per grid shape on the first context of that shape and cached for the
+// process: a replica's schedule depends on its shape only, never on what else runs beside it,
+// and neither choice changes any result bit.
+struct Sched {
+  int btile, zchunk;
+};
+std::mutex g_sched_mu;
+std::map<std::array<int, 3>, Sched> g_sched;
+
+float time_launches(fhd_ctx* c, const std::function<void()>& f, int reps) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  f();  // warm-up
+  HIPCHK(hipEventRecord(a, c->s));
+  for (int k = 0; k < reps; ++k) f();
+  HIPCHK(hipEventRecord(b, c->s));
+  HIPCHK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+void apply_schedule(fhd_ctx* c) {
+  const std::array<int, 3> key = {c->p.mx, c->p.my, c->p.mz};
+  const char* env = getenv("FHD_AUTOTUNE");  // 0: fixed defaults (8 columns, 8 planes)
+  const bool tune = !(env && env[0] == '0');
+  std::lock_guard<std::mutex> lock(g_sched_mu);
+  auto it = g_sched.find(key);
+  Sched sc = {8, 8};
+  if (it != g_sched.end()) {
+    sc = it->second;
+  } else if (tune) {
+    // scratch runs on the context's own buffers before its state exists (contents irrelevant)
+    float best = 1e30f;
+    for (int bt : {4, 8, 16}) {
+      c->fa.btile = bt;
+      const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
+      if (t < best) { best = t; sc.btile = bt; }
+    }
+    best = 1e30f;
+    for (int zc : {4, 8, 16}) {
+      c->zchunk = zc;
+      FluidWArgs w = base_w(c);
+      w.force = c->SF;
+      w.force_coef = 1.0f;
+      w.noise = 2;
+      w.f1 = w.f4 = 1.0f;
+      const float t = time_launches(c, [&] { launch_fluid_w(w, c->s); }, 4);
+      if (t < best) { best = t; sc.zchunk = zc; }
+    }
+    g_sched[key] = sc;
+  }
+  c->fa.btile = sc.btile;
+  c->zchunk = sc.zchunk;
 }
 
 // ---------------- particles ----------------
@@ -974,6 +1043,7 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
     HIPCHK(hipMemcpy(c->v, v0.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->v2, 0, N3 * sizeof(float)));
     HIPCHK(hipMemset(c->SF, 0, N3 * sizeof(float)));
+    apply_schedule(c);
     c->np = p.np;
     if (c->np > 0) {
       Tiles& t = c->t;
@@ -1318,5 +1388,158 @@ int fhd_normals(unsigned long long seed, long long step, int stream, int sub, in
 
 // Internal: lets the scheme drivers report through fhd_last_error().
 void fhd_internal_set_error(const char* e) { g_last_error = e ? e : ""; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Batches of independent replicas and the completion / memory queries the serving loop uses.
+// No fluam counterpart: fluam runs one simulation per process.
+// ------------------------------------------------------------------------------------------
+struct fhd_batch {
+  std::vector<fhd_ctx*> reps;  // each replica owns its memory and stream
+  hipStream_t timer = nullptr;
+  std::string err;
+};
+
+namespace {
+
+// Device bytes a context with these parameters allocates (mirrors fhd_create).
+long long ctx_bytes(const fhd_params& p) {
+  const long long N = (long long)p.mx * p.my * p.mz, N3 = 3 * N;
+  const long long P = p.mx / 2 + 1;
+  long long b = 6 * N3 * 4 + N * 4 + 3LL * p.mz * p.my * P * 8;  // v v2 W SF adv[2], div, spectrum
+  b += 3LL * (p.mx + p.my + p.mz) * 20;                            // tables
+  if (p.np > 0) {
+    const long long np = p.np;
+    const int te[3] = {tile_edge(p.mx), tile_edge(p.my), tile_edge(p.mz)};
+    long long nt = 1;
+    const int m[3] = {p.mx, p.my, p.mz};
+    for (int a = 0; a < 3; ++a) nt *= te[a] > 0 ? m[a] / te[a] : 1;
+    b += 2 * np * (16 + 16 + 16 + 4 + 16 + 16) + 2 * (nt + 1) * 4;  // positions, forces, sorted sets
+    b += np * 16 + nt * 4 + 2 * np * 4;                               // u, counts, slots, scratch
+    b += 2 * 2 * np * ((long long)sizeof(SrcRec) + 16);               // spreading records
+  }
+  return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long fhd_bytes_needed(const fhd_params* p) { return p ? ctx_bytes(*p) : -1; }
+
+int fhd_record(fhd_ctx* c) {
+  CTX_TRY(c, {
+    if (!c->done_ev) HIPCHK(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(c->done_ev, c->s));
+  });
+  return 1;
+}
+
+int fhd_query(fhd_ctx* c) {
+  if (!c || !c->done_ev) return 1;
+  const hipError_t e = hipEventQuery(c->done_ev);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  c->err = hipGetErrorString(e);
+  return -1;
+}
+
+fhd_batch* fhd_batch_create(const fhd_params* p, int n) {
+  fhd_batch* b = new fhd_batch();
+  try {
+    if (n <= 0) throw std::runtime_error("a batch needs at least one replica");
+    for (int i = 0; i < n; ++i) {
+      fhd_ctx* c = fhd_create(&p[i]);
+      if (!c) throw std::runtime_error("replica " + std::to_string(i) + ": " + g_last_error);
+      c->graph_mode = true;
+      b->reps.push_back(c);
+    }
+    HIPCHK(hipStreamCreate(&b->timer));
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    for (fhd_ctx* c : b->reps) fhd_destroy(c);
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
+void fhd_batch_destroy(fhd_batch* b) {
+  if (!b) return;
+  for (fhd_ctx* c : b->reps) fhd_destroy(c);
+  if (b->timer) (void)hipStreamDestroy(b->timer);
+  delete b;
+}
+
+int fhd_batch_size(const fhd_batch* b) { return b ? (int)b->reps.size() : 0; }
+
+fhd_ctx* fhd_batch_replica(fhd_batch* b, int i) {
+  return (b && i >= 0 && i < (int)b->reps.size()) ? b->reps[i] : nullptr;
+}
+
+const char* fhd_batch_error(const fhd_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+// All replicas advance step by step, each on its own stream, so their kernels overlap.
+int fhd_batch_step(fhd_batch* b, int nsteps) {
+  if (!b) return 0;
+  try {
+    for (int k = 0; k < nsteps; ++k)
+      for (fhd_ctx* c : b->reps) advance(c);
+    HIPCHK(hipGetLastError());
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return 0;
+  }
+  return 1;
+}
+
+int fhd_batch_sync(fhd_batch* b) {
+  if (!b) return 0;
+  for (fhd_ctx* c : b->reps)
+    if (!fhd_sync(c)) {
+      b->err = c->err;
+      return 0;
+    }
+  return 1;
+}
+
+// Wall time (HIP events) of nsteps batch steps: every replica stream starts after one event
+// and the timer stream waits for all of them to finish.
+int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms) {
+  if (!b) return 0;
+  std::vector<hipEvent_t> ends(b->reps.size(), nullptr);
+  hipEvent_t a = nullptr, e = nullptr;
+  try {
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&e));
+    for (auto& x : ends) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+    for (fhd_ctx* c : b->reps) HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipEventRecord(a, b->timer));
+    for (fhd_ctx* c : b->reps) HIPCHK(hipStreamWaitEvent(c->s, a, 0));
+    for (int k = 0; k < nsteps; ++k)
+      for (fhd_ctx* c : b->reps) advance(c);
+    for (size_t i = 0; i < b->reps.size(); ++i) {
+      HIPCHK(hipEventRecord(ends[i], b->reps[i]->s));
+      HIPCHK(hipStreamWaitEvent(b->timer, ends[i], 0));
+    }
+    HIPCHK(hipEventRecord(e, b->timer));
+    HIPCHK(hipEventSynchronize(e));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, a, e));
+    *ms = t;
+  } catch (const std::exception& ex) {
+    b->err = ex.what();
+    for (auto& x : ends)
+      if (x) (void)hipEventDestroy(x);
+    if (a) (void)hipEventDestroy(a);
+    if (e) (void)hipEventDestroy(e);
+    return 0;
+  }
+  for (auto& x : ends) (void)hipEventDestroy(x);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(e);
+  return 1;
+}
 
 }  // extern "C"

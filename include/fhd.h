@@ -119,6 +119,28 @@ int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_name
  * 1/time (discrete staggered divergence, src/kernelUpdateVIncompressible.cu:21-72 invariant). */
 int fhd_max_divergence(fhd_ctx* c, double* maxdiv);
 
+/* ---------------- batches and serving (no fluam counterpart) ----------------
+ * For callers that run many independent simulations on one GPU: ensembles of replicas and
+ * trajectory-serving loops.  fluam runs one simulation per process. */
+/* Device bytes fhd_create would allocate for these parameters (admission against a budget). */
+long long fhd_bytes_needed(const fhd_params* p);
+/* Mark the work enqueued so far on the context; fhd_query returns 1 once it has completed,
+ * 0 while it runs, -1 on a device error. */
+int fhd_record(fhd_ctx* c);
+int fhd_query(fhd_ctx* c);
+/* n replicas (own parameters, shapes and seeds), each on its own stream with graph-replayed
+ * steps; fhd_batch_step advances all of them step by step so their kernels overlap.
+ * Results equal those of each replica run alone. */
+typedef struct fhd_batch fhd_batch;
+fhd_batch* fhd_batch_create(const fhd_params* p, int n);   /* NULL on error (fhd_last_error) */
+void fhd_batch_destroy(fhd_batch* b);
+int fhd_batch_size(const fhd_batch* b);
+fhd_ctx* fhd_batch_replica(fhd_batch* b, int i);           /* owned by the batch */
+const char* fhd_batch_error(const fhd_batch* b);
+int fhd_batch_step(fhd_batch* b, int nsteps);
+int fhd_batch_sync(fhd_batch* b);
+int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms);
+
 /* ---------------- building blocks (exposed for parity tests) ---------------- */
 /* Batched 3D real FFT round trip on device: real field (mz,my,mx) -> half spectrum
  * (mz,my,mx/2+1) complex (interleaved re,im) -> back; unnormalised both ways. */

@@ -293,3 +293,46 @@ def test_graph_replay_matches_eager(scheme, np_):
     assert np.array_equal(e.velocity(), g.velocity())
     if np_:
         assert np.array_equal(e.particles(), g.particles())
+
+
+# ---------------- batches and schedules ----------------
+def _replica_params(i, shape, np_, scheme=SCHEME_STOKES_LIMIT):
+    kw = dict(cells=shape, temperature=1e-3, dt=0.05, seed=100 + i)
+    if np_:
+        # simple-cubic starts: no overlapping pairs, so every replica stays finite
+        kw.update(np=np_, kernel_points=3, initial_positions=1, **WCA)
+    return make_params(scheme, **kw)
+
+
+def test_batch_replicas_equal_alone():
+    """Replicas of uneven shapes, particle counts and schemes stepped together in one batch
+    (own streams, graph replay) give the same bits as each run alone with eager launches."""
+    specs = [((16, 24, 32), 100, SCHEME_STOKES_LIMIT), ((32, 32, 32), 0, SCHEME_STOKES_LIMIT),
+             ((24, 16, 48), 300, SCHEME_QUASI_NEUTRALLY_BUOYANT), ((32, 32, 32), 50, SCHEME_STOKES_LIMIT),
+             ((16, 16, 16), 0, SCHEME_INCOMPRESSIBLE)]
+    params = [_replica_params(i, s, n, sc) for i, (s, n, sc) in enumerate(specs)]
+    for p in params:
+        _announce("batch_replica", p)
+    b = F.Batch(params)
+    b.step(12)
+    b.sync()
+    for p, r in zip(params, b.replicas):
+        alone = F.Simulation(p)
+        alone.step(12)
+        assert np.array_equal(alone.velocity(), r.velocity())
+        if p.np:
+            assert np.array_equal(alone.particles(), r.particles())
+    b.close()
+
+
+def test_autotuned_schedule_changes_no_bits():
+    """The per-shape schedule (z-pass column tile, W-kernel z chunk) changes timing only."""
+    p = _replica_params(7, (48, 48, 24), 200)
+    os.environ["FHD_AUTOTUNE"] = "0"
+    fixed = F.Simulation(p)
+    os.environ["FHD_AUTOTUNE"] = "1"
+    tuned = F.Simulation(p)
+    fixed.step(6)
+    tuned.step(6)
+    assert np.array_equal(fixed.velocity(), tuned.velocity())
+    assert np.array_equal(fixed.particles(), tuned.particles())
