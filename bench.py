@@ -80,18 +80,134 @@ def cpu_baseline(args):
                       f"OpenMP {threads} threads, {dt:.2f} s/step"}
 
 
+def plain_back_to_back(F, params_steps):
+    """Seconds to run each job alone, one after another, with eager launches (the baseline the
+    batched and served ensembles are compared with): per job, a fresh context, two untimed
+    steps, then min(steps, 20) timed steps scaled to the job's step count."""
+    total = 0.0
+    for p, steps in params_steps:
+        s = F.Simulation(p)
+        s.step(2)
+        s.sync()
+        n = min(steps, 20)
+        total += s.time_steps(n) / n * steps * 1e-3
+        s.close()
+    return total
+
+
+def bench_batched(args):
+    """256 independent Stokes-limit replicas, grid edges drawn per axis from {16,24,32,48,64},
+    0-2048 seeded-random particles, 3-point kernel, own seeds, graph-replayed steps."""
+    import numpy as np
+    import fluam_amd as F
+    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
+
+    rng = np.random.default_rng(4)
+    edges = (16, 24, 32, 48, 64)
+    params = []
+    for i in range(args.replicas):
+        shape = tuple(int(rng.choice(edges)) for _ in range(3))
+        params.append(make_params(SCHEME_STOKES_LIMIT, cells=shape, np=int(rng.integers(0, 2049)), kernel_points=3,
+                                  temperature=1e-3, dt=0.05, seed=1000 + i, initial_positions=0,
+                                  cutoff=2 ** (1 / 6), wca_sigma=1.0))
+    cells = sum(p.mx * p.my * p.mz for p in params)
+    parts = sum(p.np for p in params)
+    b = F.Batch(params)
+    b.set_mode(args.batch_mode)
+    b.step(args.warmup)
+    b.sync()
+    ms = b.time_steps(args.steps)
+    b.sync()
+    b.close()
+    value = cells * args.steps / (ms * 1e-3)
+    plain_s = plain_back_to_back(F, [(p, args.steps) for p in params])
+    plain = cells * args.steps / plain_s
+    return {
+        "metric": "batched-ensemble aggregate cell-updates/s", "value": value, "unit": "cell-updates/s",
+        "particle_steps_per_s": parts * args.steps / (ms * 1e-3), "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded grid shapes, particle counts and uniform-random positions; zero initial velocity",
+        "config": {"workload": f"batched ensemble: {args.replicas} Stokes-limit replicas, edges per axis from "
+                               "{16,24,32,48,64}, 0-2048 particles, 3-point kernel, own seeds, graph-replayed steps",
+                   "replicas": args.replicas, "total_cells": cells, "total_particles": parts,
+                   "batch_mode": "one graph per batch step" if args.batch_mode else "per-replica graphs"},
+        "back_to_back_plain_cell_updates_per_s": plain,
+        "speedup_vs_back_to_back": value / plain,
+    }
+
+
+def bench_served(args):
+    """Poisson arrivals of Stokes-limit and inertial-coupling (QNB) jobs on cubic grids with edge
+    from {32,64,96,128}, seeded particle counts and 50-500 steps, served by the continuous-
+    batching loop; p50/p99 latency beside aggregate cell-updates/s."""
+    import numpy as np
+    import fluam_amd as F
+    from fluam_amd import serve
+    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT, SCHEME_QUASI_NEUTRALLY_BUOYANT
+
+    def make(i, rng):
+        edge = int(rng.choice((32, 64, 96, 128)))
+        N = edge ** 3
+        qnb = bool(rng.integers(0, 2))
+        # at most one particle per cell (uniform draw up to 65,536); QNB starts on a lattice
+        npart = int(rng.integers(0, min(65536, N) + 1))
+        p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT if qnb else SCHEME_STOKES_LIMIT, cells=(edge, edge, edge),
+                        np=npart, kernel_points=3 if qnb else 4, temperature=1e-3, dt=0.05, seed=2000 + i,
+                        initial_positions=1 if qnb else 0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+        return p, int(rng.integers(50, 501))
+
+    jobs = serve.poisson_jobs(args.rate, args.jobs, 5, make)
+    res = serve.Server().run(jobs)
+    plain_s = plain_back_to_back(F, [(j.params, j.steps) for j in jobs])
+    total = sum(j.cell_updates for j in jobs)
+    plain = total / plain_s
+    return {
+        "metric": "served trajectory jobs: aggregate cell-updates/s, p50/p99 job latency",
+        "value": res["cell_updates_per_s"], "unit": "cell-updates/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded job mix and Poisson arrivals",
+        "config": {"workload": f"{args.jobs} jobs, Poisson arrivals at {args.rate}/s, cubic edges from "
+                               "{32,64,96,128}, Stokes (4-point) or QNB (3-point), 50-500 steps",
+                   "jobs": args.jobs, "rate_per_s": args.rate},
+        "p50_latency_s": res["p50_latency_s"], "p99_latency_s": res["p99_latency_s"],
+        "mean_latency_s": res["mean_latency_s"], "wall_s": res["wall_s"],
+        "back_to_back_plain_cell_updates_per_s": plain,
+        "speedup_vs_back_to_back": res["cell_updates_per_s"] / plain,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--workload", choices=("stokes128", "batched", "served"), default="stokes128",
+                    help="stokes128 (default): the headline step; batched / served: the ensemble extensions")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--jobs", type=int, default=24)
+    ap.add_argument("--rate", type=float, default=8.0, help="served: job arrivals per second")
+    ap.add_argument("--batch-mode", type=int, default=1,
+                    help="batched: 1 = one captured graph per batch step, 0 = per-replica graphs")
+    ap.add_argument("--hw-queues", type=int, default=None,
+                    help="hardware queues for the ensemble workloads (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the step is a single-GPU kernel pipeline "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one GPU only: the step is a single-GPU kernel pipeline; "
+              "how it would shard over several GPUs is a design note, not a run", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "stokes128":
+        if args.hw_queues is not None:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 16)))
+        if args.steps is None:
+            args.steps = 200
+        line = bench_batched(args) if args.workload == "batched" else bench_served(args)
+        print(json.dumps(line))
+        return
+    if args.steps is None:
+        args.steps = 50
 
     import fluam_amd as F
     from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT

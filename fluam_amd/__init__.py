@@ -16,7 +16,7 @@ from ._abi import (FhdParams, make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_N
 __all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "get_params", "main",
            "scheme_incompressible", "scheme_quasi_neutrally_buoyant", "scheme_stokes_limit",
            "SCHEME_INCOMPRESSIBLE", "SCHEME_QUASI_NEUTRALLY_BUOYANT", "SCHEME_STOKES_LIMIT",
-           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed"]
+           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed", "mem_info"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libfhd.so")
@@ -61,6 +61,7 @@ def lib():
     L.fhd_set_graph.argtypes = [vp, C.c_int]
     L.fhd_bytes_needed.argtypes = [C.POINTER(FhdParams)]
     L.fhd_bytes_needed.restype = C.c_longlong
+    L.fhd_mem_info.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
     L.fhd_record.argtypes = [vp]
     L.fhd_query.argtypes = [vp]
     L.fhd_batch_create.argtypes = [C.POINTER(FhdParams), C.c_int]
@@ -73,6 +74,7 @@ def lib():
     L.fhd_batch_error.argtypes = [vp]
     L.fhd_batch_error.restype = C.c_char_p
     L.fhd_batch_step.argtypes = [vp, C.c_int]
+    L.fhd_batch_set_mode.argtypes = [vp, C.c_int]
     L.fhd_batch_sync.argtypes = [vp]
     L.fhd_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.fhd_get_step.argtypes = [vp]
@@ -266,6 +268,14 @@ def normals(seed, step, stream, sub, n):
     return out.reshape(n, 4)
 
 
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_ulonglong(0), C.c_ulonglong(0)
+    if not lib().fhd_mem_info(C.byref(f), C.byref(t)):
+        raise FhdError(_err())
+    return f.value, t.value
+
+
 def bytes_needed(params: FhdParams) -> int:
     """Device bytes a Simulation with these parameters allocates."""
     p = params.copy()
@@ -288,6 +298,10 @@ class Batch:
     def _check(self, ok):
         if not ok:
             raise FhdError(lib().fhd_batch_error(self.h).decode())
+
+    def set_mode(self, mode):
+        """0: per-replica graph replay on each replica's stream; 1: one graph for the batch step."""
+        self._check(lib().fhd_batch_set_mode(self.h, mode))
 
     def step(self, n=1):
         self._check(lib().fhd_batch_step(self.h, n))

@@ -1399,6 +1399,14 @@ struct fhd_batch {
   std::vector<fhd_ctx*> reps;  // each replica owns its memory and stream
   hipStream_t timer = nullptr;
   std::string err;
+  // whole-batch graphs: one graph per phase holding every replica's step as an independent
+  // branch (fork from / join to the timer stream), replayed with a single launch per step
+  int mode = 0;  // 0: each replica replays its own graphs on its stream; 1: one batch graph
+  bool built = false;
+  int period = 1, phase = 0;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 namespace {
@@ -1427,6 +1435,17 @@ long long ctx_bytes(const fhd_params& p) {
 extern "C" {
 
 long long fhd_bytes_needed(const fhd_params* p) { return p ? ctx_bytes(*p) : -1; }
+
+int fhd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes) {
+  size_t f = 0, t = 0;
+  if (hipMemGetInfo(&f, &t) != hipSuccess) {
+    g_last_error = "hipMemGetInfo failed";
+    return 0;
+  }
+  *free_bytes = f;
+  *total_bytes = t;
+  return 1;
+}
 
 int fhd_record(fhd_ctx* c) {
   CTX_TRY(c, {
@@ -1467,6 +1486,11 @@ fhd_batch* fhd_batch_create(const fhd_params* p, int n) {
 
 void fhd_batch_destroy(fhd_batch* b) {
   if (!b) return;
+  for (auto& e : b->exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  for (auto& e : b->join)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
   for (fhd_ctx* c : b->reps) fhd_destroy(c);
   if (b->timer) (void)hipStreamDestroy(b->timer);
   delete b;
@@ -1480,12 +1504,92 @@ fhd_ctx* fhd_batch_replica(fhd_batch* b, int i) {
 
 const char* fhd_batch_error(const fhd_batch* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
 
+int fhd_batch_set_mode(fhd_batch* b, int mode) {
+  if (!b) return 0;
+  b->mode = mode ? 1 : 0;
+  return 1;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Capture one graph per phase for the whole batch: the timer stream forks to every replica
+// stream, each replica records its steady-state step there, and the timer stream joins them.
+void build_batch_graphs(fhd_batch* b) {
+  b->period = 1;
+  for (fhd_ctx* c : b->reps)
+    if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) b->period = 2;
+  if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  if (b->join.empty()) {
+    b->join.resize(b->reps.size(), nullptr);
+    for (auto& e : b->join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  for (fhd_ctx* c : b->reps) {
+    if (!c->step_dev) c->step_dev = dalloc<unsigned long long>(1);
+    HIPCHK(hipStreamSynchronize(c->s));
+    const unsigned long long v = (unsigned long long)c->step;
+    HIPCHK(hipMemcpy(c->step_dev, &v, sizeof(v), hipMemcpyHostToDevice));
+    c->dev_step_valid = true;
+  }
+  for (int ph = 0; ph < b->period; ++ph) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->timer, hipStreamCaptureModeGlobal));
+    HIPCHK(hipEventRecord(b->fork, b->timer));
+    for (size_t i = 0; i < b->reps.size(); ++i) {
+      fhd_ctx* c = b->reps[i];
+      HIPCHK(hipStreamWaitEvent(c->s, b->fork, 0));
+      c->capturing = true;
+      scheme_step(c);
+      launch_advance_step(c->step_dev, c->s);
+      c->capturing = false;
+      HIPCHK(hipEventRecord(b->join[i], c->s));
+      HIPCHK(hipStreamWaitEvent(b->timer, b->join[i], 0));
+    }
+    HIPCHK(hipStreamEndCapture(b->timer, &g));
+    HIPCHK(hipGraphInstantiate(&b->exec[ph], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  // each replica's host state went through `period` steps: back where it started for QNB
+  // (period 2) and unchanged for the other schemes
+  b->phase = 0;
+  b->built = true;
+}
+
+void batch_advance(fhd_batch* b) {
+  bool all_steady = true;
+  for (fhd_ctx* c : b->reps) all_steady = all_steady && steady(c);
+  if (b->mode == 0 || !all_steady) {
+    for (fhd_ctx* c : b->reps) advance(c);
+    if (b->built) b->phase = (b->phase + 1) % b->period;  // host state moved one step
+    return;
+  }
+  if (!b->built) build_batch_graphs(b);
+  for (fhd_ctx* c : b->reps)
+    if (!c->dev_step_valid) {  // an eager step ran since the counter was last on the device
+      HIPCHK(hipStreamSynchronize(c->s));
+      const unsigned long long v = (unsigned long long)c->step;
+      HIPCHK(hipMemcpy(c->step_dev, &v, sizeof(v), hipMemcpyHostToDevice));
+      c->dev_step_valid = true;
+    }
+  HIPCHK(hipGraphLaunch(b->exec[b->phase], b->timer));
+  for (fhd_ctx* c : b->reps) {
+    if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) qnb_after_step(c);
+    c->step++;
+    if (c->graphs_built) c->gphase = (c->gphase + 1) % c->gperiod;
+  }
+  b->phase = (b->phase + 1) % b->period;
+}
+
+}  // namespace
+
+extern "C" {
+
 // All replicas advance step by step, each on its own stream, so their kernels overlap.
 int fhd_batch_step(fhd_batch* b, int nsteps) {
   if (!b) return 0;
   try {
-    for (int k = 0; k < nsteps; ++k)
-      for (fhd_ctx* c : b->reps) advance(c);
+    for (int k = 0; k < nsteps; ++k) batch_advance(b);
     HIPCHK(hipGetLastError());
   } catch (const std::exception& e) {
     b->err = e.what();
@@ -1517,8 +1621,7 @@ int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms) {
     for (fhd_ctx* c : b->reps) HIPCHK(hipStreamSynchronize(c->s));
     HIPCHK(hipEventRecord(a, b->timer));
     for (fhd_ctx* c : b->reps) HIPCHK(hipStreamWaitEvent(c->s, a, 0));
-    for (int k = 0; k < nsteps; ++k)
-      for (fhd_ctx* c : b->reps) advance(c);
+    for (int k = 0; k < nsteps; ++k) batch_advance(b);
     for (size_t i = 0; i < b->reps.size(); ++i) {
       HIPCHK(hipEventRecord(ends[i], b->reps[i]->s));
       HIPCHK(hipStreamWaitEvent(b->timer, ends[i], 0));

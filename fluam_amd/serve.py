@@ -1,0 +1,141 @@
+"""Trajectory serving: a continuous-batching request loop over the C ABI.
+
+Jobs (a parameter set and a step count) arrive over time.  The loop admits a job as soon as it
+has arrived and its device memory (fhd_bytes_needed) fits in the budget, keeps its fields,
+particles and noise counter resident in its own context (own stream, graph-replayed steps), and
+advances every admitted job by one step per round, so jobs join and leave at step boundaries
+and their kernels overlap on the GPU.  A job's result does not depend on what runs beside it:
+every kernel is deterministic and the noise is keyed by the job's own seed and step.
+
+Reports per-job latency (arrival to completion observed by the loop), p50/p99 and the
+aggregate cell-updates/s.  No fluam counterpart: fluam runs one simulation per process.
+"""
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import Simulation, bytes_needed, mem_info
+from ._abi import FhdParams
+
+
+@dataclass
+class Job:
+    name: str
+    params: FhdParams
+    steps: int
+    arrival: float = 0.0                 # seconds after the loop starts
+    # filled by the loop
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    velocity: Optional[np.ndarray] = None
+    particles: Optional[np.ndarray] = None
+    _sim: Optional[Simulation] = field(default=None, repr=False)
+    _issued: int = 0
+    _mark: int = 0
+    _bytes: int = 0
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+    @property
+    def cell_updates(self):
+        return self.params.mx * self.params.my * self.params.mz * self.steps
+
+
+class Server:
+    """Continuous-batching loop.  budget: device bytes the jobs may hold together (default 80%
+    of the free device memory).  max_inflight: rounds a job may run ahead of its completed work,
+    which bounds the host's lead over the GPU.  keep_results: copy each job's final velocity and
+    particles back before its memory is released."""
+
+    def __init__(self, budget: Optional[int] = None, max_inflight: int = 4, keep_results: bool = False,
+                 graphs: bool = True):
+        if budget is None:
+            free, _ = mem_info()
+            budget = int(0.8 * free)
+        self.budget = budget
+        self.max_inflight = max_inflight
+        self.keep_results = keep_results
+        self.graphs = graphs
+
+    def run(self, jobs: List[Job]) -> dict:
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        for j in pending:
+            j._bytes = bytes_needed(j.params)
+            if j._bytes > self.budget:
+                raise ValueError(f"job {j.name} needs {j._bytes} bytes, more than the budget {self.budget}")
+        active: List[Job] = []
+        used = 0
+        t0 = time.perf_counter()
+        done: List[Job] = []
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission: arrived jobs, in arrival order, while memory allows
+            while pending and pending[0].arrival <= now and used + pending[0]._bytes <= self.budget:
+                j = pending.pop(0)
+                j._sim = Simulation(j.params)
+                j._sim.set_graph(self.graphs)
+                j.admitted = time.perf_counter() - t0
+                used += j._bytes
+                active.append(j)
+            # one round: every admitted job advances one step.  Run-ahead is bounded: once a
+            # job has issued max_inflight steps past its last marker, it waits (without blocking
+            # the others) until the GPU has reached that marker.
+            progressed = False
+            for j in active:
+                if j._issued >= j.steps:
+                    continue
+                if j._issued - j._mark >= self.max_inflight:
+                    if not j._sim.done():
+                        continue
+                    j._sim.record()
+                    j._mark = j._issued
+                j._sim.step(1)
+                j._issued += 1
+                progressed = True
+                if j._issued == j.steps:
+                    j._sim.record()
+                    j._mark = j._issued
+            # retire jobs whose last step has completed
+            still = []
+            for j in active:
+                if j._issued == j.steps and j._sim.done():
+                    j._sim.sync()
+                    j.finished = time.perf_counter() - t0
+                    if self.keep_results:
+                        j.velocity = j._sim.velocity()
+                        if j.params.np > 0:
+                            j.particles = j._sim.particles()
+                    j._sim.close()
+                    j._sim = None
+                    used -= j._bytes
+                    done.append(j)
+                    progressed = True
+                else:
+                    still.append(j)
+            active = still
+            if active and not progressed:
+                time.sleep(0.0002)
+            if not active and pending:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+        wall = time.perf_counter() - t0
+        lat = np.array([j.latency for j in done])
+        total = sum(j.cell_updates for j in done)
+        return {"jobs": len(done), "wall_s": wall, "cell_updates_per_s": total / wall,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                "mean_latency_s": float(lat.mean())}
+
+
+def poisson_jobs(rate: float, n: int, seed: int, make) -> List[Job]:
+    """n jobs with exponential inter-arrival times (rate per second); make(i, rng) -> (params, steps)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        params, steps = make(i, rng)
+        jobs.append(Job(name=f"job{i}", params=params, steps=steps, arrival=t))
+    return jobs

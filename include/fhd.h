@@ -124,6 +124,8 @@ int fhd_max_divergence(fhd_ctx* c, double* maxdiv);
  * trajectory-serving loops.  fluam runs one simulation per process. */
 /* Device bytes fhd_create would allocate for these parameters (admission against a budget). */
 long long fhd_bytes_needed(const fhd_params* p);
+/* Free and total device memory of the current GPU. */
+int fhd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
 /* Mark the work enqueued so far on the context; fhd_query returns 1 once it has completed,
  * 0 while it runs, -1 on a device error. */
 int fhd_record(fhd_ctx* c);
@@ -137,6 +139,9 @@ void fhd_batch_destroy(fhd_batch* b);
 int fhd_batch_size(const fhd_batch* b);
 fhd_ctx* fhd_batch_replica(fhd_batch* b, int i);           /* owned by the batch */
 const char* fhd_batch_error(const fhd_batch* b);
+/* mode 0: every replica replays its own graphs on its stream; mode 1: the whole batch step is
+ * one captured graph (each replica an independent branch), replayed with one launch. */
+int fhd_batch_set_mode(fhd_batch* b, int mode);
 int fhd_batch_step(fhd_batch* b, int nsteps);
 int fhd_batch_sync(fhd_batch* b);
 int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms);

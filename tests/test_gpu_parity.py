@@ -304,9 +304,11 @@ def _replica_params(i, shape, np_, scheme=SCHEME_STOKES_LIMIT):
     return make_params(scheme, **kw)
 
 
-def test_batch_replicas_equal_alone():
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_replicas_equal_alone(mode):
     """Replicas of uneven shapes, particle counts and schemes stepped together in one batch
-    (own streams, graph replay) give the same bits as each run alone with eager launches."""
+    (mode 0: each replica's own graphs on its stream; mode 1: one graph for the batch step) give
+    the same bits as each run alone with eager launches."""
     specs = [((16, 24, 32), 100, SCHEME_STOKES_LIMIT), ((32, 32, 32), 0, SCHEME_STOKES_LIMIT),
              ((24, 16, 48), 300, SCHEME_QUASI_NEUTRALLY_BUOYANT), ((32, 32, 32), 50, SCHEME_STOKES_LIMIT),
              ((16, 16, 16), 0, SCHEME_INCOMPRESSIBLE)]
@@ -314,7 +316,10 @@ def test_batch_replicas_equal_alone():
     for p in params:
         _announce("batch_replica", p)
     b = F.Batch(params)
-    b.step(12)
+    b.set_mode(mode)
+    b.step(5)
+    b.sync()
+    b.step(7)
     b.sync()
     for p, r in zip(params, b.replicas):
         alone = F.Simulation(p)
