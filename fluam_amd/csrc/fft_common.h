@@ -66,10 +66,35 @@ template <> struct Dft<8> {
   }
 };
 
-// Radix plan: 8s first, then 4, 2, 3 (the same order the host uses for the generic kernels).
+template <> struct Dft<16> {
+  static FHD_DEV void run(float2* v, float s) {
+    float2 e[8], o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      e[k] = v[2 * k];
+      o[k] = v[2 * k + 1];
+    }
+    Dft<8>::run(e, s);
+    Dft<8>::run(o, s);
+    // w^k = e^{s 2 pi i k / 16}: cos and sin of k pi / 8
+    const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+    const float h = 0.70710678118654752440f;
+    const float2 w[8] = {make_float2(1.0f, 0.0f), make_float2(c1, s * s1), make_float2(h, s * h),
+                         make_float2(s1, s * c1), make_float2(0.0f, s), make_float2(-s1, s * c1),
+                         make_float2(-h, s * h), make_float2(-c1, s * s1)};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 t = k == 0 ? o[0] : cmul(o[k], w[k]);
+      v[k] = cadd(e[k], t);
+      v[k + 8] = csub(e[k], t);
+    }
+  }
+};
+
+// Radix plan of the size-specialised kernels: 16s first, then 8, 4, 2, 3 (fewest LDS stages).
 constexpr int plan_radix(int n, int st) {
   for (int i = 0;; ++i) {
-    int r = (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : 1;
+    int r = (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : 1;
     if (i == st || r == 1) return r;
     n /= r;
   }
