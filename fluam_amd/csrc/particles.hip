@@ -685,41 +685,24 @@ FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const Ax
     for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[xs[kx]];
     sy += wy.w[ky] * sx;
   }
-  // kz is a lane index: select the weight instead of indexing (keeps wz in registers)
-  const float wk = kz == 0 ? wz.w[0] : kz == 1 ? wz.w[1] : kz == 2 ? wz.w[2] : wz.w[3];
-  return wk * sy;
+  return sy;  // weighted by wz.w[kz] when the rows are combined (static indices, no scratch)
+}
+
+// Combine the four z-rows held by lanes 4p .. 4p+3 into lane 4p: sum_k wz[k] row_k.  Fixed
+// order, so the result does not depend on scheduling.
+FHD_DEV float combine_rows(float row, const AxisWeights& wz) {
+  const int base = threadIdx.x & ~3;
+  const float r0 = __shfl(row, base), r1 = __shfl(row, base + 1), r2 = __shfl(row, base + 2),
+              r3 = __shfl(row, base + 3);
+  return ((wz.w[0] * r0 + wz.w[1] * r1) + wz.w[2] * r2) + wz.w[3] * r3;
 }
 
 // Four lanes per particle: lane q sums z-row q of the stencil; the rows are combined by a fixed
 // xor-shuffle tree, then lane 0 applies the position update.
-template <int NPTS>
-__global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
-  const int gid = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int s = gid >> 2, q = gid & 3;
+// Position update of sorted particle s from its interpolated velocities u1 = J f1, u2 = J f2:
+// out = base + a1 u1 + a2 u2 + extra mobility + particle noise (by particle index).
+FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const float u2[3]) {
   const GridDev& g = a.g;
-  float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
-  if (s < a.np && q < NPTS) {
-    const int4 c = a.eval.cell[s];
-    const float4 d = a.eval.off[s];
-    SrcWeights w;
-    source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
-#pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
-      const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
-      const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
-      const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
-      if (a.f1) u1[comp] = interp_row<NPTS>(a.f1 + (size_t)comp * g.N, g, wx, wy, wz, q);
-      if (a.f2) u2[comp] = interp_row<NPTS>(a.f2 + (size_t)comp * g.N, g, wx, wy, wz, q);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    u1[k] += __shfl_xor(u1[k], 1);
-    u2[k] += __shfl_xor(u2[k], 1);
-    u1[k] += __shfl_xor(u1[k], 2);
-    u2[k] += __shfl_xor(u2[k], 2);
-  }
-  if (s >= a.np || q != 0) return;
   const int id = a.eval.id[s];
   float disp[3];
 #pragma unroll
@@ -764,6 +747,34 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
     a.out_cell[id] = bc;
     a.out_off[id] = make_float4(ox, oy, oz, 0.0f);
   }
+}
+
+template <int NPTS>
+__global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
+  const int gid = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int s = gid >> 2, q = gid & 3;
+  const GridDev& g = a.g;
+  float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
+  SrcWeights w;
+  const bool act = s < a.np && q < NPTS;
+  if (s < a.np) {
+    const int4 c = a.eval.cell[s];
+    const float4 d = a.eval.off[s];
+    source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
+  }
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
+    const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
+    const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
+    float r1 = 0.0f, r2 = 0.0f;
+    if (act && a.f1) r1 = interp_row<NPTS>(a.f1 + (size_t)comp * g.N, g, wx, wy, wz, q);
+    if (act && a.f2) r2 = interp_row<NPTS>(a.f2 + (size_t)comp * g.N, g, wx, wy, wz, q);
+    u1[comp] = combine_rows(r1, wz);
+    u2[comp] = combine_rows(r2, wz);
+  }
+  if (s >= a.np || q != 0) return;
+  interp_finish(a, s, u1, u2);
 }
 
 __global__ void k_zero_f4(float4* p, int n) {
