@@ -131,7 +131,8 @@ def bench_batched(args):
         "config": {"workload": f"batched ensemble: {args.replicas} Stokes-limit replicas, edges per axis from "
                                "{16,24,32,48,64}, 0-2048 particles, 3-point kernel, own seeds, graph-replayed steps",
                    "replicas": args.replicas, "total_cells": cells, "total_particles": parts,
-                   "batch_mode": "one graph per batch step" if args.batch_mode else "per-replica graphs"},
+                   "batch_mode": ("per-replica graphs", "one graph per batch step, a branch per replica",
+                                  "merged stages: one batched dispatch per stage and kernel instance")[args.batch_mode]},
         "back_to_back_plain_cell_updates_per_s": plain,
         "speedup_vs_back_to_back": value / plain,
     }
@@ -189,8 +190,9 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=8.0, help="served: job arrivals per second")
-    ap.add_argument("--batch-mode", type=int, default=1,
-                    help="batched: 1 = one captured graph per batch step, 0 = per-replica graphs")
+    ap.add_argument("--batch-mode", type=int, default=2, choices=(0, 1, 2),
+                    help="batched: 0 = per-replica graphs, 1 = one captured graph per batch step, "
+                         "2 = merged stages (one batched dispatch per stage and kernel instance)")
     ap.add_argument("--hw-queues", type=int, default=None,
                     help="hardware queues for the ensemble workloads (HIP reads GPU_MAX_HW_QUEUES at start-up)")
     args = ap.parse_args()

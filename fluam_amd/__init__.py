@@ -300,7 +300,9 @@ class Batch:
             raise FhdError(lib().fhd_batch_error(self.h).decode())
 
     def set_mode(self, mode):
-        """0: per-replica graph replay on each replica's stream; 1: one graph for the batch step."""
+        """0: per-replica graph replay on each replica's stream; 1: one graph for the batch step
+        with a branch per replica; 2: merged stages, one batched dispatch per stage and kernel
+        instance covering every replica."""
         self._check(lib().fhd_batch_set_mode(self.h, mode))
 
     def step(self, n=1):

@@ -96,12 +96,12 @@ FHD_DEV void lds_fft(float2* buf, const LineMap& lm, int n, uint32_t plan, int n
 // ---------------------------------------------------------------------------------------
 // Pass A: real plane -> half spectrum along x -> FFT along y.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_fft_xy_fwd(FftArgs a, const float* __restrict__ in,
-                                                         float2* __restrict__ spec) {
+FHD_BODY void body_fft_xy_fwd(const FftArgs& a, const float* __restrict__ in, float2* __restrict__ spec,
+                              const Blk& b) {
   __shared__ float2 buf[kLdsElems];
   __shared__ float2 twx[kMaxFftLen];
   __shared__ float2 twy[kMaxFftLen];
-  const int z = blockIdx.x, c = blockIdx.y;
+  const int z = b.x, c = b.y;
   const int mx = a.g.mx, my = a.g.my, mz = a.g.mz, P = a.g.P, h = mx / 2;
   for (int i = threadIdx.x; i < mx; i += blockDim.x) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < my; i += blockDim.x) twy[i] = a.twy[i];
@@ -143,17 +143,18 @@ __global__ __launch_bounds__(kThreads) void k_fft_xy_fwd(FftArgs a, const float*
     spec[(((size_t)c * my + ky) * mz + z) * P + kx] = buf[e];
   }
 }
+FHD_KERNEL_PAIR((), kThreads, k_fft_xy_fwd, FftPass, body_fft_xy_fwd(a.f, a.in, a.spec, b))
 
 // ---------------------------------------------------------------------------------------
 // Pass B: FFT along z, Fourier-space operator, inverse FFT along z, for all three components.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_fft_z_solve(FftArgs a, float2* __restrict__ spec) {
+FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, const Blk& b) {
   __shared__ float2 buf[kLdsElems];
   __shared__ float2 twz[kMaxFftLen];
   __shared__ float2 ezs[kMaxFftLen];
   __shared__ float gzs[kMaxFftLen];
   const int T = a.ztile;
-  const int kx0 = blockIdx.x * T, ky = blockIdx.y;
+  const int kx0 = b.x * T, ky = b.y;
   const int my = a.g.my, mz = a.g.mz, P = a.g.P;
   for (int i = threadIdx.x; i < mz; i += blockDim.x) {
     twz[i] = a.twz[i];
@@ -223,17 +224,17 @@ __global__ __launch_bounds__(kThreads) void k_fft_z_solve(FftArgs a, float2* __r
     if (kx < P) spec[(((size_t)c * my + ky) * mz + z) * P + kx] = buf[e];
   }
 }
+FHD_KERNEL_PAIR((), kThreads, k_fft_z_solve, FftPass, body_fft_z_solve(a.f, a.spec, b))
 
 // ---------------------------------------------------------------------------------------
 // Pass C: inverse FFT along y, C2R along x; out = alpha * result + beta * aux.
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_fft_xy_inv(FftArgs a, const float2* __restrict__ spec,
-                                                         float* __restrict__ out, const float* __restrict__ aux,
-                                                         float alpha, float beta) {
+FHD_BODY void body_fft_xy_inv(const FftArgs& a, const float2* __restrict__ spec, float* __restrict__ out,
+                              const float* __restrict__ aux, float alpha, float beta, const Blk& b) {
   __shared__ float2 buf[kLdsElems];
   __shared__ float2 twx[kMaxFftLen];
   __shared__ float2 twy[kMaxFftLen];
-  const int z = blockIdx.x, c = blockIdx.y;
+  const int z = b.x, c = b.y;
   const int mx = a.g.mx, my = a.g.my, mz = a.g.mz, P = a.g.P, h = mx / 2;
   for (int i = threadIdx.x; i < mx; i += blockDim.x) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < my; i += blockDim.x) twy[i] = a.twy[i];
@@ -273,19 +274,23 @@ __global__ __launch_bounds__(kThreads) void k_fft_xy_inv(FftArgs a, const float2
     out[off + e] = val;
   }
 }
+FHD_KERNEL_PAIR((), kThreads, k_fft_xy_inv, FftPass, body_fft_xy_inv(a.f, a.spec, a.out, a.aux, a.alpha, a.beta, b))
 
 void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s) {
   if (!a.generic && fast_fft_xy_fwd(a, in, spec, s)) return;
-  hipLaunchKernelGGL(k_fft_xy_fwd, dim3(a.g.mz, 3), dim3(kThreads), 0, s, a, in, spec);
+  const FftPass x{a, in, spec, nullptr, nullptr, 0.0f, 0.0f, 0};
+  launch(k_fft_xy_fwd, k_fft_xy_fwd_b, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), kThreads, x, s);
 }
 void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s) {
   if (!a.generic && fast_fft_z_solve(a, spec, s)) return;
-  hipLaunchKernelGGL(k_fft_z_solve, dim3((a.g.P + a.ztile - 1) / a.ztile, a.g.my), dim3(kThreads), 0, s, a, spec);
+  const FftPass x{a, nullptr, spec, nullptr, nullptr, 0.0f, 0.0f, 0};
+  launch(k_fft_z_solve, k_fft_z_solve_b, FAM_FFT_Z, dim3((a.g.P + a.ztile - 1) / a.ztile, a.g.my), kThreads, x, s);
 }
 void launch_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha, float beta,
                        hipStream_t s) {
   if (!a.generic && fast_fft_xy_inv(a, spec, out, aux, alpha, beta, s)) return;
-  hipLaunchKernelGGL(k_fft_xy_inv, dim3(a.g.mz, 3), dim3(kThreads), 0, s, a, spec, out, aux, alpha, beta);
+  const FftPass x{a, nullptr, const_cast<float2*>(spec), out, aux, alpha, beta, 0};
+  launch(k_fft_xy_inv, k_fft_xy_inv_b, FAM_FFT_XY_INV, dim3(a.g.mz, 3), kThreads, x, s);
 }
 
 }  // namespace fhd

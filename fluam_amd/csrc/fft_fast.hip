@@ -71,12 +71,12 @@ FHD_DEV void ffft(float2* buf, int lines, int LS, const float2* tw, float s) {
 
 // ---------------- pass A: real plane -> half spectrum (x) -> FFT along y ----------------
 template <int MX, int MY, int NT>
-__global__ __launch_bounds__(NT) void k_fftA(FftArgs a, const float* __restrict__ in, float2* __restrict__ spec) {
+FHD_BODY void body_fftA(const FftArgs& a, const float* __restrict__ in, float2* __restrict__ spec, const Blk& b) {
   constexpr int H = MX / 2, P = H + 1;
   __shared__ float2 buf[MY * P];
   __shared__ float2 twx[MX];
   __shared__ float2 twy[MY];
-  const int z = blockIdx.x, c = blockIdx.y, mz = a.g.mz;
+  const int z = b.x, c = b.y, mz = a.g.mz;
   for (int i = threadIdx.x; i < MX; i += NT) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
   const float4* src = reinterpret_cast<const float4*>(in + ((size_t)c * mz + z) * (size_t)(MX * MY));
@@ -115,16 +115,17 @@ __global__ __launch_bounds__(NT) void k_fftA(FftArgs a, const float* __restrict_
     dst[(size_t)ky * mz * P + kx] = buf[e];
   }
 }
+FHD_KERNEL_PAIR((template <int MX, int MY, int NT>), NT, k_fftA, FftPass, body_fftA<MX, MY, NT>(a.f, a.in, a.spec, b))
 
 // ---------------- pass C: FFT along y (inverse) -> C2R along x -> out = alpha r + beta aux ----
 template <int MX, int MY, int NT>
-__global__ __launch_bounds__(NT) void k_fftC(FftArgs a, const float2* __restrict__ spec, float* __restrict__ out,
-                                             const float* __restrict__ aux, float alpha, float beta) {
+FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float* __restrict__ out,
+                        const float* __restrict__ aux, float alpha, float beta, const Blk& b) {
   constexpr int H = MX / 2, P = H + 1;
   __shared__ float2 buf[MY * P];
   __shared__ float2 twx[MX];
   __shared__ float2 twy[MY];
-  const int z = blockIdx.x, c = blockIdx.y, mz = a.g.mz;
+  const int z = b.x, c = b.y, mz = a.g.mz;
   for (int i = threadIdx.x; i < MX; i += NT) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
   const float2* src = spec + (size_t)c * MY * mz * P + (size_t)z * P;
@@ -169,19 +170,21 @@ __global__ __launch_bounds__(NT) void k_fftC(FftArgs a, const float2* __restrict
     dst[e] = r;
   }
 }
+FHD_KERNEL_PAIR((template <int MX, int MY, int NT>), NT, k_fftC, FftPass,
+                body_fftC<MX, MY, NT>(a.f, a.spec, a.out, a.aux, a.alpha, a.beta, b))
 
 // ---------------- pass B: FFT along z, operator, inverse FFT along z ----------------
 // Workgroup (tile i, ky): kx columns i*T .. i*T+T-1, plus column ntile*T + i when P is not a
 // multiple of T (the Nyquist column for P = mx/2 + 1).  LDS lines: (c, t) at (c*(T+1)+t)*(MZ+1).
 template <int MZ, int NT, int TM>
-__global__ __launch_bounds__(NT) void k_fftB(FftArgs a, float2* __restrict__ spec, int T) {
+FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, const Blk& b) {
   // TM: room for T <= TM - 1 columns plus one extra column
   constexpr int LS = MZ + 1;
   __shared__ float2 buf[3 * TM * LS];
   __shared__ float2 twz[MZ], ezs[MZ];
   __shared__ float gzs[MZ];
   const int P = a.g.P, my = a.g.my;
-  const int ky = blockIdx.y, tile = blockIdx.x;
+  const int ky = b.y, tile = b.x;
   const int ntile = P / T, rem = P - ntile * T;
   const int TC = T + (tile < rem ? 1 : 0);
   for (int i = threadIdx.x; i < MZ; i += NT) {
@@ -243,14 +246,16 @@ __global__ __launch_bounds__(NT) void k_fftB(FftArgs a, float2* __restrict__ spe
     base[c * cstride + (size_t)z * P + kx] = buf[(c * TM + t) * LS + z];
   }
 }
+FHD_KERNEL_PAIR((template <int MZ, int NT, int TM>), NT, k_fftB, FftPass, body_fftB<MZ, NT, TM>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------
 template <int M>
 static bool launch_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
                       float alpha, float beta, hipStream_t s) {
   constexpr int NT = M >= 64 ? 512 : 256;
-  if (which == 0) hipLaunchKernelGGL((k_fftA<M, M, NT>), dim3(a.g.mz, 3), dim3(NT), 0, s, a, in, spec);
-  else hipLaunchKernelGGL((k_fftC<M, M, NT>), dim3(a.g.mz, 3), dim3(NT), 0, s, a, spec, out, aux, alpha, beta);
+  const FftPass x{a, in, spec, out, aux, alpha, beta, 0};
+  if (which == 0) launch(k_fftA<M, M, NT>, k_fftA_b<M, M, NT>, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), NT, x, s);
+  else launch(k_fftC<M, M, NT>, k_fftC_b<M, M, NT>, FAM_FFT_XY_INV, dim3(a.g.mz, 3), NT, x, s);
   return true;
 }
 
@@ -292,9 +297,10 @@ static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
   const int bt = a.btile == 4 || a.btile == 16 ? a.btile : 8;
   const int T = column_tile(P, bt);
   const dim3 grid(P / T, a.g.my);
-  if (bt == 4) hipLaunchKernelGGL((k_fftB<MZ, 256, 5>), grid, dim3(256), 0, s, a, spec, T);
-  else if (bt == 8) hipLaunchKernelGGL((k_fftB<MZ, 256, 9>), grid, dim3(256), 0, s, a, spec, T);
-  else hipLaunchKernelGGL((k_fftB<MZ, 256, 17>), grid, dim3(256), 0, s, a, spec, T);
+  const FftPass x{a, nullptr, spec, nullptr, nullptr, 0.0f, 0.0f, T};
+  if (bt == 4) launch(k_fftB<MZ, 256, 5>, k_fftB_b<MZ, 256, 5>, FAM_FFT_Z, grid, 256, x, s);
+  else if (bt == 8) launch(k_fftB<MZ, 256, 9>, k_fftB_b<MZ, 256, 9>, FAM_FFT_Z, grid, 256, x, s);
+  else launch(k_fftB<MZ, 256, 17>, k_fftB_b<MZ, 256, 17>, FAM_FFT_Z, grid, 256, x, s);
   return true;
 }
 

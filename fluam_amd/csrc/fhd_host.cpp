@@ -28,6 +28,10 @@
 
 using namespace fhd;
 
+namespace fhd {
+thread_local Recorder* t_recorder = nullptr;
+}
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -384,10 +388,20 @@ void apply_schedule(fhd_ctx* c) {
 // ---------------- particles ----------------
 // Counting sort of position set k by tile, then each tile ordered by particle index.
 void sort_set(fhd_ctx* c, int k) {
-  launch_bin(c->g, c->t, c->cell[k], c->count, c->slot, c->np, c->s);
-  launch_scan_scatter(c->g, c->t, c->cell[k], c->slot, c->count, c->start[k], c->tmp, c->np, c->s);
-  launch_tile_sort(c->g, c->t, c->start[k], c->tmp, c->cell[k], c->off[k], c->sid[k], c->scell[k], c->soff[k],
-                   c->count, c->s);
+  SortArgs a;
+  a.g = c->g;
+  a.t = c->t;
+  a.cell = c->cell[k];
+  a.off = c->off[k];
+  a.count = c->count;
+  a.slot = c->slot;
+  a.start = c->start[k];
+  a.tmp = c->tmp;
+  a.sid = c->sid[k];
+  a.scell = c->scell[k];
+  a.soff = c->soff[k];
+  a.np = c->np;
+  launch_sort(a, c->s);
   mark(c, "sort");
 }
 
@@ -1401,12 +1415,20 @@ struct fhd_batch {
   std::string err;
   // whole-batch graphs: one graph per phase holding every replica's step as an independent
   // branch (fork from / join to the timer stream), replayed with a single launch per step
-  int mode = 0;  // 0: each replica replays its own graphs on its stream; 1: one batch graph
+  // 0: each replica replays its own graphs on its stream; 1: one batch graph of per-replica
+  // branches; 2: merged stages, one dispatch per stage and kernel instance for all replicas
+  int mode = 0;
   bool built = false;
   int period = 1, phase = 0;
   hipGraphExec_t exec[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
+  // merged mode: per phase a graph of batched dispatches reading the tables in `tab`
+  bool merged_built = false;
+  hipGraphExec_t mexec[2] = {nullptr, nullptr};
+  void* tab[2] = {nullptr, nullptr};
+  int ndispatch[2] = {0, 0};
+  bool timer_ahead = false;  // merged work on the timer stream not yet ordered before replica streams
 };
 
 namespace {
@@ -1486,8 +1508,13 @@ fhd_batch* fhd_batch_create(const fhd_params* p, int n) {
 
 void fhd_batch_destroy(fhd_batch* b) {
   if (!b) return;
+  if (b->timer) (void)hipStreamSynchronize(b->timer);
   for (auto& e : b->exec)
     if (e) (void)hipGraphExecDestroy(e);
+  for (auto& e : b->mexec)
+    if (e) (void)hipGraphExecDestroy(e);
+  for (auto& t : b->tab)
+    if (t) (void)hipFree(t);
   for (auto& e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1506,7 +1533,11 @@ const char* fhd_batch_error(const fhd_batch* b) { return b ? b->err.c_str() : g_
 
 int fhd_batch_set_mode(fhd_batch* b, int mode) {
   if (!b) return 0;
-  b->mode = mode ? 1 : 0;
+  if (mode < 0 || mode > 2) {
+    b->err = "batch mode must be 0, 1 or 2";
+    return 0;
+  }
+  b->mode = mode;
   return 1;
 }
 
@@ -1556,23 +1587,209 @@ void build_batch_graphs(fhd_batch* b) {
   b->built = true;
 }
 
+// ---- merged stages (mode 2) ----
+// Each replica's steady-state step is recorded (not launched): the list of its launches with
+// their argument structs.  The lists are merged round by round: a round takes the stage family
+// at the head of most lists and issues one batched dispatch per kernel instance covering every
+// replica whose next launch is of that family.  Launches of one replica keep their order, and
+// the dispatches run in order on one stream, so every replica sees its own step unchanged.
+struct MergedDispatch {
+  const void* fn;
+  int threads;
+  unsigned nblocks;
+  size_t blk, dims, args;  // byte offsets into the phase's device table
+  int stride;
+};
+
+std::vector<LaunchRecord> record_step(fhd_ctx* c) {
+  Recorder rec;
+  t_recorder = &rec;
+  c->capturing = true;
+  const int btile = c->fa.btile;
+  // one column tile for every replica, so replicas of equal z-length share one kernel instance
+  // (the column tile changes the work split only, never a bit of the result)
+  c->fa.btile = 8;
+  try {
+    scheme_step(c);
+    launch_advance_step(c->step_dev, c->s);
+  } catch (...) {
+    t_recorder = nullptr;
+    c->capturing = false;
+    c->fa.btile = btile;
+    throw;
+  }
+  t_recorder = nullptr;
+  c->capturing = false;
+  c->fa.btile = btile;
+  return std::move(rec.list);
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Merge the replicas' launch lists into dispatches and their device tables (host image).
+void merge_lists(const std::vector<std::vector<LaunchRecord>>& lists, std::vector<MergedDispatch>& out,
+                 std::vector<unsigned char>& image) {
+  const int R = (int)lists.size();
+  std::vector<size_t> pos(R, 0);
+  auto put = [&](const void* src, size_t n) {
+    const size_t at = align_up(image.size(), 256);
+    image.resize(at + n);
+    memcpy(image.data() + at, src, n);
+    return at;
+  };
+  for (;;) {
+    // family with the most lists at it; ties go to the family whose list has most left
+    std::vector<std::pair<int, std::pair<int, size_t>>> fam;  // family, (count, longest rest)
+    for (int r = 0; r < R; ++r) {
+      if (pos[r] >= lists[r].size()) continue;
+      const int f = lists[r][pos[r]].family;
+      const size_t rest = lists[r].size() - pos[r];
+      bool found = false;
+      for (auto& e : fam)
+        if (e.first == f) {
+          e.second.first++;
+          e.second.second = std::max(e.second.second, rest);
+          found = true;
+        }
+      if (!found) fam.push_back({f, {1, rest}});
+    }
+    if (fam.empty()) break;
+    int best = 0;
+    for (int k = 1; k < (int)fam.size(); ++k)
+      if (fam[k].second > fam[best].second) best = k;
+    const int f = fam[best].first;
+    // group the heads of that family by kernel instance, in replica order
+    std::vector<std::pair<const void*, std::vector<int>>> groups;
+    for (int r = 0; r < R; ++r) {
+      if (pos[r] >= lists[r].size() || lists[r][pos[r]].family != f) continue;
+      const void* fn = lists[r][pos[r]].fn;
+      bool found = false;
+      for (auto& g : groups)
+        if (g.first == fn) {
+          g.second.push_back(r);
+          found = true;
+        }
+      if (!found) groups.push_back({fn, {r}});
+    }
+    for (auto& g : groups) {
+      const LaunchRecord& first = lists[g.second[0]][pos[g.second[0]]];
+      MergedDispatch d;
+      d.fn = g.first;
+      d.threads = first.threads;
+      d.stride = (int)align_up(first.args.size(), 16);
+      std::vector<int4> blk, dims;
+      std::vector<unsigned char> args(g.second.size() * (size_t)d.stride, 0);
+      for (size_t e = 0; e < g.second.size(); ++e) {
+        const LaunchRecord& L = lists[g.second[e]][pos[g.second[e]]];
+        if (L.threads != d.threads || L.args.size() != first.args.size())
+          throw std::runtime_error("merged batch: launches of one kernel instance disagree in shape");
+        dims.push_back(make_int4((int)L.grid.x, (int)L.grid.y, (int)L.grid.z, 0));
+        for (unsigned z = 0; z < L.grid.z; ++z)
+          for (unsigned y = 0; y < L.grid.y; ++y)
+            for (unsigned x = 0; x < L.grid.x; ++x) blk.push_back(make_int4((int)e, (int)x, (int)y, (int)z));
+        memcpy(args.data() + e * d.stride, L.args.data(), L.args.size());
+      }
+      if (blk.size() > 0x7fffffffu) throw std::runtime_error("merged batch: dispatch too large");
+      d.nblocks = (unsigned)blk.size();
+      d.blk = put(blk.data(), blk.size() * sizeof(int4));
+      d.dims = put(dims.data(), dims.size() * sizeof(int4));
+      d.args = put(args.data(), args.size());
+      out.push_back(d);
+      for (int r : g.second) pos[r]++;
+    }
+  }
+}
+
+// Order the timer stream after work queued on the replica streams, or the reverse.
+void join_replicas_to_timer(fhd_batch* b) {
+  if (b->join.empty()) {
+    b->join.resize(b->reps.size(), nullptr);
+    for (auto& e : b->join) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  for (size_t i = 0; i < b->reps.size(); ++i) {
+    HIPCHK(hipEventRecord(b->join[i], b->reps[i]->s));
+    HIPCHK(hipStreamWaitEvent(b->timer, b->join[i], 0));
+  }
+}
+void fork_timer_to_replicas(fhd_batch* b) {
+  if (!b->timer_ahead) return;
+  if (!b->fork) HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(b->fork, b->timer));
+  for (fhd_ctx* c : b->reps) HIPCHK(hipStreamWaitEvent(c->s, b->fork, 0));
+  b->timer_ahead = false;
+}
+
+void build_merged(fhd_batch* b) {
+  b->period = 1;
+  for (fhd_ctx* c : b->reps)
+    if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) b->period = 2;
+  for (fhd_ctx* c : b->reps) {
+    if (!c->step_dev) c->step_dev = dalloc<unsigned long long>(1);
+    HIPCHK(hipStreamSynchronize(c->s));
+    const unsigned long long v = (unsigned long long)c->step;
+    HIPCHK(hipMemcpy(c->step_dev, &v, sizeof(v), hipMemcpyHostToDevice));
+    c->dev_step_valid = true;
+  }
+  for (int ph = 0; ph < b->period; ++ph) {
+    std::vector<std::vector<LaunchRecord>> lists;
+    for (fhd_ctx* c : b->reps) lists.push_back(record_step(c));
+    std::vector<MergedDispatch> ds;
+    std::vector<unsigned char> image;
+    merge_lists(lists, ds, image);
+    HIPCHK(hipMalloc(&b->tab[ph], std::max<size_t>(image.size(), 256)));
+    HIPCHK(hipMemcpy(b->tab[ph], image.data(), image.size(), hipMemcpyHostToDevice));
+    const unsigned char* base = static_cast<const unsigned char*>(b->tab[ph]);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->timer, hipStreamCaptureModeThreadLocal));
+    try {
+      for (const MergedDispatch& d : ds) {
+        const int4* blk = reinterpret_cast<const int4*>(base + d.blk);
+        const int4* dims = reinterpret_cast<const int4*>(base + d.dims);
+        const unsigned char* args = base + d.args;
+        int stride = d.stride;
+        void* kargs[4] = {&blk, &dims, &args, &stride};
+        HIPCHK(hipLaunchKernel(d.fn, dim3(d.nblocks), dim3(d.threads), kargs, 0, b->timer));
+      }
+    } catch (...) {
+      (void)hipStreamEndCapture(b->timer, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(b->timer, &g));
+    HIPCHK(hipGraphInstantiate(&b->mexec[ph], g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    b->ndispatch[ph] = (int)ds.size();
+  }
+  b->phase = 0;
+  b->merged_built = true;
+}
+
 void batch_advance(fhd_batch* b) {
   bool all_steady = true;
   for (fhd_ctx* c : b->reps) all_steady = all_steady && steady(c);
   if (b->mode == 0 || !all_steady) {
+    fork_timer_to_replicas(b);
     for (fhd_ctx* c : b->reps) advance(c);
-    if (b->built) b->phase = (b->phase + 1) % b->period;  // host state moved one step
+    if (b->built || b->merged_built) b->phase = (b->phase + 1) % b->period;  // host state moved one step
     return;
   }
-  if (!b->built) build_batch_graphs(b);
+  if (b->mode == 2 ? !b->merged_built : !b->built) {
+    if (b->built || b->merged_built) throw std::runtime_error("batch mode changed after its step was captured");
+    if (b->mode == 2) build_merged(b);
+    else build_batch_graphs(b);
+  }
+  bool joined = false;
   for (fhd_ctx* c : b->reps)
     if (!c->dev_step_valid) {  // an eager step ran since the counter was last on the device
       HIPCHK(hipStreamSynchronize(c->s));
       const unsigned long long v = (unsigned long long)c->step;
       HIPCHK(hipMemcpy(c->step_dev, &v, sizeof(v), hipMemcpyHostToDevice));
       c->dev_step_valid = true;
+      joined = true;
     }
-  HIPCHK(hipGraphLaunch(b->exec[b->phase], b->timer));
+  if (b->mode == 2 && !b->timer_ahead && !joined) join_replicas_to_timer(b);
+  if (b->mode == 2) b->timer_ahead = true;
+  HIPCHK(hipGraphLaunch(b->mode == 2 ? b->mexec[b->phase] : b->exec[b->phase], b->timer));
   for (fhd_ctx* c : b->reps) {
     if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) qnb_after_step(c);
     c->step++;
@@ -1590,6 +1807,7 @@ int fhd_batch_step(fhd_batch* b, int nsteps) {
   if (!b) return 0;
   try {
     for (int k = 0; k < nsteps; ++k) batch_advance(b);
+    fork_timer_to_replicas(b);
     HIPCHK(hipGetLastError());
   } catch (const std::exception& e) {
     b->err = e.what();
@@ -1627,6 +1845,7 @@ int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms) {
       HIPCHK(hipStreamWaitEvent(b->timer, ends[i], 0));
     }
     HIPCHK(hipEventRecord(e, b->timer));
+    fork_timer_to_replicas(b);
     HIPCHK(hipEventSynchronize(e));
     float t = 0.0f;
     HIPCHK(hipEventElapsedTime(&t, a, e));

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "fhd_device.h"
+#include "fhd_launch.h"
 
 namespace fhd {
 
@@ -44,11 +45,47 @@ struct FluidWArgs {
   int zchunk;
 };
 
+struct FillArgs {
+  float* p;
+  size_t n;
+};
+
+struct StepArgs {
+  unsigned long long* step;
+};
+
+// One pass of the 3D FFT: the transform parameters plus the pass's buffers.
+struct FftPass {
+  FftArgs f;
+  const float* in;      // forward xy pass: real field
+  float2* spec;         // half spectrum (in place for the z pass)
+  float* out;           // inverse xy pass: out = alpha * result + beta * aux
+  const float* aux;
+  float alpha, beta;
+  int T;                // z pass: kx columns per workgroup
+};
+
 // Tile decomposition used for particle binning and deterministic spreading.
 struct Tiles {
   int tx, ty, tz;       // tile edge in cells
   int ntx, nty, ntz;    // tiles per axis
   int n;                // total tiles
+};
+
+// Counting sort of one position set by tile (bin, scan + scatter, per-tile ordering).
+struct SortArgs {
+  GridDev g;
+  Tiles t;
+  const int4* cell;     // unwrapped cells (by id)
+  const float4* off;
+  int* count;           // per-tile counts (zero on entry, cleared again by the ordering pass)
+  int* slot;            // rank of each particle within its tile, in atomic order
+  int* start;           // tile_start[ntiles + 1]
+  int* tmp;             // ids grouped by tile
+  int* sid;             // sorted outputs
+  int4* scell;
+  float4* soff;
+  int np;
 };
 
 struct SortedSet {
@@ -91,6 +128,7 @@ struct SpreadArgs {
   Key key;
   float out_coef;       // multiplies the accumulated field (1/dV)
   float* out;           // 3N field, every node written
+  int nrec0, nrec_tot;  // records of group 0 and of both groups (set by launch_src_prep)
 };
 
 struct InterpArgs {
@@ -138,19 +176,13 @@ void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipS
 void launch_zero_f(float* p, size_t n, hipStream_t s);
 void launch_advance_step(unsigned long long* step, hipStream_t s);
 
-void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, int* slot, int np,
-                hipStream_t s);
-void launch_scan(const Tiles& t, int* count, int* start, hipStream_t s);
-void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, const int* start,
-                    int* tmp, int np, hipStream_t s);
-void launch_scan_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, int* count, int* start,
-                         int* tmp, int np, hipStream_t s);
-void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
-                      const float4* off, int* sid, int4* scell, float4* soff, int* count, hipStream_t s);
+void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per-tile ordering
 void launch_wca(const WcaArgs& a, hipStream_t s);
 void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
 void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation
 void launch_interp(const InterpArgs& a, hipStream_t s);
-void launch_zero_f4(float4* p, int n, hipStream_t s);
+inline void launch_zero_f4(float4* p, int n, hipStream_t s) {
+  if (n > 0) launch_zero_f(reinterpret_cast<float*>(p), 4 * (size_t)n, s);
+}
 
 }  // namespace fhd

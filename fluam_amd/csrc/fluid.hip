@@ -62,12 +62,12 @@ FHD_DEV void gen_plane(const FluidWArgs& a, const Key& key, float* slot, int x0,
   }
 }
 
-__global__ __launch_bounds__(256) void k_fluid_w(FluidWArgs a) {
+FHD_BODY void body_fluid_w(const FluidWArgs& a, const Blk& b) {
   __shared__ float nbuf[3][6 * PLANE];
   const GridDev& g = a.g;
   const Key key = resolve_key(a.key);
-  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-  const int zs = blockIdx.z * a.zchunk;
+  const int x0 = b.x * TX, y0 = b.y * TY;
+  const int zs = b.z * a.zchunk;
   const int ze = min(zs + a.zchunk, g.mz);
   const int lx = threadIdx.x % TX, ly = threadIdx.x / TX;
   const int x = x0 + lx, y = y0 + ly;
@@ -167,6 +167,7 @@ __global__ __launch_bounds__(256) void k_fluid_w(FluidWArgs a) {
     __syncthreads();
   }
 }
+FHD_KERNEL_PAIR((), 256, k_fluid_w, FluidWArgs, body_fluid_w(a, b))
 
 // Discrete staggered divergence (diagnostic), src/kernelUpdateVIncompressible.cu invariant.
 __global__ void k_divergence(GridDev g, const float* __restrict__ v, float* __restrict__ div) {
@@ -190,30 +191,34 @@ __global__ void k_normals(Key key, int stream, int sub, int n, float* out) {
   for (int k = 0; k < 4; ++k) out[4 * i + k] = v[k];
 }
 
-__global__ void k_zero_f(float* p, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0f;
+FHD_BODY void body_zero_f(const FillArgs& a, const Blk& b) {
+  const size_t i = (size_t)b.x * blockDim.x + threadIdx.x;
+  if (i < a.n) a.p[i] = 0.0f;
 }
+FHD_KERNEL_PAIR((), 256, k_zero_f, FillArgs, body_zero_f(a, b))
 
 void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipStream_t s) {
   hipLaunchKernelGGL(k_normals, dim3((n + 255) / 256), dim3(256), 0, s, key, stream, sub, n, out);
 }
 
 void launch_zero_f(float* p, size_t n, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  const FillArgs a{p, n};
+  launch(k_zero_f, k_zero_f_b, FAM_ZERO, dim3((unsigned)((n + 255) / 256)), 256, a, s);
 }
 
 // Advances the device-side step counter that graph-replayed steps key their noise with.
-__global__ void k_advance_step(unsigned long long* step) {
-  if (threadIdx.x == 0) *step += 1ull;
+FHD_BODY void body_advance_step(const StepArgs& a) {
+  if (threadIdx.x == 0) *a.step += 1ull;
 }
+FHD_KERNEL_PAIR((), 64, k_advance_step, StepArgs, body_advance_step(a))
 void launch_advance_step(unsigned long long* step, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, s, step);
+  const StepArgs a{step};
+  launch(k_advance_step, k_advance_step_b, FAM_STEP, dim3(1), 64, a, s);
 }
 
 void launch_fluid_w(const FluidWArgs& a, hipStream_t s) {
   dim3 grid((a.g.mx + TX - 1) / TX, (a.g.my + TY - 1) / TY, (a.g.mz + a.zchunk - 1) / a.zchunk);
-  hipLaunchKernelGGL(k_fluid_w, grid, dim3(256), 0, s, a);
+  launch(k_fluid_w, k_fluid_w_b, FAM_FLUID_W, grid, 256, a, s);
 }
 
 void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t s) {

@@ -28,15 +28,19 @@ FHD_DEV int tile_of(const GridDev& g, const Tiles& t, int4 c) {
   return x + t.ntx * (y + t.nty * z);
 }
 
-__global__ void k_bin(GridDev g, Tiles t, const int4* __restrict__ cell, int* count, int* slot, int np) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  int tile = tile_of(g, t, cell[i]);
-  slot[i] = atomicAdd(&count[tile], 1);
+FHD_BODY void body_bin(const SortArgs& a, const Blk& b) {
+  int i = b.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  int tile = tile_of(a.g, a.t, a.cell[i]);
+  a.slot[i] = atomicAdd(&a.count[tile], 1);
 }
+FHD_KERNEL_PAIR((), 256, k_bin, SortArgs, body_bin(a, b))
 
 // Exclusive scan of the tile counts (single workgroup); resets the counts for the next binning.
-__global__ __launch_bounds__(1024) void k_scan(int n, int* count, int* start) {
+FHD_BODY void body_scan(const SortArgs& a) {
+  const int n = a.t.n;
+  int* __restrict__ count = a.count;
+  int* __restrict__ start = a.start;
   __shared__ int part[1024];
   const int T = blockDim.x;
   const int chunk = (n + T - 1) / T;
@@ -60,22 +64,29 @@ __global__ __launch_bounds__(1024) void k_scan(int n, int* count, int* start) {
   }
   if (threadIdx.x == T - 1) start[n] = part[T - 1];
 }
+FHD_KERNEL_PAIR((), 1024, k_scan, SortArgs, body_scan(a))
 
-__global__ void k_scatter(GridDev g, Tiles t, const int4* __restrict__ cell, const int* __restrict__ slot,
-                          const int* __restrict__ start, int* __restrict__ tmp, int np) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  tmp[start[tile_of(g, t, cell[i])] + slot[i]] = i;
+FHD_BODY void body_scatter(const SortArgs& a, const Blk& b) {
+  int i = b.x * blockDim.x + threadIdx.x;
+  if (i >= a.np) return;
+  a.tmp[a.start[tile_of(a.g, a.t, a.cell[i])] + a.slot[i]] = i;
 }
+FHD_KERNEL_PAIR((), 256, k_scatter, SortArgs, body_scatter(a, b))
 
 // Scan + scatter in one launch for up to kScanTilesMax tiles: every workgroup forms the whole
 // exclusive prefix of the tile counts (a few KB, read from L2) in LDS, workgroup 0 stores it as
 // the tile starts, and each thread places its particle.  The counts are cleared later, by the
 // per-tile ordering pass.
 constexpr int kScanTilesMax = 8192;
-__global__ __launch_bounds__(256) void k_scan_scatter(GridDev g, Tiles t, const int4* __restrict__ cell,
-                                                      const int* __restrict__ slot, const int* __restrict__ count,
-                                                      int* __restrict__ start, int* __restrict__ tmp, int np) {
+FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
+  const GridDev& g = a.g;
+  const Tiles& t = a.t;
+  const int4* __restrict__ cell = a.cell;
+  const int* __restrict__ slot = a.slot;
+  const int* __restrict__ count = a.count;
+  int* __restrict__ start = a.start;
+  int* __restrict__ tmp = a.tmp;
+  const int np = a.np;
   __shared__ int pre[kScanTilesMax + 1];
   __shared__ int wsum[4];
   const int n = t.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -94,22 +105,29 @@ __global__ __launch_bounds__(256) void k_scan_scatter(GridDev g, Tiles t, const 
   }
   if (tid == 255) pre[n] = run;
   __syncthreads();
-  if (blockIdx.x == 0)
+  if (b.x == 0)
     for (int k = tid; k <= n; k += 256) start[k] = pre[k];
-  const int i = blockIdx.x * blockDim.x + tid;
+  const int i = b.x * blockDim.x + tid;
   if (i < np) tmp[pre[tile_of(g, t, cell[i])] + slot[i]] = i;
 }
+FHD_KERNEL_PAIR((), 256, k_scan_scatter, SortArgs, body_scan_scatter(a, b))
 
 // One wavefront per tile: rank each particle by its index within the tile, then gather.  The
 // sorted copies carry the cell wrapped into the box (0 <= c < m), which is all the spatial
 // passes need; the unwrapped cell stays in the per-particle arrays.
-__global__ __launch_bounds__(256) void k_tile_sort(GridDev g, int ntiles, const int* __restrict__ start,
-                                                   const int* __restrict__ tmp, const int4* __restrict__ cell,
-                                                   const float4* __restrict__ off, int* __restrict__ sid,
-                                                   int4* __restrict__ scell, float4* __restrict__ soff,
-                                                   int* __restrict__ count) {
+FHD_BODY void body_tile_sort(const SortArgs& a, const Blk& b) {
+  const GridDev& g = a.g;
+  const int ntiles = a.t.n;
+  const int* __restrict__ start = a.start;
+  const int* __restrict__ tmp = a.tmp;
+  const int4* __restrict__ cell = a.cell;
+  const float4* __restrict__ off = a.off;
+  int* __restrict__ sid = a.sid;
+  int4* __restrict__ scell = a.scell;
+  float4* __restrict__ soff = a.soff;
+  int* __restrict__ count = a.count;
   const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int tile = b.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (tile >= ntiles) return;
   const int s0 = start[tile], n = start[tile + 1] - s0;
   if (lane == 0) count[tile] = 0;  // ready for the next binning
@@ -123,6 +141,7 @@ __global__ __launch_bounds__(256) void k_tile_sort(GridDev g, int ntiles, const 
     soff[s0 + rank] = off[id];
   }
 }
+FHD_KERNEL_PAIR((), 256, k_tile_sort, SortArgs, body_tile_sort(a, b))
 
 // d = difference of two wrapped cell coordinates, |d| < m: nearest periodic image.
 FHD_DEV int min_image(int d, int m) {
@@ -166,7 +185,7 @@ constexpr int kWcaSlices = 16;  // lanes sharing one particle's candidates
 // sorted order (a deterministic compaction); 16 lanes share each of the tile's particles,
 // lane l taking candidates l, l + 16, ..., and the partial forces are combined by a fixed
 // xor-shuffle tree, so every sum is formed in the same order on every run.
-__global__ __launch_bounds__(kWcaThreads) void k_wca(WcaArgs a) {
+FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   __shared__ int4 ccell[kWcaThreads];   // w: sorted slot
   __shared__ float4 coff[kWcaThreads];
   __shared__ int nb_start[32], nb_pref[33];
@@ -174,7 +193,7 @@ __global__ __launch_bounds__(kWcaThreads) void k_wca(WcaArgs a) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int tile = xcd_item(blockIdx.x, t.n);
+  const int tile = xcd_item(b.x, t.n);
   const int s0 = a.set.start[tile], nown = a.set.start[tile + 1] - s0;
   if (nown == 0) return;  // uniform: no barrier is skipped by part of the group
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
@@ -289,6 +308,7 @@ __global__ __launch_bounds__(kWcaThreads) void k_wca(WcaArgs a) {
     if (own && sl == 0) a.F[a.set.id[si]] = make_float4(fx, fy, fz, 0.0f);
   }
 }
+FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
 
 // Weights of one axis at an explicit node base (used for the reference-quirk RFD variant):
 // node k sits at integer position base + k in node units; the particle at c + frac - stag/2.
@@ -350,8 +370,9 @@ FHD_DEV bool reaches(int lo, int o, int T, int m, int ntiles) {
 // One record per source (sorted slot x sub-source): weights, bases, value and the cull data.
 // One launch covers every group: threads [0, n0) build group 0's records, the rest group 1's.
 template <int NPTS>
-__global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
+  const int n0 = a.nrec0, ntot = a.nrec_tot;
+  int r = b.x * blockDim.x + threadIdx.x;
   if (r >= ntot) return;
   const int gi = r < n0 ? 0 : 1;
   if (gi) r -= n0;
@@ -436,6 +457,7 @@ __global__ __launch_bounds__(256) void k_src_prep(SpreadArgs a, int n0, int ntot
   }
   G.cull[r] = make_int4(mask, 0, 0, (int)__float_as_uint(m));
 }
+FHD_KERNEL_PAIR((template <int NPTS>), 256, k_src_prep, SpreadArgs, body_src_prep<NPTS>(a, b))
 
 // Fixed-point image of one contribution (|x| <= 2^30 after scaling), as a 64-bit two's-complement
 // addend.  Integer sums are exact, so the accumulated field does not
@@ -531,7 +553,7 @@ FHD_DEV const SrcRec* rec_of(const SpreadArgs& a, int ref) {
 // candidates).  Integer addition is associative, so the result is bit-identical whatever order
 // the lanes' atomics land in.  FAST: tiles of 8 x 8 x 8 cells and at least two tiles per axis.
 template <int NPTS, bool FAST>
-__global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
+FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   constexpr int H = NPTS - 1;
   constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
   constexpr int NACC = 3 * ZS * kRowStride;
@@ -545,7 +567,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x;
-  const int tile = xcd_item(blockIdx.x, t.n);
+  const int tile = xcd_item(b.x, t.n);
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
   for (int e = tid; e < NACC + 2 * kAccPad; e += kSpreadThreads) acc[e] = 0ull;
@@ -666,6 +688,7 @@ __global__ __launch_bounds__(kSpreadThreads) void k_spread(SpreadArgs a) {
     a.out[gi] = (float)(long long)acc[ai] * inv;
   }
 }
+FHD_KERNEL_PAIR((template <int NPTS, bool FAST>), kSpreadThreads, k_spread, SpreadArgs, body_spread<NPTS, FAST>(a, b))
 
 // Sum over the NPTS x NPTS nodes of one z-row (kz) of a particle's stencil.
 template <int NPTS>
@@ -750,8 +773,8 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
 }
 
 template <int NPTS>
-__global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
-  const int gid = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+FHD_BODY void body_interp(const InterpArgs& a, const Blk& b) {
+  const int gid = xcd_item(b.x, b.nx) * blockDim.x + threadIdx.x;
   const int s = gid >> 2, q = gid & 3;
   const GridDev& g = a.g;
   float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
@@ -776,71 +799,53 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs a) {
   if (s >= a.np || q != 0) return;
   interp_finish(a, s, u1, u2);
 }
+FHD_KERNEL_PAIR((template <int NPTS>), 256, k_interp, InterpArgs, body_interp<NPTS>(a, b))
 
-__global__ void k_zero_f4(float4* p, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-}
 
 // ------------------------------- launchers -------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-void launch_bin(const GridDev& g, const Tiles& t, const int4* cell, int* count, int* slot, int np, hipStream_t s) {
-  if (np > 0) hipLaunchKernelGGL(k_bin, dim3(nblk(np, 256)), dim3(256), 0, s, g, t, cell, count, slot, np);
-}
-void launch_scan(const Tiles& t, int* count, int* start, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, t.n, count, start);
-}
-void launch_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, const int* start, int* tmp,
-                    int np, hipStream_t s) {
-  if (np > 0) hipLaunchKernelGGL(k_scatter, dim3(nblk(np, 256)), dim3(256), 0, s, g, t, cell, slot, start, tmp, np);
-}
-void launch_scan_scatter(const GridDev& g, const Tiles& t, const int4* cell, const int* slot, int* count, int* start,
-                         int* tmp, int np, hipStream_t s) {
-  if (t.n <= kScanTilesMax) {
-    hipLaunchKernelGGL(k_scan_scatter, dim3(nblk(np > 0 ? np : 1, 256)), dim3(256), 0, s, g, t, cell, slot, count,
-                       start, tmp, np);
+void launch_sort(const SortArgs& a, hipStream_t s) {
+  if (a.np > 0) launch(k_bin, k_bin_b, FAM_BIN, dim3(nblk(a.np, 256)), 256, a, s);
+  if (a.t.n <= kScanTilesMax) {
+    launch(k_scan_scatter, k_scan_scatter_b, FAM_SCAN, dim3(nblk(a.np > 0 ? a.np : 1, 256)), 256, a, s);
   } else {
-    launch_scan(t, count, start, s);
-    launch_scatter(g, t, cell, slot, start, tmp, np, s);
+    launch(k_scan, k_scan_b, FAM_SCAN, dim3(1), 1024, a, s);
+    if (a.np > 0) launch(k_scatter, k_scatter_b, FAM_SCATTER, dim3(nblk(a.np, 256)), 256, a, s);
   }
-}
-void launch_tile_sort(const GridDev& g, const Tiles& t, const int* start, const int* tmp, const int4* cell,
-                      const float4* off, int* sid, int4* scell, float4* soff, int* count, hipStream_t s) {
-  hipLaunchKernelGGL(k_tile_sort, dim3(nblk(t.n, 4)), dim3(256), 0, s, g, t.n, start, tmp, cell, off, sid, scell,
-                     soff, count);
+  launch(k_tile_sort, k_tile_sort_b, FAM_TILE_SORT, dim3(nblk(a.t.n, 4)), 256, a, s);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
-  if (a.np > 0) hipLaunchKernelGGL(k_wca, dim3(a.t.n), dim3(kWcaThreads), 0, s, a);
+  if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n), kWcaThreads, a, s);
 }
 void launch_src_prep(const SpreadArgs& a, hipStream_t s) {
   int nrec[2] = {0, 0};
   for (int gi = 0; gi < a.ngroups; ++gi) nrec[gi] = a.grp[gi].nsrc * (a.grp[gi].mode == SRC_RFD ? 2 : 1);
-  const int ntot = nrec[0] + nrec[1];
-  if (ntot <= 0) return;
-  if (a.kernel_points == 4)
-    hipLaunchKernelGGL(k_src_prep<4>, dim3(nblk(ntot, 256)), dim3(256), 0, s, a, nrec[0], ntot);
-  else hipLaunchKernelGGL(k_src_prep<3>, dim3(nblk(ntot, 256)), dim3(256), 0, s, a, nrec[0], ntot);
+  SpreadArgs x = a;
+  x.nrec0 = nrec[0];
+  x.nrec_tot = nrec[0] + nrec[1];
+  if (x.nrec_tot <= 0) return;
+  const dim3 grid(nblk(x.nrec_tot, 256));
+  if (a.kernel_points == 4) launch(k_src_prep<4>, k_src_prep_b<4>, FAM_SRC_PREP, grid, 256, x, s);
+  else launch(k_src_prep<3>, k_src_prep_b<3>, FAM_SRC_PREP, grid, 256, x, s);
 }
 void launch_spread(const SpreadArgs& a, hipStream_t s) {
   const bool fast = a.t.tx == kTileEdge && a.t.ty == kTileEdge && a.t.tz == kTileEdge && a.t.ntx >= 2 &&
                     a.t.nty >= 2 && a.t.ntz >= 2;
-  const dim3 grid(a.t.n), block(kSpreadThreads);
+  const dim3 grid(a.t.n);
   if (a.kernel_points == 4) {
-    if (fast) hipLaunchKernelGGL((k_spread<4, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_spread<4, false>), grid, block, 0, s, a);
+    if (fast) launch(k_spread<4, true>, k_spread_b<4, true>, FAM_SPREAD, grid, kSpreadThreads, a, s);
+    else launch(k_spread<4, false>, k_spread_b<4, false>, FAM_SPREAD, grid, kSpreadThreads, a, s);
   } else {
-    if (fast) hipLaunchKernelGGL((k_spread<3, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((k_spread<3, false>), grid, block, 0, s, a);
+    if (fast) launch(k_spread<3, true>, k_spread_b<3, true>, FAM_SPREAD, grid, kSpreadThreads, a, s);
+    else launch(k_spread<3, false>, k_spread_b<3, false>, FAM_SPREAD, grid, kSpreadThreads, a, s);
   }
 }
 void launch_interp(const InterpArgs& a, hipStream_t s) {
   if (a.np <= 0) return;
-  if (a.kernel_points == 4) hipLaunchKernelGGL(k_interp<4>, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_interp<3>, dim3(nblk(4 * a.np, 256)), dim3(256), 0, s, a);
-}
-void launch_zero_f4(float4* p, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_zero_f4, dim3(nblk(n, 256)), dim3(256), 0, s, p, n);
+  const dim3 grid(nblk(4 * a.np, 256));
+  if (a.kernel_points == 4) launch(k_interp<4>, k_interp_b<4>, FAM_INTERP, grid, 256, a, s);
+  else launch(k_interp<3>, k_interp_b<3>, FAM_INTERP, grid, 256, a, s);
 }
 
 }  // namespace fhd

@@ -140,7 +140,10 @@ int fhd_batch_size(const fhd_batch* b);
 fhd_ctx* fhd_batch_replica(fhd_batch* b, int i);           /* owned by the batch */
 const char* fhd_batch_error(const fhd_batch* b);
 /* mode 0: every replica replays its own graphs on its stream; mode 1: the whole batch step is
- * one captured graph (each replica an independent branch), replayed with one launch. */
+ * one captured graph (each replica an independent branch), replayed with one launch; mode 2:
+ * merged stages, one batched dispatch per stage and kernel instance covering every replica,
+ * captured as one graph.  Set the mode before the first steady-state step; changing it after
+ * the step was captured is an error. */
 int fhd_batch_set_mode(fhd_batch* b, int mode);
 int fhd_batch_step(fhd_batch* b, int nsteps);
 int fhd_batch_sync(fhd_batch* b);

@@ -304,11 +304,12 @@ def _replica_params(i, shape, np_, scheme=SCHEME_STOKES_LIMIT):
     return make_params(scheme, **kw)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_batch_replicas_equal_alone(mode):
     """Replicas of uneven shapes, particle counts and schemes stepped together in one batch
-    (mode 0: each replica's own graphs on its stream; mode 1: one graph for the batch step) give
-    the same bits as each run alone with eager launches."""
+    (mode 0: each replica's own graphs on its stream; mode 1: one graph for the batch step;
+    mode 2: one batched dispatch per stage and kernel instance) give the same bits as each run
+    alone with eager launches."""
     specs = [((16, 24, 32), 100, SCHEME_STOKES_LIMIT), ((32, 32, 32), 0, SCHEME_STOKES_LIMIT),
              ((24, 16, 48), 300, SCHEME_QUASI_NEUTRALLY_BUOYANT), ((32, 32, 32), 50, SCHEME_STOKES_LIMIT),
              ((16, 16, 16), 0, SCHEME_INCOMPRESSIBLE)]
