@@ -1,9 +1,10 @@
 // fft_fast.hip -- size-specialised versions of the three FFT passes (see fft.hip for the
 // algorithm).  Transform lengths, radix plans and LDS extents are template parameters, so every
 // index division in the Stockham stages is by a constant and the register arrays are sized for
-// the actual plane instead of the worst case.  Instantiated for cubic planes
-// m x m, m in {16, 24, 32, 48, 64, 96, 128} and pencils of those lengths; other shapes use the
-// generic kernels, which share the spectrum layout spec[c][ky][z][kx].
+// the actual plane instead of the worst case.  Instantiated for xy planes mx x my with both
+// lengths in {16, 24, 32, 48, 64} or mx = my in {96, 128}, and z pencils of any of those
+// lengths; other shapes use the generic kernels, which share the spectrum layout
+// spec[c][ky][z][kx].
 #include "fhd_device.h"
 #include "fhd_kernels.h"
 #include "fft_common.h"
@@ -161,11 +162,11 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
     const float2 p0 = buf[y * P + 2 * x4], p1 = buf[y * P + 2 * x4 + 1];
     float4 r = make_float4(alpha * p0.x, alpha * p0.y, alpha * p1.x, alpha * p1.y);
     if (ax) {
-      const float4 b = ax[e];
-      r.x += beta * b.x;
-      r.y += beta * b.y;
-      r.z += beta * b.z;
-      r.w += beta * b.w;
+      const float4 q = ax[e];
+      r.x += beta * q.x;
+      r.y += beta * q.y;
+      r.z += beta * q.z;
+      r.w += beta * q.w;
     }
     dst[e] = r;
   }
@@ -249,27 +250,42 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
 FHD_KERNEL_PAIR((template <int MZ, int NT, int TM>), NT, k_fftB, FftPass, body_fftB<MZ, NT, TM>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------
-template <int M>
+template <int MX, int MY>
 static bool launch_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
                       float alpha, float beta, hipStream_t s) {
-  constexpr int NT = M >= 64 ? 512 : 256;
+  constexpr int NT = MX * MY >= 64 * 64 ? 512 : 256;
   const FftPass x{a, in, spec, out, aux, alpha, beta, 0};
-  if (which == 0) launch(k_fftA<M, M, NT>, k_fftA_b<M, M, NT>, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), NT, x, s);
-  else launch(k_fftC<M, M, NT>, k_fftC_b<M, M, NT>, FAM_FFT_XY_INV, dim3(a.g.mz, 3), NT, x, s);
+  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), NT, x, s);
+  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(a.g.mz, 3), NT, x, s);
   return true;
+}
+
+// Rectangular planes mx x my with both lengths in {16, 24, 32, 48, 64}.
+template <int MX>
+static bool launch_AC_rect(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
+                           float alpha, float beta, hipStream_t s) {
+  switch (a.g.my) {
+    case 16: return launch_AC<MX, 16>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 24: return launch_AC<MX, 24>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 32: return launch_AC<MX, 32>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 48: return launch_AC<MX, 48>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 64: return launch_AC<MX, 64>(a, which, in, spec, out, aux, alpha, beta, s);
+    default: return false;
+  }
 }
 
 static bool fast_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
                     float alpha, float beta, hipStream_t s) {
-  if (a.g.mx != a.g.my) return false;
+  if (a.g.mx == a.g.my) {
+    if (a.g.mx == 96) return launch_AC<96, 96>(a, which, in, spec, out, aux, alpha, beta, s);
+    if (a.g.mx == 128) return launch_AC<128, 128>(a, which, in, spec, out, aux, alpha, beta, s);
+  }
   switch (a.g.mx) {
-    case 16: return launch_AC<16>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 24: return launch_AC<24>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 32: return launch_AC<32>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 48: return launch_AC<48>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 64: return launch_AC<64>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 96: return launch_AC<96>(a, which, in, spec, out, aux, alpha, beta, s);
-    case 128: return launch_AC<128>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 16: return launch_AC_rect<16>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 24: return launch_AC_rect<24>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 32: return launch_AC_rect<32>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 48: return launch_AC_rect<48>(a, which, in, spec, out, aux, alpha, beta, s);
+    case 64: return launch_AC_rect<64>(a, which, in, spec, out, aux, alpha, beta, s);
     default: return false;
   }
 }

@@ -112,7 +112,10 @@ FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
 }
 FHD_KERNEL_PAIR((), 256, k_scan_scatter, SortArgs, body_scan_scatter(a, b))
 
-// One wavefront per tile: rank each particle by its index within the tile, then gather.  The
+constexpr int kSortStage = 256;  // per-wave LDS staging of a tile's ids
+
+// One wavefront per tile (four per workgroup): rank each particle by its index within the
+// tile, then gather.  The
 // sorted copies carry the cell wrapped into the box (0 <= c < m), which is all the spatial
 // passes need; the unwrapped cell stays in the per-particle arrays.
 FHD_BODY void body_tile_sort(const SortArgs& a, const Blk& b) {
@@ -126,15 +129,29 @@ FHD_BODY void body_tile_sort(const SortArgs& a, const Blk& b) {
   int4* __restrict__ scell = a.scell;
   float4* __restrict__ soff = a.soff;
   int* __restrict__ count = a.count;
-  const int lane = threadIdx.x & 63;
-  const int tile = b.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (tile >= ntiles) return;
-  const int s0 = start[tile], n = start[tile + 1] - s0;
-  if (lane == 0) count[tile] = 0;  // ready for the next binning
+  __shared__ int ids[4][kSortStage];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tile = b.x * 4 + w;
+  const bool valid = tile < ntiles;
+  int s0 = 0, n = 0;
+  if (valid) {
+    s0 = start[tile];
+    n = start[tile + 1] - s0;
+    if (lane == 0) count[tile] = 0;  // ready for the next binning
+  }
+  // the tile's ids in LDS (the rank loop reads each one n times)
+  const bool staged = n <= kSortStage;
+  if (staged)
+    for (int e = lane; e < n; e += 64) ids[w][e] = tmp[s0 + e];
+  __syncthreads();
+  if (!valid) return;
   for (int e = lane; e < n; e += 64) {
-    int id = tmp[s0 + e];
+    const int id = staged ? ids[w][e] : tmp[s0 + e];
     int rank = 0;
-    for (int f = 0; f < n; ++f) rank += (tmp[s0 + f] < id);
+    if (staged)
+      for (int f = 0; f < n; ++f) rank += (ids[w][f] < id);
+    else
+      for (int f = 0; f < n; ++f) rank += (tmp[s0 + f] < id);
     int4 c = cell[id];
     sid[s0 + rank] = id;
     scell[s0 + rank] = make_int4(wrapi(c.x, g.mx), wrapi(c.y, g.my), wrapi(c.z, g.mz), 0);
@@ -177,17 +194,19 @@ FHD_DEV int wave_incl_scan(int v) {
 
 constexpr int kWcaThreads = 256;
 constexpr int kWcaSlices = 16;  // lanes sharing one particle's candidates
+constexpr int kWcaCap = 512;    // culled candidates staged in LDS at once
 
 // WCA, one workgroup per tile: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for
 // r < rc (fluam's cell-list pair loop, src/stokesLimitFunctions.cu:160-653, with the analytic
 // potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).  The
-// particles of the neighbouring tiles that lie within rc of the tile are staged in LDS in
-// sorted order (a deterministic compaction); 16 lanes share each of the tile's particles,
-// lane l taking candidates l, l + 16, ..., and the partial forces are combined by a fixed
-// xor-shuffle tree, so every sum is formed in the same order on every run.
+// particles of the neighbouring tiles that lie within rc of the tile are compacted into LDS in
+// sorted order (a deterministic ballot compaction), once per tile when they fit kWcaCap
+// entries, else in passes of kWcaCap; 16 lanes share each of the tile's particles, lane l
+// taking candidates l, l + 16, ... of each pass, and the partial forces are combined by a
+// fixed xor-shuffle tree, so every sum is formed in the same order on every run.
 FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
-  __shared__ int4 ccell[kWcaThreads];   // w: sorted slot
-  __shared__ float4 coff[kWcaThreads];
+  __shared__ int4 ccell[kWcaCap];   // w: sorted slot
+  __shared__ float4 coff[kWcaCap];
   __shared__ int nb_start[32], nb_pref[33];
   __shared__ int wsum[kWcaThreads / 64];
   const GridDev& g = a.g;
@@ -222,6 +241,68 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   // conservative cull radius (the pair test below is exact)
   const float rcc = sqrtf(a.rc2) * 1.001f + 1e-6f;
   const float Tx = t.tx * g.dx, Ty = t.ty * g.dy, Tz = t.tz * g.dz;
+  // The distance to the tile is measured from the candidate's image nearest the tile centre,
+  // folded on whole cells, so it can be off by up to one cell near the half-box point.  Along
+  // an axis where that point lies farther than rc + one cell outside the tile, the cull is
+  // exact; along an axis with one tile every candidate is inside.  Otherwise keep everything.
+  auto axis_ok = [&](int nt_a, int m, int T, float d) { return nt_a == 1 || 0.5f * (m - T) * d > rcc + d; };
+  const bool cull = axis_ok(t.ntx, g.mx, t.tx, g.dx) && axis_ok(t.nty, g.my, t.ty, g.dy) &&
+                    axis_ok(t.ntz, g.mz, t.tz, g.dz);
+  // Compacts the near candidates with rank in [p0, p0 + kWcaCap) into LDS; returns how many
+  // near candidates there are in all.
+  auto stage = [&](int p0) {
+    int seen = 0;
+    for (int c0 = 0; c0 < nall; c0 += kWcaThreads) {
+      const int e = c0 + tid;
+      bool near = false;
+      int4 cj = make_int4(0, 0, 0, 0);
+      float4 dj = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (e < nall) {
+        // largest lo with nb_pref[lo] <= e (entries past nnb hold nall)
+        int lo = 0;
+#pragma unroll
+        for (int step = 16; step >= 1; step >>= 1)
+          if (nb_pref[lo + step] <= e) lo += step;
+        const int j = nb_start[lo] + (e - nb_pref[lo]);
+        cj = a.set.cell[j];
+        dj = a.set.off[j];
+        cj.w = j;
+        near = true;
+        if (cull) {
+          // position relative to the tile centre (nearest image), then the distance outside
+          // the tile along each axis (zero along an axis with a single tile)
+          const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
+          const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
+          const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
+          const float ex = t.ntx == 1 ? 0.0f : fmaxf(0.0f, fabsf(px) - 0.5f * Tx);
+          const float ey = t.nty == 1 ? 0.0f : fmaxf(0.0f, fabsf(py) - 0.5f * Ty);
+          const float ez = t.ntz == 1 ? 0.0f : fmaxf(0.0f, fabsf(pz) - 0.5f * Tz);
+          near = ex * ex + ey * ey + ez * ez <= rcc * rcc;
+        }
+      }
+      const unsigned long long bal = __ballot(near);
+      const int before_w = __popcll(bal & ((1ull << lane) - 1ull));
+      __syncthreads();  // wsum of the previous chunk consumed
+      if (lane == 0) wsum[wv] = __popcll(bal);
+      __syncthreads();
+      int base = seen, nk = 0;
+#pragma unroll
+      for (int w = 0; w < kWcaThreads / 64; ++w) {
+        base += w < wv ? wsum[w] : 0;
+        nk += wsum[w];
+      }
+      const int rank = base + before_w - p0;
+      if (near && rank >= 0 && rank < kWcaCap) {
+        ccell[rank] = cj;
+        coff[rank] = dj;
+      }
+      seen += nk;
+    }
+    __syncthreads();
+    return seen;
+  };
+  const int nnear = stage(0);
+  const int npass = (nnear + kWcaCap - 1) / kWcaCap;
   for (int i0 = 0; i0 < nown; i0 += kWcaThreads / kWcaSlices) {
     const int il = tid / kWcaSlices, sl = tid % kWcaSlices;
     const int si = s0 + i0 + il;
@@ -233,49 +314,12 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
       di = a.set.off[si];
     }
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int c0 = 0; c0 < nall; c0 += kWcaThreads) {
-      // stage candidates c0 + tid that lie within rc of the tile, keeping their order
-      const int e = c0 + tid;
-      bool near = false;
-      int4 cj;
-      float4 dj;
-      if (e < nall) {
-        // largest lo with nb_pref[lo] <= e (entries past nnb hold nall)
-        int lo = 0;
-#pragma unroll
-        for (int step = 16; step >= 1; step >>= 1)
-          if (nb_pref[lo + step] <= e) lo += step;
-        const int j = nb_start[lo] + (e - nb_pref[lo]);
-        cj = a.set.cell[j];
-        dj = a.set.off[j];
-        cj.w = j;
-        // position relative to the tile centre (nearest image: the box spans at least three
-        // tiles, so m >= T + 2 rc), then the distance outside the tile along each axis
-        const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
-        const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
-        const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
-        const float ex = fmaxf(0.0f, fabsf(px) - 0.5f * Tx);
-        const float ey = fmaxf(0.0f, fabsf(py) - 0.5f * Ty);
-        const float ez = fmaxf(0.0f, fabsf(pz) - 0.5f * Tz);
-        // small boxes: min_image folds the tile onto itself, keep every candidate
-        near = (t.ntx < 3 || t.nty < 3 || t.ntz < 3) || (ex * ex + ey * ey + ez * ez <= rcc * rcc);
+    for (int p = 0; p < npass; ++p) {
+      if (npass > 1) {
+        __syncthreads();  // the previous pass's candidates consumed
+        stage(p * kWcaCap);
       }
-      const unsigned long long bal = __ballot(near);
-      const int before_w = __popcll(bal & ((1ull << lane) - 1ull));
-      __syncthreads();  // previous chunk's candidates consumed
-      if (lane == 0) wsum[wv] = __popcll(bal);
-      __syncthreads();
-      int base = 0, nk = 0;
-#pragma unroll
-      for (int w = 0; w < kWcaThreads / 64; ++w) {
-        base += w < wv ? wsum[w] : 0;
-        nk += wsum[w];
-      }
-      if (near) {
-        ccell[base + before_w] = cj;
-        coff[base + before_w] = dj;
-      }
-      __syncthreads();
+      const int nk = min(kWcaCap, nnear - p * kWcaCap);
       if (own) {
         for (int k = sl; k < nk; k += kWcaSlices) {
           const int4 cc = ccell[k];
