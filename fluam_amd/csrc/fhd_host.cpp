@@ -1409,6 +1409,7 @@ void fhd_internal_set_error(const char* e) { g_last_error = e ? e : ""; }
 // Batches of independent replicas and the completion / memory queries the serving loop uses.
 // No fluam counterpart: fluam runs one simulation per process.
 // ------------------------------------------------------------------------------------------
+constexpr int kMaxBranches = 8;
 struct fhd_batch {
   std::vector<fhd_ctx*> reps;  // each replica owns its memory and stream
   hipStream_t timer = nullptr;
@@ -1429,6 +1430,10 @@ struct fhd_batch {
   void* tab[2] = {nullptr, nullptr};
   int ndispatch[2] = {0, 0};
   bool timer_ahead = false;  // merged work on the timer stream not yet ordered before replica streams
+  int branches = 4;          // parallel graph branches per round of merged dispatches
+  hipStream_t side[kMaxBranches] = {};
+  hipEvent_t side_done[kMaxBranches] = {};
+  hipEvent_t round_fork = nullptr;
 };
 
 namespace {
@@ -1515,6 +1520,11 @@ void fhd_batch_destroy(fhd_batch* b) {
     if (e) (void)hipGraphExecDestroy(e);
   for (auto& t : b->tab)
     if (t) (void)hipFree(t);
+  for (auto& x : b->side)
+    if (x) (void)hipStreamDestroy(x);
+  for (auto& x : b->side_done)
+    if (x) (void)hipEventDestroy(x);
+  if (b->round_fork) (void)hipEventDestroy(b->round_fork);
   for (auto& e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1599,6 +1609,7 @@ struct MergedDispatch {
   unsigned nblocks;
   size_t blk, dims, args;  // byte offsets into the phase's device table
   int stride;
+  int round;               // dispatches of one round touch disjoint replicas
 };
 
 std::vector<LaunchRecord> record_step(fhd_ctx* c) {
@@ -1631,6 +1642,7 @@ void merge_lists(const std::vector<std::vector<LaunchRecord>>& lists, std::vecto
                  std::vector<unsigned char>& image) {
   const int R = (int)lists.size();
   std::vector<size_t> pos(R, 0);
+  int round = 0;
   auto put = [&](const void* src, size_t n) {
     const size_t at = align_up(image.size(), 256);
     image.resize(at + n);
@@ -1674,6 +1686,7 @@ void merge_lists(const std::vector<std::vector<LaunchRecord>>& lists, std::vecto
     for (auto& g : groups) {
       const LaunchRecord& first = lists[g.second[0]][pos[g.second[0]]];
       MergedDispatch d;
+      d.round = round;
       d.fn = g.first;
       d.threads = first.threads;
       d.stride = (int)align_up(first.args.size(), 16);
@@ -1697,6 +1710,7 @@ void merge_lists(const std::vector<std::vector<LaunchRecord>>& lists, std::vecto
       out.push_back(d);
       for (int r : g.second) pos[r]++;
     }
+    ++round;
   }
 }
 
@@ -1720,6 +1734,12 @@ void fork_timer_to_replicas(fhd_batch* b) {
 }
 
 void build_merged(fhd_batch* b) {
+  if (const char* e = getenv("FHD_BATCH_BRANCHES")) b->branches = std::max(1, std::min(kMaxBranches, atoi(e)));
+  if (!b->round_fork) HIPCHK(hipEventCreateWithFlags(&b->round_fork, hipEventDisableTiming));
+  for (int k = 0; k < b->branches; ++k) {
+    if (!b->side[k]) HIPCHK(hipStreamCreate(&b->side[k]));
+    if (!b->side_done[k]) HIPCHK(hipEventCreateWithFlags(&b->side_done[k], hipEventDisableTiming));
+  }
   b->period = 1;
   for (fhd_ctx* c : b->reps)
     if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) b->period = 2;
@@ -1742,13 +1762,32 @@ void build_merged(fhd_batch* b) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(b->timer, hipStreamCaptureModeThreadLocal));
     try {
-      for (const MergedDispatch& d : ds) {
-        const int4* blk = reinterpret_cast<const int4*>(base + d.blk);
-        const int4* dims = reinterpret_cast<const int4*>(base + d.dims);
-        const unsigned char* args = base + d.args;
-        int stride = d.stride;
-        void* kargs[4] = {&blk, &dims, &args, &stride};
-        HIPCHK(hipLaunchKernel(d.fn, dim3(d.nblocks), dim3(d.threads), kargs, 0, b->timer));
+      // the dispatches of a round are independent: up to `branches` of them run as parallel
+      // branches of the graph (fork from the timer stream, join back before the next round)
+      for (size_t i0 = 0; i0 < ds.size();) {
+        size_t i1 = i0;
+        while (i1 < ds.size() && ds[i1].round == ds[i0].round) ++i1;
+        const int nb = std::min<int>((int)(i1 - i0), b->branches);
+        if (nb > 1) {
+          HIPCHK(hipEventRecord(b->round_fork, b->timer));
+          for (int k = 0; k < nb; ++k) HIPCHK(hipStreamWaitEvent(b->side[k], b->round_fork, 0));
+        }
+        for (size_t i = i0; i < i1; ++i) {
+          const MergedDispatch& d = ds[i];
+          const int4* blk = reinterpret_cast<const int4*>(base + d.blk);
+          const int4* dims = reinterpret_cast<const int4*>(base + d.dims);
+          const unsigned char* args = base + d.args;
+          int stride = d.stride;
+          void* kargs[4] = {&blk, &dims, &args, &stride};
+          hipStream_t st = nb > 1 ? b->side[(i - i0) % nb] : b->timer;
+          HIPCHK(hipLaunchKernel(d.fn, dim3(d.nblocks), dim3(d.threads), kargs, 0, st));
+        }
+        if (nb > 1)
+          for (int k = 0; k < nb; ++k) {
+            HIPCHK(hipEventRecord(b->side_done[k], b->side[k]));
+            HIPCHK(hipStreamWaitEvent(b->timer, b->side_done[k], 0));
+          }
+        i0 = i1;
       }
     } catch (...) {
       (void)hipStreamEndCapture(b->timer, &g);

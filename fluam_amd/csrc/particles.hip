@@ -193,7 +193,7 @@ FHD_DEV int wave_incl_scan(int v) {
 }
 
 constexpr int kWcaThreads = 256;
-constexpr int kWcaSlices = 16;  // lanes sharing one particle's candidates
+constexpr int kWcaSlices = 16;  // most lanes sharing one particle's candidates
 constexpr int kWcaCap = 512;    // culled candidates staged in LDS at once
 
 // WCA, one workgroup per tile: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for
@@ -201,9 +201,10 @@ constexpr int kWcaCap = 512;    // culled candidates staged in LDS at once
 // potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).  The
 // particles of the neighbouring tiles that lie within rc of the tile are compacted into LDS in
 // sorted order (a deterministic ballot compaction), once per tile when they fit kWcaCap
-// entries, else in passes of kWcaCap; 16 lanes share each of the tile's particles, lane l
-// taking candidates l, l + 16, ... of each pass, and the partial forces are combined by a
-// fixed xor-shuffle tree, so every sum is formed in the same order on every run.
+// entries, else in passes of kWcaCap; S lanes share each of the tile's particles (S = 16, or
+// the largest power of two with S x particles <= 256 in crowded tiles), lane l taking
+// candidates l, l + S, ... of each pass, and the partial forces are combined by a fixed
+// xor-shuffle tree, so every sum is formed in the same order on every run.
 FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   __shared__ int4 ccell[kWcaCap];   // w: sorted slot
   __shared__ float4 coff[kWcaCap];
@@ -303,8 +304,12 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   };
   const int nnear = stage(0);
   const int npass = (nnear + kWcaCap - 1) / kWcaCap;
-  for (int i0 = 0; i0 < nown; i0 += kWcaThreads / kWcaSlices) {
-    const int il = tid / kWcaSlices, sl = tid % kWcaSlices;
+  // lanes per own particle: up to kWcaSlices, fewer in crowded tiles so that one group of
+  // kWcaThreads / S particles covers the tile and the candidates are staged once
+  int S = kWcaSlices;
+  while (S > 1 && S * nown > kWcaThreads) S >>= 1;
+  for (int i0 = 0; i0 < nown; i0 += kWcaThreads / S) {
+    const int il = tid / S, sl = tid % S;
     const int si = s0 + i0 + il;
     const bool own = i0 + il < nown;
     int4 ci = make_int4(0, 0, 0, 0);
@@ -321,7 +326,7 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
       }
       const int nk = min(kWcaCap, nnear - p * kWcaCap);
       if (own) {
-        for (int k = sl; k < nk; k += kWcaSlices) {
+        for (int k = sl; k < nk; k += S) {
           const int4 cc = ccell[k];
           if (cc.w == si) continue;
           const float4 dd = coff[k];
@@ -343,8 +348,7 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
         }
       }
     }
-#pragma unroll
-    for (int o = 1; o < kWcaSlices; o <<= 1) {
+    for (int o = 1; o < S; o <<= 1) {
       fx += __shfl_xor(fx, o);
       fy += __shfl_xor(fy, o);
       fz += __shfl_xor(fz, o);
