@@ -97,7 +97,10 @@ def plain_back_to_back(F, params_steps):
 
 def bench_batched(args):
     """256 independent Stokes-limit replicas, grid edges drawn per axis from {16,24,32,48,64},
-    0-2048 seeded-random particles, 3-point kernel, own seeds, graph-replayed steps."""
+    seeded particle counts in [0, 2048] on fluam's default simple-cubic start, 3-point kernel,
+    own seeds, graph-replayed steps.  (Uniform-random starts at up to half a particle per cell
+    put pairs deep inside the WCA core, whose force then asks for displacements of ~1e16 cells;
+    the lattice is the physically sensible start at these densities.)"""
     import numpy as np
     import fluam_amd as F
     from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
@@ -108,7 +111,7 @@ def bench_batched(args):
     for i in range(args.replicas):
         shape = tuple(int(rng.choice(edges)) for _ in range(3))
         params.append(make_params(SCHEME_STOKES_LIMIT, cells=shape, np=int(rng.integers(0, 2049)), kernel_points=3,
-                                  temperature=1e-3, dt=0.05, seed=1000 + i, initial_positions=0,
+                                  temperature=1e-3, dt=0.05, seed=1000 + i, initial_positions=1,
                                   cutoff=2 ** (1 / 6), wca_sigma=1.0))
     cells = sum(p.mx * p.my * p.mz for p in params)
     parts = sum(p.np for p in params)
@@ -127,7 +130,7 @@ def bench_batched(args):
         "particle_steps_per_s": parts * args.steps / (ms * 1e-3), "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: seeded grid shapes, particle counts and uniform-random positions; zero initial velocity",
+        "data": "synthetic: seeded grid shapes and particle counts, simple-cubic particle start; zero initial velocity",
         "config": {"workload": f"batched ensemble: {args.replicas} Stokes-limit replicas, edges per axis from "
                                "{16,24,32,48,64}, 0-2048 particles, 3-point kernel, own seeds, graph-replayed steps",
                    "replicas": args.replicas, "total_cells": cells, "total_particles": parts,
@@ -151,11 +154,12 @@ def bench_served(args):
         edge = int(rng.choice((32, 64, 96, 128)))
         N = edge ** 3
         qnb = bool(rng.integers(0, 2))
-        # at most one particle per cell (uniform draw up to 65,536); QNB starts on a lattice
+        # at most one particle per cell (uniform draw up to 65,536), on fluam's simple-cubic start:
+        # uniform-random starts this dense put pairs deep inside the WCA core
         npart = int(rng.integers(0, min(65536, N) + 1))
         p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT if qnb else SCHEME_STOKES_LIMIT, cells=(edge, edge, edge),
                         np=npart, kernel_points=3 if qnb else 4, temperature=1e-3, dt=0.05, seed=2000 + i,
-                        initial_positions=1 if qnb else 0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+                        initial_positions=1, cutoff=2 ** (1 / 6), wca_sigma=1.0)
         return p, int(rng.integers(50, 501))
 
     jobs = serve.poisson_jobs(args.rate, args.jobs, 5, make)

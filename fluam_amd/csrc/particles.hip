@@ -811,6 +811,11 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
       atomicOr(a.err, 1);
       disp[0] = disp[1] = disp[2] = 0.0f;
     }
+    // a displacement of 2^24 cells or more along an axis (a blown-up step, e.g. two particles
+    // started on top of each other under the WCA core) leaves that coordinate where it was
+    if (!(fabsf(disp[0]) < 16777216.0f * g.dx)) disp[0] = 0.0f;
+    if (!(fabsf(disp[1]) < 16777216.0f * g.dy)) disp[1] = 0.0f;
+    if (!(fabsf(disp[2]) < 16777216.0f * g.dz)) disp[2] = 0.0f;
     float ox = bo.x + disp[0], oy = bo.y + disp[1], oz = bo.z + disp[2];
     renorm_axis(bc.x, ox, g.dx, g.idx);
     renorm_axis(bc.y, oy, g.dy, g.idy);

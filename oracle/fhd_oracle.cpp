@@ -252,6 +252,14 @@ static inline int cid(const orc_ctx* c, int x, int y, int z) {
 }
 static inline double wrapd(double r, double L) { return r - L * floor(r / L + 0.5); }
 
+// One step's particle displacement along axis a.  A displacement of 2^24 cells or more (a
+// blown-up step, e.g. two particles started on top of each other under the WCA core) leaves
+// that coordinate where it was, as the GPU path does: its FP32 cell offsets cannot carry it.
+static inline double step_disp(const orc_ctx* c, int a, double d) {
+  const double h = a == 0 ? c->dx : a == 1 ? c->dy : c->dz;
+  return fabs(d) < 16777216.0 * h ? d : 0.0;
+}
+
 // ---- fluid stencils (src/kernelConstructW.cu:32-259) ----
 // Laplacian term 0.5 dt (eta/rho) L v, component a at cell i.
 static void lap_term(const orc_ctx* c, const std::vector<double>* v, std::vector<double>* out) {
@@ -838,7 +846,7 @@ static void step_stokes(orc_ctx* c) {
     for (int a = 0; a < 3; ++a) {
       double d = 0.5 * dt * up[a];
       if (c->p.extra_mobility != 0.0) d += 0.5 * dt * c->p.extra_mobility * c->F[a][i] + fact2 * P1[a][i];
-      qh[a][i] = qp[a] + d;
+      qh[a][i] = qp[a] + step_disp(c, a, d);
     }
   }
   // 4. F^{n+1/2}; f = S(q^{n+1/2}) F^{n+1/2} + (kT/delta)[S(q+) W - S(q-) W]
@@ -862,7 +870,7 @@ static void step_stokes(orc_ctx* c) {
     for (int a = 0; a < 3; ++a) {
       double d = dt * up[a];
       if (c->p.extra_mobility != 0.0) d += dt * c->p.extra_mobility * Fh[a][i] + fact2 * (P1[a][i] + P2[a][i]);
-      c->q[a][i] += d;
+      c->q[a][i] += step_disp(c, a, d);
       c->u[a][i] = up[a];
     }
   }
@@ -894,7 +902,7 @@ static void qnb_half_positions(orc_ctx* c, const std::vector<double>* v, double 
   for (int i = 0; i < np; ++i) {
     double qp[3] = {qeval[0][i], qeval[1][i], qeval[2][i]}, up[3];
     interp_one(c, qp, v, up);
-    for (int a = 0; a < 3; ++a) out[a][i] = qbase[a][i] + frac * c->p.dt * up[a];
+    for (int a = 0; a < 3; ++a) out[a][i] = qbase[a][i] + step_disp(c, a, frac * c->p.dt * up[a]);
   }
 }
 

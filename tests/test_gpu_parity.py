@@ -342,3 +342,24 @@ def test_autotuned_schedule_changes_no_bits():
     tuned.step(6)
     assert np.array_equal(fixed.velocity(), tuned.velocity())
     assert np.array_equal(fixed.particles(), tuned.particles())
+
+
+
+def test_blown_up_start_moves_no_coordinate_by_2p24_cells():
+    """A dense uniform-random start (pairs deep inside the WCA core) asks for displacements of
+    ~1e16 cells.  Such a coordinate stays where it was, so no coordinate moves by 2^24 cells or
+    more in a step, everything stays finite, and particles are not stacked on one another."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=2048, kernel_points=3, temperature=1e-3, dt=0.05,
+                    seed=5, initial_positions=0, **WCA)
+    _announce("blown_up_start", p)
+    g = F.Simulation(p)
+    q = g.particles()
+    for _ in range(4):
+        g.step(1)
+        g.sync()
+        qn = g.particles()
+        assert np.all(np.isfinite(qn))
+        assert np.abs(qn - q).max() < 2.0 ** 24
+        q = qn
+    pos = np.stack([q[:2048], q[2048:4096], q[4096:]], axis=1)
+    assert len(np.unique(pos, axis=0)) == 2048

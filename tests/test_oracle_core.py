@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
+from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT, SCHEME_QUASI_NEUTRALLY_BUOYANT
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fft_golden.npz")
 
@@ -226,3 +226,22 @@ def test_initial_positions_uniform_and_cubic():
     p.np = 27
     q = O.initial_positions(p).reshape(3, -1)
     assert sorted(set(np.round(q[0], 9))) == pytest.approx([-16 / 3, 0.0, 16 / 3])
+
+
+
+@pytest.mark.parametrize("scheme", [SCHEME_STOKES_LIMIT, SCHEME_QUASI_NEUTRALLY_BUOYANT])
+def test_blown_up_step_leaves_the_coordinate(scheme):
+    """A dense uniform-random start puts pairs deep inside the WCA core, and the capped pair
+    force then asks for displacements of ~1e16 cells.  No coordinate may move by 2^24 cells or
+    more in one step (step_disp keeps it where it was), and everything stays finite."""
+    p = make_params(scheme, cells=(8, 8, 8), np=512, kernel_points=3, temperature=1e-3, dt=0.05, seed=7,
+                    initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+    o = O.Oracle(p)
+    q0 = o.particles()
+    o.step(1)
+    q1 = o.particles()
+    assert np.all(np.isfinite(q1))
+    assert np.abs(q1 - q0).max() < 2.0 ** 24
+    # the start is blown up: the Stokes half step dt/2 J L^-1 S F asks for >= 2^24 cells somewhere
+    u = o.interp(q0, o.solve_stokes(o.spread(q0, o.wca(q0))))
+    assert np.abs(0.5 * p.dt * u).max() > 2.0 ** 24
