@@ -191,6 +191,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--workload", choices=("stokes128", "batched", "served"), default="stokes128",
                     help="stokes128 (default): the headline step; batched / served: the ensemble extensions")
+    ap.add_argument("--graph", type=int, default=1, choices=(0, 1),
+                    help="stokes128: 1 = replay the steady-state step as a captured HIP graph, 0 = eager launches")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=8.0, help="served: job arrivals per second")
@@ -223,7 +225,8 @@ def main():
     p = make_params(SCHEME_STOKES_LIMIT, cells=(mx, my, mz), np=npart, kernel_points=4, temperature=1e-3,
                     dt=0.05, seed=3, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
     sim = F.Simulation(p)
-    sim.step(args.warmup)
+    sim.set_graph(bool(args.graph))  # steady-state steps replay one captured HIP graph
+    sim.step(max(args.warmup, 1))
     sim.sync()
     t0 = time.perf_counter()
     ev_ms = sim.time_steps(args.steps)  # HIP events on the context's stream
@@ -257,7 +260,8 @@ def main():
         "data": "synthetic: seeded uniform-random particle positions, zero initial velocity, Philox noise",
         "config": {"workload": "stokesLimit midpoint step, 128^3 periodic staggered grid, 65536 particles, "
                                "4-point Peskin kernel, WCA sigma=1 rc=2^(1/6) eps=kT, kT=1e-3, eta=1, dt=0.05",
-                   "cells": [mx, my, mz], "particles": npart},
+                   "cells": [mx, my, mz], "particles": npart,
+                   "launch": "HIP graph replay of the step" if args.graph else "eager launches"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic(dom),
                      "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},

@@ -185,7 +185,9 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
   __shared__ float2 twz[MZ], ezs[MZ];
   __shared__ float gzs[MZ];
   const int P = a.g.P, my = a.g.my;
-  const int ky = b.y, tile = b.x;
+  // neighbouring column tiles share the 128-byte lines of each z row: keep them on one XCD
+  const int item = xcd_item(b.x + b.nx * b.y, b.nx * b.ny);
+  const int ky = item / b.nx, tile = item - ky * b.nx;
   const int ntile = P / T, rem = P - ntile * T;
   const int TC = T + (tile < rem ? 1 : 0);
   for (int i = threadIdx.x; i < MZ; i += NT) {

@@ -6,6 +6,15 @@
 
 #define FHD_DEV __device__ __forceinline__
 
+// Work item (tile, column block, or block of sorted particles) of workgroup b out of n.
+// Workgroups are dealt round-robin to the eight XCDs, each with its own L2; giving every XCD a
+// contiguous range of items keeps neighbouring items, which share cache lines, on one L2.
+FHD_DEV int xcd_item(int b, int n) {
+  constexpr int kXcd = 8;
+  if (n % kXcd) return b;
+  return (b % kXcd) * (n / kXcd) + b / kXcd;
+}
+
 // Noise streams (counter word 1 = stream << 4 | sub).
 enum { FHD_STREAM_FLUID = 1, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4, FHD_STREAM_PART2 = 5,
        FHD_STREAM_INIT_POS = 8 };

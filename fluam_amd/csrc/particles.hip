@@ -172,15 +172,6 @@ FHD_DEV void nb_range(int nt, int& lo, int& hi) {
   hi = nt >= 2 ? 1 : 0;
 }
 
-// Work item (tile, or block of sorted particles) of workgroup b out of n.  Workgroups are dealt
-// round-robin to the eight XCDs, each with its own L2; giving every XCD a contiguous range of
-// items keeps neighbouring tiles, which share particles and field cache lines, on one L2.
-FHD_DEV int xcd_item(int b, int n) {
-  constexpr int kXcd = 8;
-  if (n % kXcd) return b;
-  return (b % kXcd) * (n / kXcd) + b / kXcd;
-}
-
 // Inclusive prefix sum over the 64 lanes of a wave.
 FHD_DEV int wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;
