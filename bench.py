@@ -1,5 +1,6 @@
 """Benchmark: the Stokes-limit (midpoint) step on one MI355X, 128^3 staggered grid, 65,536
-particles at seeded uniform-random positions, 4-point Peskin kernel, WCA (sigma=1,
+particles at seeded uniform-random positions (random sequential addition: no pair starts
+closer than sigma, so no pair sits inside the WCA core), 4-point Peskin kernel, WCA (sigma=1,
 rc=2^(1/6), epsilon=kT), kT=1e-3, eta=1, dt=0.05, FP32 fields.
 
 Prints one JSON line: whole-job throughput (cell-updates/s; particle-steps/s alongside), the
@@ -36,9 +37,9 @@ def kernel_bytes(name, mx, my, mz, npart):
     if name == "fft_xy_inv":
         return spec + 3 * N * 4              # read half spectrum, write real v
     if name == "fluid_w":
-        return 3 * N * 4 + 3 * N * 4         # read S F, write W (noise in registers)
+        return 3 * N * 4                     # Stokes: write W = D.Sigma (noise in registers)
     if name == "spread":
-        return 3 * N * 4 + 32 * npart        # write S F, read sorted particles
+        return 2 * 3 * N * 4 + 32 * npart    # read W, write W + S F, read sorted particles
     return None
 
 
@@ -62,7 +63,7 @@ def cpu_baseline(args):
 
     threads = O.lib().orc_set_threads(0)
     p = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
-                    dt=0.05, seed=3, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+                    dt=0.05, seed=3, initial_positions=3, cutoff=2 ** (1 / 6), wca_sigma=1.0)
     o = O.Oracle(p)
     o.step(1)  # warm-up (page-in, plans)
     t0 = time.perf_counter()
@@ -141,6 +142,54 @@ def bench_batched(args):
     }
 
 
+def bench_config(args):
+    """Config 1 (incompressible, 32^3, no particles) or config 2 (inertial coupling, mass 0,
+    64^3, 4,096 particles at seeded uniform-random positions, 3-point kernel, WCA), eager
+    launches, inputs resident; the FP64 oracle timed on a bounded sample beside it."""
+    import fluam_amd as F
+    from fluam_amd._abi import make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_NEUTRALLY_BUOYANT
+
+    if args.workload == "incomp32":
+        p = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=11)
+        desc = "schemeIncompressible, 32^3 periodic staggered grid, no particles, kT=1e-3, rho=eta=1, dt=0.05"
+    else:
+        p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(64, 64, 64), np=4096, kernel_points=3,
+                        temperature=1e-3, dt=0.05, seed=5, initial_positions=3, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+        desc = ("schemeQuasiNeutrallyBuoyant (mass 0), 64^3 periodic staggered grid, 4096 particles at seeded "
+                "uniform-random positions without pairs closer than sigma, 3-point kernel, WCA sigma=1 "
+                "rc=2^(1/6) eps=kT, kT=1e-3, dt=0.05")
+    N = p.mx * p.my * p.mz
+    sim = F.Simulation(p)
+    sim.step(max(args.warmup, 2))
+    sim.sync()
+    ms = sim.time_steps(args.steps)
+    sim.sync()
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        threads = O.lib().orc_set_threads(0)
+        o = O.Oracle(p)
+        o.step(1)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 10.0) and n < 200:
+            o.step(1)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+        cpu = {"value": N / dt, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
+               "sample": f"FP64 oracle, same workload, {n} timed steps after 1 warm-up, OpenMP {threads} threads"}
+    return {
+        "metric": "fluid cell-updates/s + IB particle-steps/s", "value": N * args.steps / (ms * 1e-3),
+        "unit": "cell-updates/s", "particle_steps_per_s": p.np * args.steps / (ms * 1e-3), "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: zero initial velocity, Philox noise" + (", seeded uniform-random positions" if p.np else ""),
+        "config": {"workload": desc, "cells": [p.mx, p.my, p.mz], "particles": p.np},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_served(args):
     """Poisson arrivals of Stokes-limit and inertial-coupling (QNB) jobs on cubic grids with edge
     from {32,64,96,128}, seeded particle counts and 50-500 steps, served by the continuous-
@@ -189,8 +238,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--workload", choices=("stokes128", "batched", "served"), default="stokes128",
-                    help="stokes128 (default): the headline step; batched / served: the ensemble extensions")
+    ap.add_argument("--workload", choices=("stokes128", "batched", "served", "incomp32", "qnb64"), default="stokes128",
+                    help="stokes128 (default): the headline step; batched / served: the ensemble extensions; "
+                         "incomp32 / qnb64: the incompressible and inertial-coupling configurations")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="stokes128: 1 = replay the steady-state step as a captured HIP graph, 0 = eager launches "
                          "(measured 0.501 vs 0.507 ms per step: the host keeps the queue full either way)")
@@ -207,6 +257,11 @@ def main():
         print("bench.py measures one GPU only: the step is a single-GPU kernel pipeline; "
               "how it would shard over several GPUs is a design note, not a run", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("incomp32", "qnb64"):
+        if args.steps is None:
+            args.steps = 1000 if args.workload == "qnb64" else 100
+        print(json.dumps(bench_config(args)))
+        return
     if args.workload != "stokes128":
         if args.hw_queues is not None:
             os.environ["GPU_MAX_HW_QUEUES"] = str(max(1, min(args.hw_queues, 16)))
@@ -224,7 +279,7 @@ def main():
     mx = my = mz = 128
     npart = 65536
     p = make_params(SCHEME_STOKES_LIMIT, cells=(mx, my, mz), np=npart, kernel_points=4, temperature=1e-3,
-                    dt=0.05, seed=3, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+                    dt=0.05, seed=3, initial_positions=3, cutoff=2 ** (1 / 6), wca_sigma=1.0)
     sim = F.Simulation(p)
     sim.set_graph(bool(args.graph))  # steady-state steps replay one captured HIP graph
     sim.step(max(args.warmup, 1))
@@ -258,7 +313,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded uniform-random particle positions, zero initial velocity, Philox noise",
+        "data": "synthetic: seeded uniform-random particle positions without pairs closer than sigma (random "
+                "sequential addition), zero initial velocity, Philox noise",
         "config": {"workload": "stokesLimit midpoint step, 128^3 periodic staggered grid, 65536 particles, "
                                "4-point Peskin kernel, WCA sigma=1 rc=2^(1/6) eps=kT, kT=1e-3, eta=1, dt=0.05",
                    "cells": [mx, my, mz], "particles": npart,

@@ -134,6 +134,9 @@ struct fhd_ctx {
   bool own_stream = true;
   int zchunk = 8;                        // W kernel: z planes per workgroup (schedule)
   hipEvent_t done_ev = nullptr;          // fhd_record / fhd_query
+  // Stokes: the stochastic part of W is built on a side stream while the particle kernels run
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_noise = nullptr;
 };
 
 namespace {
@@ -267,6 +270,12 @@ void free_ctx(fhd_ctx* c) {
     if (e) (void)hipGraphExecDestroy(e);
   if (c->step_dev) (void)hipFree(c->step_dev);
   if (c->done_ev) (void)hipEventDestroy(c->done_ev);
+  if (c->s2) {
+    (void)hipStreamSynchronize(c->s2);
+    (void)hipStreamDestroy(c->s2);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_noise) (void)hipEventDestroy(c->ev_noise);
   if (c->s && c->own_stream) (void)hipStreamDestroy(c->s);
 }
 
@@ -498,6 +507,31 @@ void step_incompressible(fhd_ctx* c) {
 
 // Stokes limit, midpoint: src/runSchemeStokesLimit.cu:163-276,
 // src/boundaryParticlesFunctionStokesLimit.cu:21-249.
+// W = S F (+ RFD drift) + noise D.Sigma.  The noise part needs nothing but the step counter, so
+// it is built into W on the side stream s2 while the particle kernels (sorting, WCA, source
+// records) run on the main stream; the spreading pass then adds S F into W node by node
+// (the same single float addition as a combined pass, so the same bits).  W is free for the
+// next noise pass once the xy-forward FFT pass has read it.  When profiling, everything runs
+// on the main stream so per-kernel times stay separable.
+void stokes_noise_begin(fhd_ctx* c, int noise, float f1, float f4) {
+  const bool side = !c->profiling;
+  if (side) {
+    HIPCHK(hipEventRecord(c->ev_fork, c->s));
+    HIPCHK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
+  }
+  FluidWArgs w = base_w(c);
+  w.trace_free = 0;
+  w.noise = noise;
+  w.f1 = f1;
+  w.f4 = f4;
+  launch_fluid_w(w, side ? c->s2 : c->s);
+  mark(c, "fluid_w");
+  if (side) HIPCHK(hipEventRecord(c->ev_noise, c->s2));
+}
+void stokes_noise_join(fhd_ctx* c) {
+  if (!c->profiling) HIPCHK(hipStreamWaitEvent(c->s, c->ev_noise, 0));
+}
+
 void step_stokes(fhd_ctx* c) {
   const fhd_params& p = c->p;
   const double dV = c->dV, dt = p.dt;
@@ -505,10 +539,8 @@ void step_stokes(fhd_ctx* c) {
   const float f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity / (dV * dt));
   const float fact2 = (float)sqrt(p.temperature * p.extra_mobility * dt);
   const bool xm = p.extra_mobility != 0.0;
-  FluidWArgs w = base_w(c);
-  w.force = c->SF;
-  w.force_coef = 1.0f;
-  w.trace_free = 0;
+  // solve 1: W = S F^n + sqrt(2) D.Sigma_1
+  stokes_noise_begin(c, 1, (float)(sqrt(2.0) * f1), (float)(sqrt(2.0) * f4));
   if (c->np > 0) {
     sort_set(c, 0);
     wca(c, 0, c->F[0]);
@@ -518,16 +550,16 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[0].mode = SRC_VALUE;
     sa.grp[0].val = c->F[0];
     sa.grp[0].coef = 1.0f;
+    sa.out = c->W;
+    sa.out_add = c->W;
+    stokes_noise_join(c);
     run_spread(c, sa);
   } else {
-    launch_zero_f(c->SF, 3 * (size_t)c->g.N, c->s);
+    stokes_noise_join(c);
   }
-  w.noise = 1;
-  w.f1 = (float)(sqrt(2.0) * f1);
-  w.f4 = (float)(sqrt(2.0) * f4);
-  launch_fluid_w(w, c->s);
-  mark(c, "fluid_w");
   fft_forward(c, c->W);
+  // solve 2's noise: W = sqrt(1/2) D.(Sigma_1 + Sigma_2), overlapping the rest of solve 1
+  stokes_noise_begin(c, 2, (float)(sqrt(0.5) * f1), (float)(sqrt(0.5) * f4));
   fft_solve(c, 1, 0.0f);
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
   if (c->np > 0) {
@@ -559,13 +591,13 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[1].coef = (float)(p.temperature / p.rfd_delta);
     sa.grp[1].disp = (float)(0.5 * p.rfd_delta);
     sa.grp[1].rfd_quirk = p.rfd_mode == FHD_RFD_REFERENCE;
+    sa.out = c->W;
+    sa.out_add = c->W;
+    stokes_noise_join(c);
     run_spread(c, sa);
+  } else {
+    stokes_noise_join(c);
   }
-  w.noise = 2;
-  w.f1 = (float)(sqrt(0.5) * f1);
-  w.f4 = (float)(sqrt(0.5) * f4);
-  launch_fluid_w(w, c->s);
-  mark(c, "fluid_w");
   fft_forward(c, c->W);
   fft_solve(c, 1, 0.0f);
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
@@ -968,14 +1000,65 @@ void initial_positions(const fhd_params& p, std::vector<double>& q) {
         }
     return;
   }
-  // seeded uniform positions: Philox draw (seed, step 0, stream 8, index i), 24-bit uniforms
+  // seeded uniform positions: Philox draw (seed, attempt k, stream 8, index i), 24-bit uniforms.
+  // Mode 0 takes attempt 0 of every particle.  Mode 3 (random sequential addition) redraws a
+  // particle (attempt 1, 2, ...) while it lies closer than the WCA sigma to an earlier one, by
+  // the minimum-image distance, so no pair starts inside the repulsive core.
   const uint32_t key[2] = {(uint32_t)p.seed, (uint32_t)(p.seed >> 32)};
-  for (int i = 0; i < np; ++i) {
-    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)FHD_STREAM_INIT_POS << 4, 0u, 0u}, w[4];
+  const double L[3] = {p.lx, p.ly, p.lz};
+  auto draw = [&](int i, uint32_t attempt, double x[3]) {
+    uint32_t ctr[4] = {(uint32_t)i, (uint32_t)FHD_STREAM_INIT_POS << 4, attempt, 0u}, w[4];
     philox_host(ctr, key, w);
-    q[i] = ((double)(w[0] >> 8) / 16777216.0 - 0.5) * p.lx;
-    q[np + i] = ((double)(w[1] >> 8) / 16777216.0 - 0.5) * p.ly;
-    q[2 * (size_t)np + i] = ((double)(w[2] >> 8) / 16777216.0 - 0.5) * p.lz;
+    for (int a = 0; a < 3; ++a) x[a] = ((double)(w[a] >> 8) / 16777216.0 - 0.5) * L[a];
+  };
+  const bool rsa = p.initial_positions == 3 && p.nonbonded && p.cutoff > 0;
+  const double dmin = p.wca_sigma > 0 ? p.wca_sigma : 2.0 * p.lx / p.mx;
+  int m[3] = {1, 1, 1};
+  for (int a = 0; a < 3; ++a) m[a] = rsa ? std::max(1, (int)(L[a] / dmin)) : 1;
+  std::vector<std::vector<int>> cells(rsa ? (size_t)m[0] * m[1] * m[2] : 0);
+  auto cell_of = [&](const double x[3], int j[3]) {
+    for (int a = 0; a < 3; ++a) j[a] = std::min(m[a] - 1, std::max(0, (int)((x[a] / L[a] + 0.5) * m[a])));
+  };
+  constexpr int kMaxAttempts = 100000;
+  for (int i = 0; i < np; ++i) {
+    double x[3];
+    int attempt = 0;
+    for (;; ++attempt) {
+      if (attempt >= kMaxAttempts)
+        throw std::runtime_error("initialPositions 3: no overlap-free place for particle " + std::to_string(i) +
+                                 " (too dense for random sequential addition; use the lattice, initialPositions 1)");
+      draw(i, (uint32_t)attempt, x);
+      if (!rsa) break;
+      int j[3];
+      cell_of(x, j);
+      bool ok = true;
+      const int lo[3] = {m[0] >= 3 ? -1 : 0, m[1] >= 3 ? -1 : 0, m[2] >= 3 ? -1 : 0};
+      const int hi[3] = {m[0] >= 2 ? 1 : 0, m[1] >= 2 ? 1 : 0, m[2] >= 2 ? 1 : 0};
+      for (int dz = lo[2]; dz <= hi[2] && ok; ++dz)
+        for (int dy = lo[1]; dy <= hi[1] && ok; ++dy)
+          for (int dx = lo[0]; dx <= hi[0] && ok; ++dx) {
+            const int cx = (j[0] + dx + m[0]) % m[0], cy = (j[1] + dy + m[1]) % m[1], cz = (j[2] + dz + m[2]) % m[2];
+            for (int k : cells[cx + (size_t)m[0] * (cy + (size_t)m[1] * cz)]) {
+              double r2 = 0.0;
+              for (int a = 0; a < 3; ++a) {
+                double d = x[a] - q[(size_t)a * np + k];
+                d -= L[a] * std::floor(d / L[a] + 0.5);
+                r2 += d * d;
+              }
+              if (r2 < dmin * dmin) {
+                ok = false;
+                break;
+              }
+            }
+          }
+      if (ok) break;
+    }
+    for (int a = 0; a < 3; ++a) q[(size_t)a * np + i] = x[a];
+    if (rsa) {
+      int j[3];
+      cell_of(x, j);
+      cells[j[0] + (size_t)m[0] * (j[1] + (size_t)m[1] * j[2])].push_back(i);
+    }
   }
 }
 }  // namespace
@@ -1024,6 +1107,9 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device");
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_noise, hipEventDisableTiming));
     const fhd_params& p = c->p;
     c->dx = p.lx / p.mx;
     c->dy = p.ly / p.my;

@@ -127,7 +127,8 @@ struct SpreadArgs {
   SpreadGroup grp[2];
   Key key;
   float out_coef;       // multiplies the accumulated field (1/dV)
-  float* out;           // 3N field, every node written
+  float* out;           // 3N field, every node written: out = out_add + coef * sum (out_add may be null)
+  const float* out_add; // may alias out (each node is read, then written, by one thread)
   int nrec0, nrec_tot;  // records of group 0 and of both groups (set by launch_src_prep)
 };
 

@@ -609,6 +609,18 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   const int tile = xcd_item(b.x, t.n);
   const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
   const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
+  // FAST tiles: each thread's output nodes are e = tid + k * kSpreadThreads, k < kOutPer; their
+  // out_add values are loaded now, so the loads complete under the accumulation
+  constexpr int kOutPer = 3 * kTileEdge * kTileEdge * kTileEdge / kSpreadThreads;
+  float pre[kOutPer];
+  if (FAST && a.out_add) {
+#pragma unroll
+    for (int k = 0; k < kOutPer; ++k) {
+      const int e = tid + k * kSpreadThreads;
+      const int C = e >> 9, lx = e & 7, ly = (e >> 3) & 7, lz = (e >> 6) & 7;
+      pre[k] = a.out_add[(size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz))];
+    }
+  }
   for (int e = tid; e < NACC + 2 * kAccPad; e += kSpreadThreads) acc[e] = 0ull;
   // neighbour-tile table: lane k of wave 0 loads the record range of (group, neighbour) k
   int lox, hix, loy, hiy, loz, hiz;
@@ -708,6 +720,18 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   __syncthreads();
   // interior nodes -> field (8 x 8 x 8 tiles: index arithmetic by shifts)
   const int ncell = t.tx * t.ty * t.tz;
+  if (FAST) {
+#pragma unroll
+    for (int k = 0; k < kOutPer; ++k) {
+      const int e = tid + k * kSpreadThreads;
+      const int C = e >> 9, lx = e & 7, ly = (e >> 3) & 7, lz = (e >> 6) & 7;
+      const int ai = kAccPad + (C * ZS + lz + H) * kRowStride + ly * kTileEdge + lx;
+      const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
+      const float val = (float)(long long)acc[ai] * inv;
+      a.out[gi] = a.out_add ? pre[k] + val : val;
+    }
+    return;
+  }
   for (int e = tid; e < 3 * ncell; e += kSpreadThreads) {
     int C, lx, ly, lz;
     if (FAST) {
@@ -724,7 +748,8 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
     }
     const int ai = (FAST ? kAccPad : 0) + (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
     const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-    a.out[gi] = (float)(long long)acc[ai] * inv;
+    const float val = (float)(long long)acc[ai] * inv;
+    a.out[gi] = a.out_add ? a.out_add[gi] + val : val;
   }
 }
 FHD_KERNEL_PAIR((template <int NPTS, bool FAST>), kSpreadThreads, k_spread, SpreadArgs, body_spread<NPTS, FAST>(a, b))

@@ -72,7 +72,9 @@ typedef struct fhd_params {
   int rfd_mode;               /* extension keyword `rfdMode` FHD_RFD_* */
   int max_part_in_cell;       /* maxNumberPartInCell (capacity hint, per spreading tile) */
   int max_part_in_cell_nonbonded; /* maxNumberPartInCellNonBonded (capacity hint) */
-  int initial_positions;      /* 0: seeded uniform-random, 1: simple cubic (src/simpleCubic.cpp:25) */
+  int initial_positions;      /* 0: seeded uniform-random, 1: simple cubic (src/simpleCubic.cpp:25),
+                                 2: coordinates file, 3: seeded uniform-random without pairs closer
+                                 than the WCA sigma (random sequential addition) */
   char outputname[512];       /* outputname */
   char coordinates[512];      /* coordinates file (loadparticles 1) */
 } fhd_params;

@@ -620,12 +620,61 @@ extern "C" int orc_initial_positions(const fhd_params* p, double* q) {
         }
     return 1;
   }
+  // seeded uniform positions (attempt 0 of every particle), or, for mode 3, random sequential
+  // addition: a particle is redrawn (attempt 1, 2, ... in the step word of the counter) while
+  // its minimum-image distance to an earlier particle is below the WCA sigma.  Earlier
+  // particles are found through linked cell lists with cells no smaller than sigma, visiting
+  // each distinct neighbour cell once.
+  const bool rsa = p->initial_positions == 3 && p->nonbonded && p->cutoff > 0;
+  const double dmin = p->wca_sigma > 0 ? p->wca_sigma : 2.0 * p->lx / p->mx;
+  const double L[3] = {p->lx, p->ly, p->lz};
+  int m[3];
+  for (int a = 0; a < 3; ++a) m[a] = rsa ? std::max(1, (int)(L[a] / dmin)) : 1;
+  std::vector<int> head(rsa ? (size_t)m[0] * m[1] * m[2] : 0, -1), next(rsa ? np : 0, -1);
+  auto cell_index = [&](const double x[3], int j[3]) {
+    for (int a = 0; a < 3; ++a) {
+      j[a] = (int)((x[a] / L[a] + 0.5) * m[a]);
+      j[a] = std::min(std::max(j[a], 0), m[a] - 1);
+    }
+    return j[0] + m[0] * (j[1] + m[1] * j[2]);
+  };
   for (int i = 0; i < np; ++i) {
-    double u[4];
-    uniforms4(p->seed, 0, STREAM_INIT_POS, 0, (uint32_t)i, u);
-    q[i] = (u[0] - 0.5) * p->lx;
-    q[np + i] = (u[1] - 0.5) * p->ly;
-    q[2 * np + i] = (u[2] - 0.5) * p->lz;
+    double x[3];
+    for (int attempt = 0;; ++attempt) {
+      if (attempt >= 100000) return 0;
+      double u[4];
+      uniforms4(p->seed, attempt, STREAM_INIT_POS, 0, (uint32_t)i, u);
+      for (int a = 0; a < 3; ++a) x[a] = (u[a] - 0.5) * L[a];
+      if (!rsa) break;
+      int j[3];
+      cell_index(x, j);
+      std::vector<int> seen;
+      bool ok = true;
+      for (int dz = -1; dz <= 1 && ok; ++dz)
+        for (int dy = -1; dy <= 1 && ok; ++dy)
+          for (int dxx = -1; dxx <= 1 && ok; ++dxx) {
+            const int nc = wrapi(j[0] + dxx, m[0]) + m[0] * (wrapi(j[1] + dy, m[1]) + m[1] * wrapi(j[2] + dz, m[2]));
+            if (std::find(seen.begin(), seen.end(), nc) != seen.end()) continue;
+            seen.push_back(nc);
+            for (int k = head[nc]; k >= 0 && ok; k = next[k]) {
+              double r2 = 0.0;
+              for (int a = 0; a < 3; ++a) {
+                double d = x[a] - q[(size_t)a * np + k];
+                d -= L[a] * floor(d / L[a] + 0.5);
+                r2 += d * d;
+              }
+              ok = r2 >= dmin * dmin;
+            }
+          }
+      if (ok) break;
+    }
+    for (int a = 0; a < 3; ++a) q[(size_t)a * np + i] = x[a];
+    if (rsa) {
+      int j[3];
+      const int c = cell_index(x, j);
+      next[i] = head[c];
+      head[c] = i;
+    }
   }
   return 1;
 }

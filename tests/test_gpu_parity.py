@@ -246,20 +246,24 @@ def test_qnb_no_particles_equals_incompressible_path():
 
 # ---------------- headline configuration (smoke at full size) ----------------
 def test_stokes_config3_full_size_steps():
-    """128^3, 65,536 uniform-random particles, 4-point kernel, WCA: a few steps stay finite and
-    divergence-free; the oracle's first step agrees on the fluid velocity."""
+    """128^3, 65,536 seeded uniform-random particles (no pair inside sigma), 4-point kernel, WCA:
+    the first step agrees with the oracle on the fluid velocity and the positions, and a few
+    more steps stay finite and divergence-free."""
     p = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
-                    dt=0.05, seed=3, initial_positions=0, **WCA)
+                    dt=0.05, seed=3, initial_positions=3, **WCA)
     _announce("stokes_config3", p)
     g = F.Simulation(p)
-    g.step(3)
+    o = O.Oracle(p)
+    assert np.abs(g.particles() - o.particles()).max() < 1e-5
+    g.step(1)
     g.sync()
-    q = g.particles()
-    assert np.isfinite(q).all()
-    # uniform-random starts overlap strongly under WCA, so velocities are large: divergence is
-    # checked relative to the velocity scale (FP32 round-off level)
-    v = g.velocity()
-    assert g.max_divergence() * 1.0 / np.abs(v).max() < 1e-6
+    o.step(1)
+    assert rel(g.velocity(), o.velocity()) < 1e-5
+    assert rel(g.particles(), o.particles()) < 1e-5
+    g.step(2)
+    g.sync()
+    assert np.isfinite(g.particles()).all()
+    assert g.max_divergence() < 1e-6
 
 
 # ---------------- graph replay ----------------
@@ -363,3 +367,14 @@ def test_blown_up_start_moves_no_coordinate_by_2p24_cells():
         q = qn
     pos = np.stack([q[:2048], q[2048:4096], q[4096:]], axis=1)
     assert len(np.unique(pos, axis=0)) == 2048
+
+
+def test_overlap_free_random_start_matches_oracle():
+    """The GPU host's random sequential addition (cell list) places every particle where the
+    oracle's direct search does."""
+    p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(32, 32, 32), np=3000, kernel_points=3,
+                    temperature=1e-3, dt=0.05, seed=21, initial_positions=3, **WCA)
+    _announce("overlap_free_start", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    assert np.abs(g.particles() - o.particles()).max() < 1e-6

@@ -245,3 +245,31 @@ def test_blown_up_step_leaves_the_coordinate(scheme):
     # the start is blown up: the Stokes half step dt/2 J L^-1 S F asks for >= 2^24 cells somewhere
     u = o.interp(q0, o.solve_stokes(o.spread(q0, o.wca(q0))))
     assert np.abs(0.5 * p.dt * u).max() > 2.0 ** 24
+
+
+def test_overlap_free_random_start():
+    """initialPositions 3: the first draw of every particle is the uniform start's draw, and a
+    particle is redrawn only while it lies closer than sigma (minimum image) to an earlier one."""
+    p0 = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=600, kernel_points=3, temperature=1e-3,
+                     dt=0.05, seed=9, initial_positions=0, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+    p3 = p0.copy()
+    p3.initial_positions = 3
+    q0 = O.Oracle(p0).particles().reshape(3, -1).T
+    q3 = O.Oracle(p3).particles().reshape(3, -1).T
+    L = np.array([p0.lx, p0.ly, p0.lz])
+
+    def dmin_to_earlier(q, i):
+        d = q[:i] - q[i]
+        d -= L * np.floor(d / L + 0.5)
+        return np.sqrt((d * d).sum(axis=1)).min() if i else np.inf
+
+    moved = 0
+    for i in range(len(q3)):
+        assert dmin_to_earlier(q3, i) >= 1.0
+        if not np.array_equal(q3[i], q0[i]):
+            moved += 1
+            # redrawn only because the first draw overlapped an earlier accepted particle
+            d = q3[:i] - q0[i]
+            d -= L * np.floor(d / L + 0.5)
+            assert np.sqrt((d * d).sum(axis=1)).min() < 1.0
+    assert moved > 0  # at this density some first draws overlap
