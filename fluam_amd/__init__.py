@@ -16,7 +16,8 @@ from ._abi import (FhdParams, make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_N
 __all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "get_params", "main",
            "scheme_incompressible", "scheme_quasi_neutrally_buoyant", "scheme_stokes_limit",
            "SCHEME_INCOMPRESSIBLE", "SCHEME_QUASI_NEUTRALLY_BUOYANT", "SCHEME_STOKES_LIMIT",
-           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed", "mem_info"]
+           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed", "mem_info",
+           "read_fluid_file", "write_fluid_file"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libfhd.so")
@@ -46,6 +47,8 @@ def lib():
     for n in ("fhd_scheme_incompressible", "fhd_scheme_quasi_neutrally_buoyant", "fhd_scheme_stokes_limit"):
         getattr(L, n).argtypes = []
     L.fhd_main.argtypes = [C.c_char_p]
+    L.fhd_read_fluid_file.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
     L.fhd_last_error.restype = C.c_char_p
     L.fhd_create.argtypes = [C.POINTER(FhdParams)]
     L.fhd_create.restype = vp
@@ -120,6 +123,31 @@ def get_params():
 
 def set_params(p):
     lib().fhd_set_params(C.byref(p))
+
+
+def read_fluid_file(path, cells):
+    """Velocity (3N, component-major, x fastest) and density of a .fluidFinalConfiguration file
+    for a grid of `cells` = (mx, my, mz); the restart input of fluam's `fluid` keyword."""
+    mx, my, mz = cells
+    v = np.zeros(3 * mx * my * mz)
+    d = C.c_double(0.0)
+    if not lib().fhd_read_fluid_file(str(path).encode(), mx, my, mz, _p(v), C.byref(d)):
+        raise FhdError(_err())
+    return v, d.value
+
+
+def write_fluid_file(path, cells, box, v, densfluid=1.0, temperature=0.0):
+    """Writes a velocity field in fluam's .fluidFinalConfiguration layout
+    (src/saveFluidFinalConfiguration.cu:30-75), 17 significant digits so FP32 values survive."""
+    mx, my, mz = cells
+    N = mx * my * mz
+    v = np.asarray(v, dtype=np.float64).reshape(3, N)
+    with open(path, "w") as f:
+        f.write(f"Number of cells {N}\nmx my mz\n{mx} {my} {mz}\nlx ly lz\n{box[0]} {box[1]} {box[2]}\n")
+        f.write(f"Density {densfluid!r}\nPressure parameters 0 0 0\nThermostat 0\nTemperature {temperature}\n")
+        f.write("Cells properties \nDensity \nVelocity\n")
+        for i in range(N):
+            f.write(f"{densfluid!r}\n{v[0, i]:.17g} {v[1, i]:.17g} {v[2, i]:.17g}\n")
 
 
 def scheme_incompressible():

@@ -43,6 +43,8 @@ class FhdParams(C.Structure):
         ("initial_positions", C.c_int),
         ("outputname", C.c_char * 512),
         ("coordinates", C.c_char * 512),
+        ("fluid", C.c_char * 512),
+        ("initial_step", C.c_longlong),
     ]
 
     def copy(self):
@@ -92,7 +94,7 @@ def make_params(scheme=SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), box=None, **kw
         if k == "background":
             for a in range(3):
                 p.background[a] = v[a]
-        elif k in ("outputname", "coordinates"):
+        elif k in ("outputname", "coordinates", "fluid"):
             setattr(p, k, v.encode() if isinstance(v, str) else v)
         else:
             if not hasattr(p, k):

@@ -885,6 +885,8 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
       else if (word == "outputname") { std::string s; need(bool(f >> s)); snprintf(p.outputname, sizeof(p.outputname), "%s", s.c_str()); }
       else if (word == "loadparticles") { int v; need(bool(f >> v)); if (v) p.initial_positions = 2; }
       else if (word == "coordinates") { std::string s; need(bool(f >> s)); snprintf(p.coordinates, sizeof(p.coordinates), "%s", s.c_str()); }
+      else if (word == "fluid") { std::string s; need(bool(f >> s)); snprintf(p.fluid, sizeof(p.fluid), "%s", s.c_str()); }
+      else if (word == "initialStep") need(bool(f >> p.initial_step));
       else if (word == "velocities") { std::string s; need(bool(f >> s)); }
       else if (word == "computeNonBondedForces") need(bool(f >> p.nonbonded));
       else if (word == "extraMobility") need(bool(f >> p.extra_mobility));
@@ -936,6 +938,43 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
 }
 
 // coordinates file: first number np, then np lines "x y z" (src/createParticles.cpp:81-100).
+// .fluidFinalConfiguration reader: five header lines (cell count, "mx my mz", their values,
+// "lx ly lz", their values), "Density d", "Pressure parameters a0 a1 a2", five more lines,
+// then per cell its density and vx vy vz (src/initializeFluid.cpp:168-192).
+bool read_fluid_file(const char* path, int mx, int my, int mz, std::vector<double>& v, double& dens,
+                     std::string& err) {
+  std::ifstream f(path);
+  if (!f) {
+    err = std::string("cannot open fluid file ") + path;
+    return false;
+  }
+  std::string line, w;
+  std::getline(f, line);
+  std::getline(f, line);
+  int fm[3] = {0, 0, 0};
+  f >> fm[0] >> fm[1] >> fm[2];
+  if (!f || fm[0] != mx || fm[1] != my || fm[2] != mz) {
+    err = std::string("fluid file ") + path + ": grid differs from cells";
+    return false;
+  }
+  std::getline(f, line);
+  std::getline(f, line);
+  std::getline(f, line);
+  double a0, a1, a2;
+  f >> w >> dens >> w >> w >> a0 >> a1 >> a2;
+  for (int k = 0; k < 6; ++k) std::getline(f, line);
+  const long long N = (long long)mx * my * mz;
+  v.assign(3 * (size_t)N, 0.0);
+  for (long long i = 0; i < N; ++i) {
+    double d;
+    if (!(f >> d >> v[i] >> v[N + i] >> v[2 * N + i])) {
+      err = std::string("fluid file ") + path + " truncated";
+      return false;
+    }
+  }
+  return true;
+}
+
 bool read_coordinates(const fhd_params& p, std::vector<double>& q, std::string& err) {
   std::ifstream f(p.coordinates);
   if (!f) {
@@ -1070,6 +1109,19 @@ extern "C" {
 
 const char* fhd_last_error(void) { return g_last_error.c_str(); }
 
+int fhd_read_fluid_file(const char* path, int mx, int my, int mz, double* v, double* densfluid) {
+  std::vector<double> vf;
+  std::string err;
+  double dens = 0.0;
+  if (!path || !v || !read_fluid_file(path, mx, my, mz, vf, dens, err)) {
+    g_last_error = err.empty() ? "bad arguments" : err;
+    return 0;
+  }
+  std::copy(vf.begin(), vf.end(), v);
+  if (densfluid) *densfluid = dens;
+  return 1;
+}
+
 int fhd_load_datamain(const char* path) {
   fhd_params p;
   std::string err;
@@ -1140,6 +1192,16 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
     std::vector<float> v0(N3, 0.0f);
     if (p.scheme != FHD_SCHEME_STOKES_LIMIT)
       for (int a = 0; a < 3; ++a) std::fill(v0.begin() + (size_t)a * g.N, v0.begin() + (size_t)(a + 1) * g.N, (float)p.background[a]);
+    if (p.fluid[0]) {
+      // restart: velocity and density from a saved configuration (fluid keyword)
+      std::vector<double> vf;
+      std::string err;
+      double dens = p.densfluid;
+      if (!read_fluid_file(p.fluid, p.mx, p.my, p.mz, vf, dens, err)) throw std::runtime_error(err);
+      for (size_t i = 0; i < N3; ++i) v0[i] = (float)vf[i];
+      c->p.densfluid = dens;
+    }
+    c->step = p.initial_step;
     HIPCHK(hipMemcpy(c->v, v0.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->v2, 0, N3 * sizeof(float)));
     HIPCHK(hipMemset(c->SF, 0, N3 * sizeof(float)));

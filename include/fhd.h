@@ -77,6 +77,10 @@ typedef struct fhd_params {
                                  than the WCA sigma (random sequential addition) */
   char outputname[512];       /* outputname */
   char coordinates[512];      /* coordinates file (loadparticles 1) */
+  char fluid[512];            /* fluid: initial velocity and density from a .fluidFinalConfiguration
+                                 file (src/loadDataMain.cu:443-445, src/initializeFluid.cpp:168-192) */
+  long long initial_step;     /* initialStep (extension): step counter of the first step, so a run
+                                 restarted from its saved state draws the same noise */
 } fhd_params;
 
 /* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
@@ -85,7 +89,11 @@ void fhd_default_params(fhd_params* p);
 unsigned long fhd_params_size(void);
 
 /* ---------------- global (fluam-style) API ---------------- */
-int fhd_load_datamain(const char* path);          /* parse data.main into the global params */
+int fhd_load_datamain(const char* path);
+/* Reads a .fluidFinalConfiguration file (the format of src/saveFluidFinalConfiguration.cu:30-75,
+ * read as src/initializeFluid.cpp:168-192 does) for an mx x my x mz grid: v[3N] component-major,
+ * x fastest; *densfluid receives the file's density.  CPU only; 1 on success. */
+int fhd_read_fluid_file(const char* path, int mx, int my, int mz, double* v, double* densfluid);          /* parse data.main into the global params */
 int fhd_get_params(fhd_params* out);
 int fhd_set_params(const fhd_params* in);
 int fhd_scheme_incompressible(void);

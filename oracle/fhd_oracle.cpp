@@ -679,6 +679,27 @@ extern "C" int orc_initial_positions(const fhd_params* p, double* q) {
   return 1;
 }
 
+// Restart from a .fluidFinalConfiguration file (src/initializeFluid.cpp:168-192): skip the five
+// grid lines, read "Density d" (it replaces densfluid) and the pressure line, skip the rest of
+// the header, then per cell: density, vx, vy, vz.
+static bool orc_read_fluid(orc_ctx* c, const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  char line[1024];
+  bool ok = true;
+  for (int k = 0; k < 5 && ok; ++k) ok = fgets(line, sizeof line, f) != nullptr;
+  double dens = 0, a0, a1, a2;
+  ok = ok && fscanf(f, "%*s %lf %*s %*s %lf %lf %lf", &dens, &a0, &a1, &a2) == 4;
+  for (int k = 0; k < 6 && ok; ++k) ok = fgets(line, sizeof line, f) != nullptr;
+  for (int i = 0; i < c->N && ok; ++i) {
+    double d;
+    ok = fscanf(f, "%lf %lf %lf %lf", &d, &c->v[0][i], &c->v[1][i], &c->v[2][i]) == 4;
+  }
+  fclose(f);
+  if (ok) c->p.densfluid = dens;
+  return ok;
+}
+
 extern "C" orc_ctx* orc_create(const fhd_params* p) {
   if (!p || p->mx <= 0 || p->my <= 0 || p->mz <= 0) return nullptr;
   orc_ctx* c = new orc_ctx();
@@ -691,6 +712,11 @@ extern "C" orc_ctx* orc_create(const fhd_params* p) {
   for (int a = 0; a < 3; ++a) c->v[a].assign(c->N, p->initfluid == 0 ? p->background[a] : 0.0);
   if (p->scheme == FHD_SCHEME_STOKES_LIMIT)
     for (int a = 0; a < 3; ++a) c->v[a].assign(c->N, 0.0);
+  c->step = p->initial_step;
+  if (p->fluid[0] && !orc_read_fluid(c, p->fluid)) {
+    delete c;
+    return nullptr;
+  }
   const int np = p->np;
   for (int a = 0; a < 3; ++a) { c->q[a].assign(np, 0.0); c->F[a].assign(np, 0.0); c->u[a].assign(np, 0.0); }
   if (np > 0) {

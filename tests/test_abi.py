@@ -94,3 +94,55 @@ def test_comment_lines_are_skipped(tmp_path):
     f.write_text("# a comment incompressible cells 1 1 1\nstokesLimit\n#x\ncells 16 16 16\n")
     p = F.load_datamain(str(f))
     assert p.scheme == F.SCHEME_STOKES_LIMIT and p.mx == 16
+
+
+def _write_reference_layout(path, cells, box, v, dens):
+    """The layout of src/saveFluidFinalConfiguration.cu:36-71, written independently of the
+    package's writer (precision 15, as fluam writes it)."""
+    mx, my, mz = cells
+    N = mx * my * mz
+    lines = [f"Number of cells {N}", "mx my mz", f"{mx} {my} {mz}", "lx ly lz", f"{box[0]} {box[1]} {box[2]}",
+             f"Density {dens}", "Pressure parameters 0 0 0", "Thermostat 0", "Temperature 0.001",
+             "Cells properties ", "Density ", "Velocity"]
+    for i in range(N):
+        lines.append(f"{dens}")
+        lines.append(f"{v[i]:.15g} {v[N + i]:.15g} {v[2 * N + i]:.15g}")
+    path.write_text("\n".join(lines) + "\n")
+
+
+def test_read_fluid_file_reference_layout(tmp_path):
+    import numpy as np
+    rng = np.random.default_rng(3)
+    cells = (4, 6, 5)
+    v = rng.standard_normal(3 * 120)
+    f = tmp_path / "run.fluidFinalConfiguration"
+    _write_reference_layout(f, cells, (4.0, 6.0, 5.0), v, 1.25)
+    got, dens = F.read_fluid_file(f, cells)
+    assert dens == 1.25
+    assert np.allclose(got, v, rtol=1e-14, atol=0)
+    # the package's writer round-trips through the same reader exactly
+    g = tmp_path / "w.fluidFinalConfiguration"
+    F.write_fluid_file(g, cells, (4.0, 6.0, 5.0), v, densfluid=0.75)
+    got2, dens2 = F.read_fluid_file(g, cells)
+    assert dens2 == 0.75 and np.array_equal(got2, v)
+
+
+def test_read_fluid_file_rejects_wrong_grid_and_truncation(tmp_path):
+    import numpy as np
+    v = np.zeros(3 * 64)
+    f = tmp_path / "a.fluidFinalConfiguration"
+    _write_reference_layout(f, (4, 4, 4), (4.0, 4.0, 4.0), v, 1.0)
+    with pytest.raises(F.FhdError, match="grid differs"):
+        F.read_fluid_file(f, (4, 4, 5))
+    text = f.read_text().splitlines()
+    (tmp_path / "b").write_text("\n".join(text[:-10]) + "\n")
+    with pytest.raises(F.FhdError, match="truncated"):
+        F.read_fluid_file(tmp_path / "b", (4, 4, 4))
+
+
+def test_parse_fluid_restart_keywords(tmp_path):
+    f = tmp_path / "d.main"
+    f.write_text("incompressible\ncells 8 8 8\ncelldimension 8 8 8\nfluid saved.fluidFinalConfiguration\n"
+                 "initialStep 42\n")
+    p = F.load_datamain(str(f))
+    assert p.fluid == b"saved.fluidFinalConfiguration" and p.initial_step == 42

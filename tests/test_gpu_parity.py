@@ -378,3 +378,49 @@ def test_overlap_free_random_start_matches_oracle():
     g = F.Simulation(p)
     o = O.Oracle(p)
     assert np.abs(g.particles() - o.particles()).max() < 1e-6
+
+
+
+def test_stokes_resume_is_bit_exact():
+    """A Stokes run's whole state is the particle positions and the step counter: a run
+    restarted at step 10 (positions read back, initialStep 10) continues with the same bits."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=300, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=71, initial_positions=3, **WCA)
+    _announce("stokes_resume", p)
+    full = F.Simulation(p)
+    full.step(20)
+    half = F.Simulation(p)
+    half.step(10)
+    q10 = half.particles()
+    r = p.copy()
+    r.initial_step = 10
+    resumed = F.Simulation(r)
+    resumed.set_particles(q10)
+    resumed.step(10)
+    assert np.array_equal(resumed.particles(), full.particles())
+    assert np.array_equal(resumed.velocity(), full.velocity())
+
+
+def test_incompressible_restart_from_fluid_file(tmp_path):
+    """fluam's `fluid` restart: a run restarted from the saved configuration (and initialStep)
+    matches the oracle restarted from the same file, and the uninterrupted run to round-off
+    (the loaded field is projected once more, as every fresh run projects its start)."""
+    p = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=72)
+    _announce("incompressible_restart", p)
+    full = F.Simulation(p)
+    full.step(10)
+    v10 = full.velocity()
+    f = tmp_path / "r.fluidFinalConfiguration"
+    F.write_fluid_file(f, (32, 32, 32), (p.lx, p.ly, p.lz), v10, densfluid=p.densfluid)
+    r = p.copy()
+    r.fluid = str(f).encode()
+    r.initial_step = 10
+    g = F.Simulation(r)
+    o = O.Oracle(r)
+    assert np.array_equal(g.velocity(), v10)
+    g.step(10)
+    o.step(10)
+    full.step(10)
+    assert rel(g.velocity(), o.velocity()) < 1e-5
+    assert rel(g.velocity(), full.velocity()) < 1e-5
+    assert g.max_divergence() < 1e-6

@@ -269,3 +269,26 @@ def test_qnb_particle_velocity_is_interpolated_fluid():
     assert np.isfinite(q).all()
     u = o.particle_velocities()
     assert np.isfinite(u).all() and np.abs(u).max() > 0
+
+
+
+def test_oracle_restart_reads_fluid_file_and_step(tmp_path):
+    """The oracle's restart path: velocity and density from the file, noise counter from
+    initialStep (a restarted oracle continues the uninterrupted run's noise)."""
+    import fluam_amd as F
+    p = make_params(SCHEME_INCOMPRESSIBLE, cells=(8, 8, 8), temperature=1e-3, dt=0.05, seed=4)
+    a = O.Oracle(p)
+    a.step(3)
+    v3 = a.velocity()
+    f = tmp_path / "r.fluidFinalConfiguration"
+    F.write_fluid_file(f, (8, 8, 8), (p.lx, p.ly, p.lz), v3, densfluid=p.densfluid)
+    q = p.copy()
+    q.fluid = str(f).encode()
+    q.initial_step = 3
+    b = O.Oracle(q)
+    assert np.array_equal(b.velocity(), v3)
+    a.step(2)
+    b.step(2)
+    # the restarted run projects its loaded field once more (as a fresh run does), so it agrees
+    # to round-off rather than bit for bit
+    assert np.abs(a.velocity() - b.velocity()).max() < 1e-12 * max(1.0, np.abs(v3).max())
