@@ -1,7 +1,7 @@
 // fhd_schemes.cpp -- fluam-style scheme entry points on top of the context API: read the global
 // data.main parameters, run numstepsRelaxation + numsteps steps, and write fluam's output files
 // (<out>.data.main, .seed, .particles / .velocityParticles / .velocityParticlesI,
-// .fluidFinalConfiguration, .time).
+// .fluidFinalConfiguration, .temperature, .time).
 //
 // Restates schemeIncompressible (src/schemeIncompressible.cu:25-55), schemeQuasiNeutrallyBuoyant
 // (src/schemeQuasiNeutrallyBuoyant.cu:27-96), schemeStokesLimit (src/schemeStokesLimit.cu:27-98)
@@ -10,6 +10,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <cmath>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -51,6 +52,56 @@ void save_fluid_final(const fhd_params& p, const std::vector<double>& v) {
     f << v[i] << " " << v[N + i] << " " << v[2 * N + i] << "\n";
   }
 }
+
+// Fluid temperature summary written as <out>.temperature by the incompressible and inertial
+// schemes: temperatureFunction (src/temperature.cu:38-212) and the fluid lines of
+// temperatureBoundary (src/temperatureBoundary.cu).  Per sample, t_a = (1/N) sum_i rho dV
+// (v_a,i - <v_a>)^2 (uniform density, so the face density average is rho) and t = mean of t_a.
+struct TemperatureSummary {
+  double t_sum = 0, dev2 = 0, tx = 0, ty = 0, tz = 0;
+  long long n = 0;
+  void sample(const fhd_params& p, const std::vector<double>& v) {
+    const long long N = (long long)p.mx * p.my * p.mz;
+    const double dV = p.lx * p.ly * p.lz / (double)N, rho = p.densfluid;
+    double t[3];
+    for (int a = 0; a < 3; ++a) {
+      double mean = 0.0;
+      for (long long i = 0; i < N; ++i) mean += v[a * N + i];
+      mean /= (double)N;
+      double s = 0.0;
+      for (long long i = 0; i < N; ++i) {
+        const double d = v[a * N + i] - mean;
+        s += rho * dV * d * d;
+      }
+      t[a] = s / (double)N;
+    }
+    const double tt = (t[0] + t[1] + t[2]) / 3.0;
+    dev2 += (tt - p.temperature) * (tt - p.temperature);
+    t_sum += tt;
+    tx += t[0];
+    ty += t[1];
+    tz += t[2];
+    ++n;
+  }
+  void write(const fhd_params& p, const std::vector<double>& v) const {
+    if (n == 0) return;
+    const long long N = (long long)p.mx * p.my * p.mz;
+    const double dV = p.lx * p.ly * p.lz / (double)N, rho = p.densfluid, T = p.temperature;
+    std::ofstream f(out_name(p, ".temperature"));
+    f.precision(15);
+    const double tm = t_sum / (double)n;
+    f << "TEMPERATURE FLUID " << tm << " " << (tm - T) / T << " " << std::sqrt(dev2 / (double)n) / T << "  " << n
+      << "\n";
+    f << "tx                " << tx / (double)n << "\n";
+    f << "ty                " << ty / (double)n << "\n";
+    f << "tz                " << tz / (double)n << "\n";
+    double P[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a)
+      for (long long i = 0; i < N; ++i) P[a] += rho * dV * v[a * N + i];
+    f << "PX " << P[0] << "\nPY " << P[1] << "\nPZ " << P[2] << "\n\n";
+    f << "MASS FLUID    " << rho * dV * (double)N << "\n\n";
+  }
+};
 
 struct ParticleFiles {
   std::ofstream pos, vel, velI;
@@ -103,6 +154,10 @@ bool run_scheme(int scheme, const char* label) {
   const bool parts = p.np > 0;
   if (parts) pf.open(p);
   std::vector<double> q(3 * (size_t)p.np), u(3 * (size_t)p.np);
+  const bool temp = scheme != FHD_SCHEME_STOKES_LIMIT;  // fluam's Stokes scheme writes none
+  TemperatureSummary ts;
+  std::vector<double> vs;
+  if (temp) vs.resize(3 * (size_t)p.mx * p.my * p.mz);
   long long step = -p.numsteps_relaxation;
   bool ok = true;
   while (step < p.numsteps) {
@@ -116,6 +171,10 @@ bool run_scheme(int scheme, const char* label) {
         fhd_get_particle_velocities(c, u.data());
         pf.sample(p, step, q, u);
       }
+      if (temp) {
+        fhd_get_velocity(c, vs.data());
+        ts.sample(p, vs);
+      }
     }
   }
   if (ok && !fhd_sync(c)) ok = fail(fhd_ctx_error(c));
@@ -123,6 +182,7 @@ bool run_scheme(int scheme, const char* label) {
     std::vector<double> v(3 * (size_t)p.mx * p.my * p.mz);
     fhd_get_velocity(c, v.data());
     save_fluid_final(p, v);
+    if (temp) ts.write(p, v);
     long long t = (long long)(time(nullptr) - t0);
     std::cout << "\nTIME RUN_GPU                       " << t << "\n\n";
     std::ofstream f(out_name(p, ".time"));

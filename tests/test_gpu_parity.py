@@ -424,3 +424,75 @@ def test_incompressible_restart_from_fluid_file(tmp_path):
     assert rel(g.velocity(), o.velocity()) < 1e-5
     assert rel(g.velocity(), full.velocity()) < 1e-5
     assert g.max_divergence() < 1e-6
+
+
+
+def _temperature_expected(vs, p):
+    """fluam's temperatureFunction (src/temperature.cu:56-132) on the sampled fields."""
+    N = p.mx * p.my * p.mz
+    dV = p.lx * p.ly * p.lz / N
+    ts = []
+    for v in vs:
+        v = v.reshape(3, N)
+        ta = [(p.densfluid * dV * (v[a] - v[a].mean()) ** 2).sum() / N for a in range(3)]
+        ts.append(ta)
+    ts = np.array(ts)
+    t = ts.mean(axis=1)
+    return t.mean(), np.sqrt(((t - p.temperature) ** 2).mean()), ts.mean(axis=0)
+
+
+def test_scheme_run_writes_fluam_outputs(tmp_path):
+    """The drop-in entry point end to end: a fluam data.main through fhd_main writes the .seed,
+    .data.main, .fluidFinalConfiguration, .temperature and .time files, with the same fields
+    and temperature summary as the context API run of the same parameters."""
+    out = tmp_path / "run"
+    dm = tmp_path / "data.main"
+    dm.write_text("incompressible\ndensfluid 1\nshearviscosity 1\ntemperature 0.001\ncells 16 16 16\n"
+                  "celldimension 16 16 16\ninitfluid 0\nbackgroundvelocity 0 0 0\nseed 7\n"
+                  f"numstepsRelaxation 0\nnumsteps 20\nsamplefreq 5\ndt 0.05\noutputname {out}\n")
+    p = F.load_datamain(str(dm))
+    _announce("scheme_run_incompressible", p)
+    assert F.main(str(dm))
+    for ext in (".seed", ".data.main", ".fluidFinalConfiguration", ".temperature", ".time"):
+        assert (tmp_path / ("run" + ext)).exists(), ext
+    sim = F.Simulation(p)
+    vs = []
+    for _ in range(4):
+        sim.step(5)
+        vs.append(sim.velocity())
+    v_file, dens = F.read_fluid_file(str(out) + ".fluidFinalConfiguration", (16, 16, 16))
+    assert dens == 1.0
+    assert np.abs(v_file - vs[-1]).max() <= 1e-12 * np.abs(vs[-1]).max()
+    lines = open(str(out) + ".temperature").read().split("\n")
+    head = lines[0].split()
+    tm, dev, n = float(head[2]), float(head[4]), int(head[5])
+    t_exp, dev_exp, tabc = _temperature_expected(vs, p)
+    assert n == 4
+    assert abs(tm - t_exp) <= 1e-9 * t_exp
+    assert abs(dev - dev_exp / p.temperature) <= 1e-9 * max(dev, 1e-30)
+    for k in range(3):
+        assert abs(float(lines[1 + k].split()[1]) - tabc[k]) <= 1e-9 * tabc[k]
+
+
+def test_stokes_scheme_run_writes_particle_trajectory(tmp_path):
+    """fhd_main on a Stokes-limit data.main: the .particles trajectory holds the positions the
+    context API reaches at every sample (saveParticles, src/saveParticles.cu:55-73)."""
+    out = tmp_path / "st"
+    dm = tmp_path / "data.main"
+    dm.write_text("stokesLimit\nnumberparticles 50\ncutoff 1.122462048309373\nsigma 1\nkernel 3\n"
+                  "initialPositions 3\nshearviscosity 1\ntemperature 0.001\ncells 16 16 16\n"
+                  f"celldimension 16 16 16\nseed 9\nnumsteps 6\nsamplefreq 3\ndt 0.05\noutputname {out}\n")
+    p = F.load_datamain(str(dm))
+    _announce("scheme_run_stokes", p)
+    assert F.main(str(dm))
+    rows = [ln for ln in open(str(out) + ".particles").read().split("\n")[1:] if ln.strip()]
+    # per sample: the time, then one line per particle
+    assert len(rows) == 2 * (1 + 50)
+    sim = F.Simulation(p)
+    for k in range(2):
+        sim.step(3)
+        q = sim.particles().reshape(3, -1).T
+        got = np.array([[float(x) for x in ln.split()] for ln in rows[k * 51 + 1:(k + 1) * 51]])
+        assert abs(float(rows[k * 51]) - 3 * (k + 1) * p.dt) < 1e-12
+        assert np.abs(got - q).max() < 1e-12 * max(1.0, np.abs(q).max())
+    assert not (tmp_path / "st.temperature").exists()  # fluam's Stokes scheme writes none
