@@ -512,20 +512,18 @@ constexpr int kItemCap = 1024;
 // Accumulator z-row stride (64-bit words): the 8 x 8 plane plus 2, so the rows that the four
 // lanes of one candidate update start in different LDS banks.
 constexpr int kRowStride = kTileEdge * kTileEdge + 2;
-// Guard words before and after the accumulator: a stencil's zero-weight columns address up to
-// 3 (x) + 3 x 8 (y) words before its row and 2 x 8 + 2 + ... words past the last row.
-constexpr int kAccPad = 32;
 
 // Contributions of stencil row kz (nodes base_z + kz) of one source record to the three
 // components of the tile accumulator acc[component][z row][y][x] (x, y: the 8 x 8 tile columns).
-// FAST: the z rows carry a halo of NPTS-1 rows on each side, so the row index needs no test;
-// columns outside the tile are skipped.  Otherwise (small boxes) every node is folded
-// periodically and tested against the tile.
+// Only nodes inside the tile are added: a lane whose row lies outside skips it, and out-of-tile
+// columns are masked off, so no LDS atomic is spent on a node the tile does not own.  FAST: the
+// first node relative to the tile comes from one periodic fold; otherwise (small boxes) every
+// node is folded periodically and tested against the tile.
 template <int NPTS, bool FAST>
 FHD_DEV void add_source_row(unsigned long long* acc, const SrcRec* __restrict__ R, int kz, int ox, int oy, int oz,
                             const GridDev& g, const Tiles& t, float scale) {
   constexpr int H = NPTS - 1;
-  constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
+  constexpr int ZS = kTileEdge;
   constexpr int PL = kRowStride;
   const float4 cx = *reinterpret_cast<const float4*>(R->w[0][0]);
   const float4 cy = *reinterpret_cast<const float4*>(R->w[0][1]);
@@ -546,21 +544,19 @@ FHD_DEV void add_source_row(unsigned long long* acc, const SrcRec* __restrict__ 
       dx += dx < -H ? g.mx : (dx >= g.mx - H ? -g.mx : 0);
       dy += dy < -H ? g.my : (dy >= g.my - H ? -g.my : 0);
       dz += dz < -H ? g.mz : (dz >= g.mz - H ? -g.mz : 0);
-      if (dx >= kTileEdge || dy >= kTileEdge || dz >= kTileEdge) continue;  // misses in this staggering
-      // columns outside the tile get zero weight: their (zero) additions land on other nodes of
-      // the padded accumulator and change nothing, so no node needs a test
-      float zx[4], zy[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        zx[k] = (unsigned)(dx + k) < (unsigned)kTileEdge ? wx[k] : 0.0f;
-        zy[k] = (unsigned)(dy + k) < (unsigned)kTileEdge ? wy[k] : 0.0f;
-      }
-      unsigned long long* p = acc + kAccPad + (C * ZS + dz + kz + H) * PL + dy * kTileEdge + dx;
+      const int lz = dz + kz;
+      if ((unsigned)lz >= (unsigned)kTileEdge) continue;  // this row misses the tile
+      const int rb = (C * ZS + lz) * PL;
 #pragma unroll
       for (int ky = 0; ky < NPTS; ++ky) {
-        const float wyz = zy[ky] * wzs;
+        const float wyz = wy[ky] * wzs;
+        const int ly = dy + ky;
 #pragma unroll
-        for (int kx = 0; kx < NPTS; ++kx) atomicAdd(&p[ky * kTileEdge + kx], fixed64(zx[kx] * wyz));
+        for (int kx = 0; kx < NPTS; ++kx) {
+          const int lx = dx + kx;
+          if ((unsigned)ly < (unsigned)kTileEdge && (unsigned)lx < (unsigned)kTileEdge)
+            atomicAdd(&acc[rb + ly * kTileEdge + lx], fixed64(wx[kx] * wyz));
+        }
       }
     } else {
       const int lz = fold(bz + kz - oz, g.mz);
@@ -593,12 +589,10 @@ FHD_DEV const SrcRec* rec_of(const SpreadArgs& a, int ref) {
 // the lanes' atomics land in.  FAST: tiles of 8 x 8 x 8 cells and at least two tiles per axis.
 template <int NPTS, bool FAST>
 FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
-  constexpr int H = NPTS - 1;
-  constexpr int ZS = FAST ? kTileEdge + 2 * H : kTileEdge;
+  constexpr int ZS = kTileEdge;
   constexpr int NACC = 3 * ZS * kRowStride;
   constexpr int LPC = 4;  // lanes per candidate (one stencil z-row each)
-  __shared__ unsigned long long acc_buf[NACC + 2 * kAccPad];
-  unsigned long long* acc = acc_buf;  // the node arrays start at acc + kAccPad (FAST) or acc
+  __shared__ unsigned long long acc[NACC];
   __shared__ int nb_start[64], nb_pref[65], nb_sel[64];
   __shared__ int items[kItemCap];
   __shared__ int nitem;
@@ -621,7 +615,7 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       pre[k] = a.out_add[(size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz))];
     }
   }
-  for (int e = tid; e < NACC + 2 * kAccPad; e += kSpreadThreads) acc[e] = 0ull;
+  for (int e = tid; e < NACC; e += kSpreadThreads) acc[e] = 0ull;
   // neighbour-tile table: lane k of wave 0 loads the record range of (group, neighbour) k
   int lox, hix, loy, hiy, loz, hiz;
   nb_range(t.ntx, lox, hix);
@@ -725,7 +719,7 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
     for (int k = 0; k < kOutPer; ++k) {
       const int e = tid + k * kSpreadThreads;
       const int C = e >> 9, lx = e & 7, ly = (e >> 3) & 7, lz = (e >> 6) & 7;
-      const int ai = kAccPad + (C * ZS + lz + H) * kRowStride + ly * kTileEdge + lx;
+      const int ai = (C * ZS + lz) * kRowStride + ly * kTileEdge + lx;
       const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
       const float val = (float)(long long)acc[ai] * inv;
       a.out[gi] = a.out_add ? pre[k] + val : val;
@@ -746,7 +740,7 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       ly = (l / t.tx) % t.ty;
       lz = l / (t.tx * t.ty);
     }
-    const int ai = (FAST ? kAccPad : 0) + (C * ZS + lz + (FAST ? H : 0)) * kRowStride + ly * kTileEdge + lx;
+    const int ai = (C * ZS + lz) * kRowStride + ly * kTileEdge + lx;
     const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
     const float val = (float)(long long)acc[ai] * inv;
     a.out[gi] = a.out_add ? a.out_add[gi] + val : val;
