@@ -144,6 +144,19 @@ def test_stokes_parity_no_wca(kernel):
     _particle_parity(p, 50)
 
 
+@pytest.mark.parametrize("cells,scheme,kernel", [((12, 24, 8), SCHEME_STOKES_LIMIT, 4),
+                                                  ((8, 16, 12), SCHEME_QUASI_NEUTRALLY_BUOYANT, 3),
+                                                  ((36, 8, 16), SCHEME_STOKES_LIMIT, 3)])
+def test_small_box_tiles_parity(cells, scheme, kernel):
+    """Boxes whose tiles are not 8^3 with at least two per axis (tile edge 4 along 12- and
+    36-cell axes, a single tile along an 8-cell axis) take the general spreading path, the
+    folded node indexing and the small-box WCA cull; trajectories match the oracle."""
+    p = make_params(scheme, cells=cells, np=60, kernel_points=kernel, temperature=1e-3, dt=0.05,
+                    seed=41 + kernel, initial_positions=3, **WCA)
+    _announce(f"small_box_{cells}", p)
+    _particle_parity(p, 20)
+
+
 def test_stokes_parity_with_wca_lattice():
     p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=512, kernel_points=4, temperature=1e-3,
                     dt=0.05, seed=22, initial_positions=1, **WCA)
