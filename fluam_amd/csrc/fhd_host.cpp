@@ -558,8 +558,11 @@ void step_stokes(fhd_ctx* c) {
     stokes_noise_join(c);
   }
   fft_forward(c, c->W);
-  // solve 2's noise: W = sqrt(1/2) D.(Sigma_1 + Sigma_2), overlapping the rest of solve 1
-  stokes_noise_begin(c, 2, (float)(sqrt(0.5) * f1), (float)(sqrt(0.5) * f4));
+  // solve 2's noise: W = sqrt(1/2) D.(Sigma_1 + Sigma_2).  With particles it starts after the
+  // first interpolation, beside the latency-bound sort / WCA / record kernels rather than the
+  // FFT passes (measured 5 us per step faster); without particles it starts at once.
+  const float g1 = (float)(sqrt(0.5) * f1), g4 = (float)(sqrt(0.5) * f4);
+  if (c->np == 0) stokes_noise_begin(c, 2, g1, g4);
   fft_solve(c, 1, 0.0f);
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
   if (c->np > 0) {
@@ -578,6 +581,7 @@ void step_stokes(fhd_ctx* c) {
     }
     launch_interp(ia, c->s);
     mark(c, "interp");
+    stokes_noise_begin(c, 2, g1, g4);
     sort_set(c, 1);
     wca(c, 1, c->F[1]);
     SpreadArgs sa = spread_base(c);
