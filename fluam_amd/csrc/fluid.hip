@@ -2,9 +2,10 @@
 // family) with the stochastic stress generated in registers.
 //
 // One workgroup owns a 16 x 16 column of cells and marches up a chunk of z-planes.  The 6
-// stress normals of every cell of plane z+1 (plus the one-cell ring the divergence needs) are
-// drawn from the counter-based generator into an LDS ring of three planes, so each normal is
-// drawn ~1.25 times instead of once per neighbour that reads it.
+// stress normals of every cell of plane z+2 (plus the one-cell ring the divergence needs) are
+// drawn from the counter-based generator into an LDS ring of four planes while plane z's
+// stencil reads planes z-1..z+1, so each normal is drawn ~1.25 times instead of once per
+// neighbour that reads it.
 //
 // Restates: kernelConstructW_1/_2 (src/kernelConstructW.cu:32-521), the QNB variants
 // (src/kernelConstructWQuasiNeutrallyBuoyant.cu:26-705,1366-2025) and
@@ -63,7 +64,10 @@ FHD_DEV void gen_plane(const FluidWArgs& a, const Key& key, float* slot, int x0,
 }
 
 FHD_BODY void body_fluid_w(const FluidWArgs& a, const Blk& b) {
-  __shared__ float nbuf[3][6 * PLANE];
+  // ring of four noise planes: while the stencil of plane z reads planes z - 1, z and z + 1,
+  // plane z + 2 is drawn into the fourth slot, so one barrier per plane suffices
+  constexpr int NR = 4;
+  __shared__ float nbuf[NR][6 * PLANE];
   const GridDev& g = a.g;
   const Key key = resolve_key(a.key);
   const int x0 = b.x * TX, y0 = b.y * TY;
@@ -74,22 +78,23 @@ FHD_BODY void body_fluid_w(const FluidWArgs& a, const Blk& b) {
   const bool own = (x < g.mx) && (y < g.my);
   const int N = g.N;
   if (a.noise) {
-    gen_plane(a, key, nbuf[(zs - 1 + 3) % 3], x0, y0, zs - 1);
-    gen_plane(a, key, nbuf[zs % 3], x0, y0, zs);
+    gen_plane(a, key, nbuf[(zs - 1 + NR) % NR], x0, y0, zs - 1);
+    gen_plane(a, key, nbuf[zs % NR], x0, y0, zs);
+    gen_plane(a, key, nbuf[(zs + 1) % NR], x0, y0, zs + 1);
   }
+  __syncthreads();
   const int xm = wrapi(x - 1, g.mx), xp = wrapi(x + 1, g.mx);
   const int ym = wrapi(y - 1, g.my), yp = wrapi(y + 1, g.my);
   for (int z = zs; z < ze; ++z) {
-    if (a.noise) gen_plane(a, key, nbuf[(z + 1) % 3], x0, y0, z + 1);
-    __syncthreads();
+    if (a.noise && z + 2 <= ze) gen_plane(a, key, nbuf[(z + 2) % NR], x0, y0, z + 2);
     if (own) {
       const int zm = wrapi(z - 1, g.mz), zp = wrapi(z + 1, g.mz);
       const int i = x + g.mx * (y + g.my * z);
       float wx = 0.0f, wy = 0.0f, wz = 0.0f;
       if (a.noise) {
-        const float* cur = nbuf[z % 3];
-        const float* prv = nbuf[(z + 2) % 3];
-        const float* nxt = nbuf[(z + 1) % 3];
+        const float* cur = nbuf[z % NR];
+        const float* prv = nbuf[(z - 1 + NR) % NR];
+        const float* nxt = nbuf[(z + 1) % NR];
         const int p = (lx + 1) + PX * (ly + 1);
         const float f1 = a.f1, f4 = a.f4;
         wx = g.idx * f1 * (cur[0 * PLANE + p + 1] - cur[0 * PLANE + p]) +
