@@ -428,6 +428,12 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
   if (G.mode == SRC_VALUE) {
     const float4 v = G.val[id];
     val = make_float3(v.x * G.coef, v.y * G.coef, v.z * G.coef);
+    if (val.x == 0.0f && val.y == 0.0f && val.z == 0.0f) {
+      // a zero source (e.g. a particle with no WCA neighbour) adds exactly zero to every node:
+      // an empty reach mask makes every tile cull it, and its record is never read
+      G.cull[r] = make_int4(0, 0, 0, 0);
+      return;
+    }
   } else if (G.mode == SRC_DRIFT) {
     val = make_float3(G.coef * g.idx, G.coef * g.idy, G.coef * g.idz);
   } else {
