@@ -62,6 +62,7 @@ def lib():
     L.fhd_step.argtypes = [vp, C.c_int]
     L.fhd_sync.argtypes = [vp]
     L.fhd_set_graph.argtypes = [vp, C.c_int]
+    L.fhd_set_noise_overlap.argtypes = [vp, C.c_int]
     L.fhd_bytes_needed.argtypes = [C.POINTER(FhdParams)]
     L.fhd_bytes_needed.restype = C.c_longlong
     L.fhd_mem_info.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
@@ -209,6 +210,11 @@ class Simulation:
     def set_graph(self, on=True):
         """Replay the steady-state step from captured HIP graphs (identical results)."""
         self._check(lib().fhd_set_graph(self.h, 1 if on else 0))
+
+    def set_noise_overlap(self, on=True):
+        """Stokes limit: build the noise on a second stream while the particle kernels run (the
+        default) or on the context's own stream (identical results)."""
+        self._check(lib().fhd_set_noise_overlap(self.h, 1 if on else 0))
 
     def sync(self):
         self._check(lib().fhd_sync(self.h))

@@ -45,6 +45,7 @@ class FhdParams(C.Structure):
         ("coordinates", C.c_char * 512),
         ("fluid", C.c_char * 512),
         ("initial_step", C.c_longlong),
+        ("bonded_forces", C.c_char * 512),
     ]
 
     def copy(self):
@@ -94,7 +95,7 @@ def make_params(scheme=SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), box=None, **kw
         if k == "background":
             for a in range(3):
                 p.background[a] = v[a]
-        elif k in ("outputname", "coordinates", "fluid"):
+        elif k in ("outputname", "coordinates", "fluid", "bonded_forces"):
             setattr(p, k, v.encode() if isinstance(v, str) else v)
         else:
             if not hasattr(p, k):

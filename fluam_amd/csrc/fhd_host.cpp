@@ -114,6 +114,10 @@ struct fhd_ctx {
   SrcRec* rec[2] = {nullptr, nullptr};   // spreading source records per group (2 np each)
   int4* cull[2] = {nullptr, nullptr};
   int* errd = nullptr;
+  // harmonic bonds (bondedForces); null when the run has none
+  int *bpp_start = nullptr, *bfp_start = nullptr;
+  int4 *bpp = nullptr, *bfpc = nullptr;
+  float4* bfpo = nullptr;
   long long step = 0;
   bool projected = false;
   bool qnb_started = false;
@@ -137,6 +141,7 @@ struct fhd_ctx {
   // Stokes: the stochastic part of W is built on a side stream while the particle kernels run
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_noise = nullptr;
+  bool noise_overlap = true;             // fhd_set_noise_overlap
 };
 
 namespace {
@@ -262,7 +267,7 @@ void free_ctx(fhd_ctx* c) {
                   c->ex, c->ey, c->ez, c->gx, c->gy, c->gz, c->cell[0], c->cell[1], c->off[0], c->off[1],
                   c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
                   c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
-                  c->cull[0], c->cull[1]};
+                  c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -422,9 +427,27 @@ SortedSet sset(fhd_ctx* c, int k) {
   s.start = c->start[k];
   return s;
 }
+// Harmonic bond forces added to F (position set k), after the WCA pass wrote it.
+void bonded(fhd_ctx* c, int k, float4* F) {
+  if (!c->bpp) return;
+  BondArgs a;
+  a.g = c->g;
+  a.cell = c->cell[k];
+  a.off = c->off[k];
+  a.pp_start = c->bpp_start;
+  a.pp = c->bpp;
+  a.fp_start = c->bfp_start;
+  a.fpc = c->bfpc;
+  a.fpo = c->bfpo;
+  a.F = F;
+  a.np = c->np;
+  launch_bonded(a, c->s);
+  mark(c, "bonded");
+}
 void wca(fhd_ctx* c, int k, float4* F) {
   if (!c->p.nonbonded || c->p.cutoff <= 0) {
     launch_zero_f4(F, c->np, c->s);
+    bonded(c, k, F);
     return;
   }
   WcaArgs a;
@@ -438,6 +461,7 @@ void wca(fhd_ctx* c, int k, float4* F) {
   a.np = c->np;
   launch_wca(a, c->s);
   mark(c, "wca");
+  bonded(c, k, F);
 }
 // Attach the per-group record buffers, then prepare the source records and spread.
 void run_spread(fhd_ctx* c, SpreadArgs& sa) {
@@ -513,8 +537,9 @@ void step_incompressible(fhd_ctx* c) {
 // (the same single float addition as a combined pass, so the same bits).  W is free for the
 // next noise pass once the xy-forward FFT pass has read it.  When profiling, everything runs
 // on the main stream so per-kernel times stay separable.
+bool noise_on_side(const fhd_ctx* c) { return !c->profiling && c->noise_overlap; }
 void stokes_noise_begin(fhd_ctx* c, int noise, float f1, float f4) {
-  const bool side = !c->profiling;
+  const bool side = noise_on_side(c);
   if (side) {
     HIPCHK(hipEventRecord(c->ev_fork, c->s));
     HIPCHK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
@@ -529,7 +554,7 @@ void stokes_noise_begin(fhd_ctx* c, int noise, float f1, float f4) {
   if (side) HIPCHK(hipEventRecord(c->ev_noise, c->s2));
 }
 void stokes_noise_join(fhd_ctx* c) {
-  if (!c->profiling) HIPCHK(hipStreamWaitEvent(c->s, c->ev_noise, 0));
+  if (noise_on_side(c)) HIPCHK(hipStreamWaitEvent(c->s, c->ev_noise, 0));
 }
 
 void step_stokes(fhd_ctx* c) {
@@ -852,7 +877,7 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
                                "quasiNeutrallyBuoyant4pt2D", "quasi2D", "stokesLimit2D", "particlesWall",
                                "freeEnergyCompressibleParticles", "semiImplicitCompressibleParticles",
                                "momentumCoupling", "stokesLimitFirstOrder", "incompressibleBinaryMixture",
-                               "incompressibleBinaryMixtureMidPoint", "ghost", "boundary", "bondedForces"};
+                               "incompressibleBinaryMixtureMidPoint", "ghost", "boundary"};
   try {
     while (f >> word) {
       if (word[0] == '#') {
@@ -894,6 +919,12 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
       else if (word == "velocities") { std::string s; need(bool(f >> s)); }
       else if (word == "computeNonBondedForces") need(bool(f >> p.nonbonded));
       else if (word == "extraMobility") need(bool(f >> p.extra_mobility));
+      else if (word == "bondedForces") { std::string s; need(bool(f >> s)); snprintf(p.bonded_forces, sizeof(p.bonded_forces), "%s", s.c_str()); }
+      else if (word == "bondedForcesVersion") {
+        int v;
+        need(bool(f >> v));
+        if (v) throw std::runtime_error("bondedForcesVersion 1 (the older bond-file layout) is not supported");
+      }
       // extension keywords
       else if (word == "kernel") need(bool(f >> p.kernel_points));
       else if (word == "sigma") need(bool(f >> p.wca_sigma));
@@ -997,6 +1028,72 @@ bool read_coordinates(const fhd_params& p, std::vector<double>& q, std::string& 
       err = "coordinates file truncated";
       return false;
     }
+  return true;
+}
+
+// Bonds file (bin/inputfiles/README:55-108): the number of particle-particle bonds, one line
+// "i j k r0" each, then the number of particle-fixed-point bonds, one line "i k r0 x y z" each.
+// Each particle-particle bond is listed under both particles, in file order, as
+// src/initializeBondedForces.cpp:110-160 builds its per-particle rows.
+struct HostBonds {
+  std::vector<int> pp_start, fp_start;
+  std::vector<int4> pp, fpc;
+  std::vector<float4> fpo;
+  bool any() const { return !pp.empty() || !fpc.empty(); }
+};
+bool read_bonds(const fhd_params& p, double h[3], HostBonds& b, std::string& err) {
+  std::ifstream f(p.bonded_forces);
+  if (!f) {
+    err = std::string("cannot open bonds file ") + p.bonded_forces;
+    return false;
+  }
+  const int np = p.np;
+  auto bad = [&](const char* what) {
+    err = std::string("bonds file ") + p.bonded_forces + ": " + what;
+    return false;
+  };
+  long long npp = -1, nfp = -1;
+  if (!(f >> npp) || npp < 0 || npp > (1LL << 28)) return bad("bad particle-particle bond count");
+  std::vector<int> pi(npp), pj(npp);
+  std::vector<double> pk(npp), pr(npp);
+  for (long long e = 0; e < npp; ++e) {
+    if (!(f >> pi[e] >> pj[e] >> pk[e] >> pr[e])) return bad("particle-particle bonds truncated");
+    if (pi[e] < 0 || pi[e] >= np || pj[e] < 0 || pj[e] >= np) return bad("bond index outside 0..numberparticles-1");
+  }
+  if (!(f >> nfp) || nfp < 0 || nfp > (1LL << 28)) return bad("bad fixed-point bond count");
+  std::vector<int> fi(nfp);
+  std::vector<double> fk(nfp), fr(nfp), fx(3 * nfp);
+  for (long long e = 0; e < nfp; ++e) {
+    if (!(f >> fi[e] >> fk[e] >> fr[e] >> fx[3 * e] >> fx[3 * e + 1] >> fx[3 * e + 2]))
+      return bad("fixed-point bonds truncated");
+    if (fi[e] < 0 || fi[e] >= np) return bad("bond index outside 0..numberparticles-1");
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(fx[3 * e + a])) return bad("non-finite fixed point");
+  }
+  b.pp_start.assign(np + 1, 0);
+  b.fp_start.assign(np + 1, 0);
+  for (long long e = 0; e < npp; ++e) { b.pp_start[pi[e] + 1]++; b.pp_start[pj[e] + 1]++; }
+  for (long long e = 0; e < nfp; ++e) b.fp_start[fi[e] + 1]++;
+  for (int i = 0; i < np; ++i) { b.pp_start[i + 1] += b.pp_start[i]; b.fp_start[i + 1] += b.fp_start[i]; }
+  b.pp.assign(2 * npp, make_int4(0, 0, 0, 0));
+  b.fpc.assign(nfp, make_int4(0, 0, 0, 0));
+  b.fpo.assign(nfp, make_float4(0, 0, 0, 0));
+  std::vector<int> fill(b.pp_start.begin(), b.pp_start.end() - 1);
+  auto fbits = [](double v) { float x = (float)v; int r; memcpy(&r, &x, 4); return r; };
+  for (long long e = 0; e < npp; ++e) {
+    b.pp[fill[pi[e]]++] = make_int4(pj[e], fbits(pk[e]), fbits(pr[e]), 0);
+    b.pp[fill[pj[e]]++] = make_int4(pi[e], fbits(pk[e]), fbits(pr[e]), 0);
+  }
+  fill.assign(b.fp_start.begin(), b.fp_start.end() - 1);
+  const double L[3] = {p.lx, p.ly, p.lz};
+  for (long long e = 0; e < nfp; ++e) {
+    int cc[3];
+    float d[3];
+    for (int a = 0; a < 3; ++a) to_cell(fx[3 * e + a], L[a], h[a], cc[a], d[a]);
+    const int slot = fill[fi[e]]++;
+    b.fpc[slot] = make_int4(cc[0], cc[1], cc[2], fbits(fk[e]));
+    b.fpo[slot] = make_float4(d[0], d[1], d[2], (float)fr[e]);
+  }
   return true;
 }
 
@@ -1242,6 +1339,29 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
         initial_positions(p, q);
       }
       if (!fhd_set_particles(c, q.data())) throw std::runtime_error(c->err);
+      if (p.bonded_forces[0]) {
+        HostBonds hb;
+        std::string err;
+        double h[3] = {c->dx, c->dy, c->dz};
+        if (!read_bonds(p, h, hb, err)) throw std::runtime_error(err);
+        if (hb.any()) {
+          c->bpp_start = dalloc<int>(c->np + 1);
+          c->bfp_start = dalloc<int>(c->np + 1);
+          HIPCHK(hipMemcpy(c->bpp_start, hb.pp_start.data(), (c->np + 1) * sizeof(int), hipMemcpyHostToDevice));
+          HIPCHK(hipMemcpy(c->bfp_start, hb.fp_start.data(), (c->np + 1) * sizeof(int), hipMemcpyHostToDevice));
+          c->bpp = dalloc<int4>(std::max<size_t>(hb.pp.size(), 1));
+          c->bfpc = dalloc<int4>(std::max<size_t>(hb.fpc.size(), 1));
+          c->bfpo = dalloc<float4>(std::max<size_t>(hb.fpo.size(), 1));
+          if (!hb.pp.empty())
+            HIPCHK(hipMemcpy(c->bpp, hb.pp.data(), hb.pp.size() * sizeof(int4), hipMemcpyHostToDevice));
+          if (!hb.fpc.empty()) {
+            HIPCHK(hipMemcpy(c->bfpc, hb.fpc.data(), hb.fpc.size() * sizeof(int4), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(c->bfpo, hb.fpo.data(), hb.fpo.size() * sizeof(float4), hipMemcpyHostToDevice));
+          }
+        }
+      }
+    } else if (p.bonded_forces[0]) {
+      throw std::runtime_error("bondedForces needs particles");
     }
     HIPCHK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
@@ -1373,6 +1493,24 @@ int fhd_step(fhd_ctx* c, int nsteps) {
 
 int fhd_set_graph(fhd_ctx* c, int on) {
   CTX_TRY(c, { c->graph_mode = on != 0; });
+  return 1;
+}
+
+int fhd_set_noise_overlap(fhd_ctx* c, int on) {
+  CTX_TRY(c, {
+    const bool v = on != 0;
+    if (v != c->noise_overlap) {
+      // captured steps hold the old stream layout: drop them, the next graph step recaptures
+      HIPCHK(hipStreamSynchronize(c->s));
+      for (auto& e : c->gexec)
+        if (e) {
+          HIPCHK(hipGraphExecDestroy(e));
+          e = nullptr;
+        }
+      c->graphs_built = false;
+      c->noise_overlap = v;
+    }
+  });
   return 1;
 }
 

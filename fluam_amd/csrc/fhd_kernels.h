@@ -160,6 +160,23 @@ struct WcaArgs {
   int np;
 };
 
+// Harmonic bonds (V = k/2 (r - r0)^2) of each particle, in compressed rows by particle id:
+// particle i's particle-particle bonds are pp[pp_start[i] .. pp_start[i+1]) and its fixed-point
+// bonds fp*[fp_start[i] .. fp_start[i+1]).  A fixed point is stored like a position, as
+// (cell, offset), so r_i - x is formed from exact cell differences.
+struct BondArgs {
+  GridDev g;
+  const int4* cell;     // positions by id
+  const float4* off;
+  const int* pp_start;  // np + 1
+  const int4* pp;       // {other particle, k (float bits), r0 (float bits), 0}
+  const int* fp_start;  // np + 1
+  const int4* fpc;      // {cell x, y, z, k (float bits)}
+  const float4* fpo;    // {offset x, y, z, r0}
+  float4* F;            // by id; the bond forces are added
+  int np;
+};
+
 // Size-specialised FFT passes (fft_fast.hip); return false when the shape has no specialisation.
 bool fast_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s);
 bool fast_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s);
@@ -179,6 +196,7 @@ void launch_advance_step(unsigned long long* step, hipStream_t s);
 
 void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per-tile ordering
 void launch_wca(const WcaArgs& a, hipStream_t s);
+void launch_bonded(const BondArgs& a, hipStream_t s);  // F += bond forces (after the WCA pass)
 void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
 void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation
 void launch_interp(const InterpArgs& a, hipStream_t s);

@@ -349,6 +349,50 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
 }
 FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
 
+// Harmonic bonds, one lane per particle (src/forceBondedGPU.cu:21-121): for each bond partner
+// (another particle, then fixed points, each in file order) F_i += -k (1 - r0/r) (r_i - r_j),
+// with -k (r_i - r_j) when r0 = 0 and nothing when r = 0 and r0 != 0.  As in fluam the
+// separation is the plain difference of the unwrapped positions, with no periodic image.
+constexpr int kBondThreads = 256;
+FHD_DEV void bond_term(float rx, float ry, float rz, float k, float r0, float& fx, float& fy, float& fz) {
+  float s = k;
+  if (r0 != 0.0f) {
+    const float r = sqrtf(rx * rx + ry * ry + rz * rz);
+    s = r > 0.0f ? k * (1.0f - r0 / r) : 0.0f;
+  }
+  fx -= s * rx;
+  fy -= s * ry;
+  fz -= s * rz;
+}
+FHD_BODY void body_bonded(const BondArgs& a, const Blk& b) {
+  const int i = b.x * kBondThreads + (int)threadIdx.x;
+  if (i >= a.np) return;
+  const int4 ci = a.cell[i];
+  const float4 oi = a.off[i];
+  float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+  const int pe = a.pp_start[i + 1];
+  for (int e = a.pp_start[i]; e < pe; ++e) {
+    const int4 bd = a.pp[e];
+    const int4 cj = a.cell[bd.x];
+    const float4 oj = a.off[bd.x];
+    bond_term((float)(ci.x - cj.x) * a.g.dx + (oi.x - oj.x), (float)(ci.y - cj.y) * a.g.dy + (oi.y - oj.y),
+              (float)(ci.z - cj.z) * a.g.dz + (oi.z - oj.z), __int_as_float(bd.y), __int_as_float(bd.z), fx, fy, fz);
+  }
+  const int fe = a.fp_start[i + 1];
+  for (int e = a.fp_start[i]; e < fe; ++e) {
+    const int4 cx = a.fpc[e];
+    const float4 ox = a.fpo[e];
+    bond_term((float)(ci.x - cx.x) * a.g.dx + (oi.x - ox.x), (float)(ci.y - cx.y) * a.g.dy + (oi.y - ox.y),
+              (float)(ci.z - cx.z) * a.g.dz + (oi.z - ox.z), __int_as_float(cx.w), ox.w, fx, fy, fz);
+  }
+  float4 f = a.F[i];
+  f.x += fx;
+  f.y += fy;
+  f.z += fz;
+  a.F[i] = f;
+}
+FHD_KERNEL_PAIR((), kBondThreads, k_bonded, BondArgs, body_bonded(a, b))
+
 // Weights of one axis at an explicit node base (used for the reference-quirk RFD variant):
 // node k sits at integer position base + k in node units; the particle at c + frac - stag/2.
 template <int NPTS>
@@ -886,6 +930,9 @@ void launch_sort(const SortArgs& a, hipStream_t s) {
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n), kWcaThreads, a, s);
+}
+void launch_bonded(const BondArgs& a, hipStream_t s) {
+  if (a.np > 0) launch(k_bonded, k_bonded_b, FAM_BONDED, dim3((a.np + kBondThreads - 1) / kBondThreads), kBondThreads, a, s);
 }
 void launch_src_prep(const SpreadArgs& a, hipStream_t s) {
   int nrec[2] = {0, 0};

@@ -78,6 +78,9 @@ class Server:
                 j = pending.pop(0)
                 j._sim = Simulation(j.params)
                 j._sim.set_graph(self.graphs)
+                # the admitted jobs already fill the GPU: a second stream per job only adds
+                # cross-stream waits
+                j._sim.set_noise_overlap(False)
                 j.admitted = time.perf_counter() - t0
                 used += j._bytes
                 active.append(j)

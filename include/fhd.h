@@ -81,6 +81,8 @@ typedef struct fhd_params {
                                  file (src/loadDataMain.cu:443-445, src/initializeFluid.cpp:168-192) */
   long long initial_step;     /* initialStep (extension): step counter of the first step, so a run
                                  restarted from its saved state draws the same noise */
+  char bonded_forces[512];    /* bondedForces: harmonic bonds file (format of bin/inputfiles/README:55-108,
+                                 read as src/initializeBondedForces.cpp:29-160); empty = no bonds */
 } fhd_params;
 
 /* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
@@ -89,11 +91,11 @@ void fhd_default_params(fhd_params* p);
 unsigned long fhd_params_size(void);
 
 /* ---------------- global (fluam-style) API ---------------- */
-int fhd_load_datamain(const char* path);
+int fhd_load_datamain(const char* path);          /* parse data.main into the global params */
 /* Reads a .fluidFinalConfiguration file (the format of src/saveFluidFinalConfiguration.cu:30-75,
  * read as src/initializeFluid.cpp:168-192 does) for an mx x my x mz grid: v[3N] component-major,
  * x fastest; *densfluid receives the file's density.  CPU only; 1 on success. */
-int fhd_read_fluid_file(const char* path, int mx, int my, int mz, double* v, double* densfluid);          /* parse data.main into the global params */
+int fhd_read_fluid_file(const char* path, int mx, int my, int mz, double* v, double* densfluid);
 int fhd_get_params(fhd_params* out);
 int fhd_set_params(const fhd_params* in);
 int fhd_scheme_incompressible(void);
@@ -118,6 +120,11 @@ int fhd_sync(fhd_ctx* c);                                /* wait, then check dev
 /* Replay the steady-state step from captured HIP graphs (on != 0) instead of launching each
  * kernel.  No fluam counterpart (fluam launches every kernel each step); results are identical. */
 int fhd_set_graph(fhd_ctx* c, int on);
+/* Stokes limit: build the noise part of W on a second stream while the particle kernels run
+ * (on != 0, the default), or on the context's own stream.  No fluam counterpart (fluam runs one
+ * stream); results are identical either way.  Callers that keep many contexts busy at once (a
+ * serving loop) already fill the GPU and gain from turning it off. */
+int fhd_set_noise_overlap(fhd_ctx* c, int on);
 long long fhd_get_step(const fhd_ctx* c);
 /* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
  * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names

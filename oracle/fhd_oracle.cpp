@@ -244,6 +244,11 @@ struct orc_ctx {
   bool first_step_done = false;
   int64_t step = 0;
   bool projected = false;
+  // harmonic bonds (bondedForces file), in file order
+  struct PairBond { int i, j; double k, r0; };
+  struct FixedBond { int i; double k, r0, x[3]; };
+  std::vector<PairBond> pair_bonds;
+  std::vector<FixedBond> fixed_bonds;
 };
 
 static inline int wrapi(int i, int m) { i %= m; return i < 0 ? i + m : i; }
@@ -582,6 +587,63 @@ static void wca_forces(const orc_ctx* c, const std::vector<double>* q, std::vect
   }
 }
 
+// Harmonic bond forces added to F (src/forceBondedGPU.cu:21-121): for a bond of stiffness k
+// and rest length r0 between positions a and b, F_a += -k (1 - r0/|r|) r with r = a - b the
+// plain difference (no periodic image), -k r when r0 = 0, and nothing when r = 0 and r0 != 0.
+// Particle-particle bonds act on both ends (src/initializeBondedForces.cpp:126-137).
+static void bond_term(const double r[3], double k, double r0, double* f) {
+  double s = k;
+  if (r0 != 0.0) {
+    const double d = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    s = d > 0.0 ? k * (1.0 - r0 / d) : 0.0;
+  }
+  for (int a = 0; a < 3; ++a) f[a] -= s * r[a];
+}
+static void bond_forces(const orc_ctx* c, const std::vector<double>* q, std::vector<double>* F) {
+  for (const auto& b : c->pair_bonds) {
+    double r[3], fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) r[a] = q[a][b.i] - q[a][b.j];
+    bond_term(r, b.k, b.r0, fi);
+    for (int a = 0; a < 3; ++a) r[a] = -r[a];
+    bond_term(r, b.k, b.r0, fj);
+    for (int a = 0; a < 3; ++a) { F[a][b.i] += fi[a]; F[a][b.j] += fj[a]; }
+  }
+  for (const auto& b : c->fixed_bonds) {
+    double r[3], f[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a) r[a] = q[a][b.i] - b.x[a];
+    bond_term(r, b.k, b.r0, f);
+    for (int a = 0; a < 3; ++a) F[a][b.i] += f[a];
+  }
+}
+// All particle forces: WCA plus bonds.
+static void particle_forces(const orc_ctx* c, const std::vector<double>* q, std::vector<double>* F) {
+  wca_forces(c, q, F);
+  bond_forces(c, q, F);
+}
+
+// Bonds file (bin/inputfiles/README:55-108): count, then "i j k r0" lines; count, then
+// "i k r0 x y z" lines.
+static bool orc_read_bonds(orc_ctx* c, const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  const int np = c->p.np;
+  long long n = -1;
+  bool ok = fscanf(f, "%lld", &n) == 1 && n >= 0;
+  for (long long e = 0; ok && e < n; ++e) {
+    orc_ctx::PairBond b;
+    ok = fscanf(f, "%d %d %lf %lf", &b.i, &b.j, &b.k, &b.r0) == 4 && b.i >= 0 && b.i < np && b.j >= 0 && b.j < np;
+    if (ok) c->pair_bonds.push_back(b);
+  }
+  ok = ok && fscanf(f, "%lld", &n) == 1 && n >= 0;
+  for (long long e = 0; ok && e < n; ++e) {
+    orc_ctx::FixedBond b;
+    ok = fscanf(f, "%d %lf %lf %lf %lf %lf", &b.i, &b.k, &b.r0, &b.x[0], &b.x[1], &b.x[2]) == 6 && b.i >= 0 && b.i < np;
+    if (ok) c->fixed_bonds.push_back(b);
+  }
+  fclose(f);
+  return ok;
+}
+
 // ------------------------------------------------------------------------------------
 // Public API.
 // ------------------------------------------------------------------------------------
@@ -725,6 +787,10 @@ extern "C" orc_ctx* orc_create(const fhd_params* p) {
     for (int a = 0; a < 3; ++a)
       for (int i = 0; i < np; ++i) c->q[a][i] = q0[(size_t)a * np + i];
   }
+  if (p->bonded_forces[0] && !orc_read_bonds(c, p->bonded_forces)) {
+    delete c;
+    return nullptr;
+  }
   return c;
 }
 
@@ -767,6 +833,18 @@ extern "C" int orc_wca(orc_ctx* c, const double* qin, double* f) {
   std::vector<double> q[3], F[3];
   for (int a = 0; a < 3; ++a) q[a].assign(qin + (size_t)a * c->p.np, qin + (size_t)(a + 1) * c->p.np);
   wca_forces(c, q, F);
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < c->p.np; ++i) f[(size_t)a * c->p.np + i] = F[a][i];
+  return 1;
+}
+
+extern "C" int orc_bonded(orc_ctx* c, const double* qin, double* f) {
+  std::vector<double> q[3], F[3];
+  for (int a = 0; a < 3; ++a) {
+    q[a].assign(qin + (size_t)a * c->p.np, qin + (size_t)(a + 1) * c->p.np);
+    F[a].assign(c->p.np, 0.0);
+  }
+  bond_forces(c, q, F);
   for (int a = 0; a < 3; ++a)
     for (int i = 0; i < c->p.np; ++i) f[(size_t)a * c->p.np + i] = F[a][i];
   return 1;
@@ -901,7 +979,7 @@ static void step_stokes(orc_ctx* c) {
   }
   // 1. F^n and S F^n
   std::vector<double> f[3], W[3], v[3];
-  wca_forces(c, c->q, c->F);
+  particle_forces(c, c->q, c->F);
   for (int a = 0; a < 3; ++a) f[a].assign(N, 0.0);
   for (int i = 0; i < np; ++i) {
     double qp[3] = {c->q[0][i], c->q[1][i], c->q[2][i]};
@@ -925,7 +1003,7 @@ static void step_stokes(orc_ctx* c) {
     }
   }
   // 4. F^{n+1/2}; f = S(q^{n+1/2}) F^{n+1/2} + (kT/delta)[S(q+) W - S(q-) W]
-  wca_forces(c, qh, Fh);
+  particle_forces(c, qh, Fh);
   for (int a = 0; a < 3; ++a) f[a].assign(N, 0.0);
   for (int i = 0; i < np; ++i) {
     double qp[3] = {qh[0][i], qh[1][i], qh[2][i]};
@@ -956,7 +1034,7 @@ static void step_stokes(orc_ctx* c) {
 // src/firstStepQuasiNeutrallyBuoyant.cu:35-1152 ----
 static void qnb_forcing(orc_ctx* c, const std::vector<double>* qh, std::vector<double>* f) {
   const int N = c->N, np = c->p.np;
-  wca_forces(c, qh, c->F);
+  particle_forces(c, qh, c->F);
   for (int a = 0; a < 3; ++a) f[a].assign(N, 0.0);
   // m_f / (m_e + m_f) with m_f = 1.5 dV volumeParticle rho (src/quasiNeutrallyBuoyantFunctions.cu:6394)
   const double mf = 1.5 * c->dV * c->p.volume_particle * c->p.densfluid;

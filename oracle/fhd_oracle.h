@@ -16,6 +16,8 @@
  *   orc_fft3d             numpy.fft golden fixtures (tests/golden) + brute-force DFT
  *   orc_delta3/orc_delta4 closed-form values and Peskin moment conditions
  *   orc_wca               analytic WCA (zero at 2^(1/6) sigma), Newton's third law
+ *   orc_bonded            closed-form traps and springs, F = -grad V by finite differences of
+ *                         V = sum k/2 (r - r0)^2, Newton's third law
  *   orc_spread/orc_interp direct triple loops; adjointness <J v, F> = dV <v, S F>
  *   incompressible step   dense-matrix brute force (I - dt nu L/2)^-1 P on a 4^3 grid,
  *                         divergence, momentum conservation, equal-time S(k)
@@ -60,6 +62,7 @@ int orc_set_threads(int n);
 
 /* Building blocks on the context's grid / particles. */
 int orc_wca(orc_ctx* c, const double* q, double* f);               /* q,f: 3*np */
+int orc_bonded(orc_ctx* c, const double* q, double* f);            /* harmonic bonds only, f = bond forces */
 int orc_spread(orc_ctx* c, const double* q, const double* F, double* field); /* field += S F (3*N) */
 int orc_interp(orc_ctx* c, const double* q, const double* field, double* u);  /* u = J field */
 int orc_divergence(orc_ctx* c, const double* v, double* div);     /* div: N */

@@ -26,8 +26,9 @@ def lib():
         for name in ("orc_set_velocity", "orc_get_velocity", "orc_set_particles", "orc_get_particles",
                      "orc_get_particle_velocities", "orc_get_forces", "orc_divergence"):
             getattr(L, name).argtypes = [C.c_void_p, dp]
-        for name in ("orc_solve_incompressible", "orc_solve_stokes", "orc_wca", "orc_interp"):
-            getattr(L, name).argtypes = [C.c_void_p, dp, dp] if name in ("orc_solve_incompressible", "orc_solve_stokes", "orc_wca") else [C.c_void_p, dp, dp, dp]
+        for name in ("orc_solve_incompressible", "orc_solve_stokes", "orc_wca", "orc_bonded"):
+            getattr(L, name).argtypes = [C.c_void_p, dp, dp]
+        L.orc_interp.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_spread.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_step.restype = C.c_longlong
@@ -143,6 +144,12 @@ class Oracle:
         q = np.ascontiguousarray(q, dtype=np.float64)
         f = np.zeros_like(q)
         lib().orc_wca(self.h, _p(q), _p(f))
+        return f
+
+    def bonded(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        f = np.zeros_like(q)
+        lib().orc_bonded(self.h, _p(q), _p(f))
         return f
 
     def spread(self, q, F, field=None):

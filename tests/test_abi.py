@@ -146,3 +146,18 @@ def test_parse_fluid_restart_keywords(tmp_path):
                  "initialStep 42\n")
     p = F.load_datamain(str(f))
     assert p.fluid == b"saved.fluidFinalConfiguration" and p.initial_step == 42
+
+
+def test_parse_bonded_forces_keywords(tmp_path):
+    f = tmp_path / "d.main"
+    f.write_text("stokesLimit\ncells 8 8 8\ncelldimension 8 8 8\nnumberparticles 2\n"
+                 "bondedForces springs.dat\nbondedForcesVersion 0\n")
+    p = F.load_datamain(str(f))
+    assert p.bonded_forces == b"springs.dat"
+
+
+def test_old_bond_file_layout_is_reported(tmp_path):
+    f = tmp_path / "d.main"
+    f.write_text("stokesLimit\ncells 8 8 8\nbondedForcesVersion 1\n")
+    with pytest.raises(F.FhdError, match="bondedForcesVersion"):
+        F.load_datamain(str(f))

@@ -509,3 +509,126 @@ def test_stokes_scheme_run_writes_particle_trajectory(tmp_path):
         assert abs(float(rows[k * 51]) - 3 * (k + 1) * p.dt) < 1e-12
         assert np.abs(got - q).max() < 1e-12 * max(1.0, np.abs(q).max())
     assert not (tmp_path / "st.temperature").exists()  # fluam's Stokes scheme writes none
+
+
+# ---------------- harmonic bonds ----------------
+def _chains(tmp_path, name, nchain=8, per=8, spacing=1.1, r0=1.0, k=1.0, trap_k=0.5):
+    """nchain chains of `per` particles along x, starting near the +x face of a 32-cell box so
+    the chains run through the periodic boundary (unwrapped), bonded neighbour to neighbour,
+    and the first particle of each chain trapped half a cell off its start."""
+    pts, pairs, fixed = [], [], []
+    for c in range(nchain):
+        y, z = -12.0 + 6.5 * (c % 4), -9.0 + 11.0 * (c // 4)
+        for m in range(per):
+            i = len(pts)
+            pts.append((11.3 + spacing * m, y, z))
+            if m:
+                pairs.append((i - 1, i, k, r0))
+        fixed.append((c * per, trap_k, 0.0, (11.8, y + 0.25, z)))
+    path = tmp_path / name
+    lines = [str(len(pairs))] + [f"{i} {j} {kk} {rr}" for i, j, kk, rr in pairs]
+    lines += [str(len(fixed))] + [f"{i} {kk} {rr} {x} {y} {z}" for i, kk, rr, (x, y, z) in fixed]
+    path.write_text("\n".join(lines) + "\n")
+    return np.asarray(pts).T.ravel(), str(path)
+
+
+@pytest.mark.parametrize("scheme", [SCHEME_STOKES_LIMIT, SCHEME_QUASI_NEUTRALLY_BUOYANT])
+def test_bonded_chains_parity(tmp_path, scheme):
+    """Bonded chains (springs with rest length, fixed-point traps, bonds across the periodic
+    boundary) with WCA: forces and 20-step trajectories match the oracle."""
+    q, path = _chains(tmp_path, "bonds.dat")
+    p = make_params(scheme, cells=(32, 32, 32), np=64, kernel_points=4, temperature=1e-3, dt=0.05, seed=51,
+                    initial_positions=0, bonded_forces=path, **WCA)
+    _announce(f"bonded_chains_s{scheme}", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.set_particles(q)
+    o.set_particles(q)
+    g.step(1)
+    o.step(1)
+    fg, fo = g.forces(), o.forces()
+    assert np.abs(fo).max() > 0.05
+    assert rel(fg, fo) < 1e-4
+    g.step(19)
+    g.sync()
+    o.step(19)
+    qg, qo = g.particles(), o.particles()
+    assert np.isfinite(qg).all()
+    assert rel(qg, qo) < 1e-5
+    assert rel(qg - q, qo - q) < 1e-3
+    assert g.max_divergence() < 1e-6
+
+
+def test_bonds_without_wca_and_bad_file(tmp_path):
+    """Bond forces alone (computeNonBondedForces 0) match the oracle; a bond naming a particle
+    outside 0..np-1 is an error at creation."""
+    q, path = _chains(tmp_path, "bonds.dat", nchain=4, per=6)
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=24, kernel_points=3, nonbonded=0,
+                    temperature=1e-3, dt=0.05, seed=52, initial_positions=0, bonded_forces=path, **WCA)
+    _announce("bonded_no_wca", p)
+    g = F.Simulation(p)
+    o = O.Oracle(p)
+    g.set_particles(q)
+    o.set_particles(q)
+    g.step(10)
+    g.sync()
+    o.step(10)
+    assert rel(g.forces(), o.forces()) < 1e-4
+    assert rel(g.particles(), o.particles()) < 1e-5
+    bad = tmp_path / "bad.dat"
+    bad.write_text("1\n0 24 1.0 0.0\n0\n")
+    pb = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=24, kernel_points=3, initial_positions=1,
+                     bonded_forces=str(bad), **WCA)
+    with pytest.raises(F.FhdError, match="bond index"):
+        F.Simulation(pb)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_batch_with_bonds_equals_alone(tmp_path, mode):
+    """Replicas with and without bonds in one batch give the same bits as each run alone."""
+    qa, pa = _chains(tmp_path, "a.dat", nchain=4, per=8)
+    qb, pb = _chains(tmp_path, "b.dat", nchain=8, per=4, r0=0.9, k=2.0)
+    params = [make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=32, kernel_points=4, temperature=1e-3,
+                          dt=0.05, seed=61, initial_positions=0, bonded_forces=pa, **WCA),
+              make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(32, 32, 32), np=32, kernel_points=3,
+                          temperature=1e-3, dt=0.05, seed=62, initial_positions=0, bonded_forces=pb, **WCA),
+              _replica_params(3, (16, 24, 32), 100)]
+    qs = [qa, qb, None]
+    for p in params:
+        _announce("batch_bonded_replica", p)
+    b = F.Batch(params)
+    for r, q in zip(b.replicas, qs):
+        if q is not None:
+            r.set_particles(q)
+    b.set_mode(mode)
+    b.step(6)
+    b.sync()
+    for p, r, q in zip(params, b.replicas, qs):
+        alone = F.Simulation(p)
+        if q is not None:
+            alone.set_particles(q)
+        alone.step(6)
+        assert np.array_equal(alone.velocity(), r.velocity())
+        assert np.array_equal(alone.particles(), r.particles())
+    b.close()
+
+
+def test_noise_overlap_switch_changes_no_bits():
+    """The Stokes noise built on the context's own stream instead of the second stream, switched
+    mid-run and under graph replay, gives the same bits as the default."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(32, 32, 32), np=400, kernel_points=4, temperature=1e-3,
+                    dt=0.05, seed=71, initial_positions=1, **WCA)
+    _announce("noise_overlap_switch", p)
+    a = F.Simulation(p)
+    b = F.Simulation(p)
+    b.set_graph(True)
+    a.step(3)
+    b.step(3)
+    b.set_noise_overlap(False)
+    a.step(4)
+    b.step(4)
+    b.set_noise_overlap(True)
+    a.step(2)
+    b.step(2)
+    assert np.array_equal(a.particles(), b.particles())
+    assert np.array_equal(a.velocity(), b.velocity())

@@ -27,6 +27,9 @@ class FakeSim:
     def set_graph(self, on):
         pass
 
+    def set_noise_overlap(self, on):
+        self.overlap = on
+
     def step(self, n=1):
         self.t_ready = max(self.t_ready, time.perf_counter()) + n * FakeSim.dt
         self.steps += n
