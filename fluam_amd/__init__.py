@@ -63,6 +63,9 @@ def lib():
     L.fhd_sync.argtypes = [vp]
     L.fhd_set_graph.argtypes = [vp, C.c_int]
     L.fhd_set_noise_overlap.argtypes = [vp, C.c_int]
+    L.fhd_sk_reset.argtypes = [vp]
+    L.fhd_sk_sample.argtypes = [vp]
+    L.fhd_sk_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.fhd_bytes_needed.argtypes = [C.POINTER(FhdParams)]
     L.fhd_bytes_needed.restype = C.c_longlong
     L.fhd_mem_info.argtypes = [C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)]
@@ -210,6 +213,23 @@ class Simulation:
     def set_graph(self, on=True):
         """Replay the steady-state step from captured HIP graphs (identical results)."""
         self._check(lib().fhd_set_graph(self.h, 1 if on else 0))
+
+    def sk_reset(self):
+        """Clear the structure-factor accumulator."""
+        self._check(lib().fhd_sk_reset(self.h))
+
+    def sk_sample(self):
+        """Add the current velocity's |v_hat_a(k)|^2 to the structure-factor accumulator (GPU)."""
+        self._check(lib().fhd_sk_sample(self.h))
+
+    def structure_factor(self):
+        """(S, samples): S[a, z, ky, kx] = rho dV / kT <|v_hat_a(k)|^2> / N over the half
+        spectrum (kx = 0..mx/2); the diagonal of the discrete projector at equilibrium."""
+        p = self.p
+        s = np.zeros((3, p.mz, p.my, p.mx // 2 + 1))
+        n = C.c_longlong(0)
+        self._check(lib().fhd_sk_read(self.h, s.ctypes.data_as(C.POINTER(C.c_double)), C.byref(n)))
+        return s, n.value
 
     def set_noise_overlap(self, on=True):
         """Stokes limit: build the noise on a second stream while the particle kernels run (the

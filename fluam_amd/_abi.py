@@ -46,6 +46,7 @@ class FhdParams(C.Structure):
         ("fluid", C.c_char * 512),
         ("initial_step", C.c_longlong),
         ("bonded_forces", C.c_char * 512),
+        ("structure_factor", C.c_int),
     ]
 
     def copy(self):

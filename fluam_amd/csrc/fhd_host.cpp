@@ -142,6 +142,9 @@ struct fhd_ctx {
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_noise = nullptr;
   bool noise_overlap = true;             // fhd_set_noise_overlap
+  // equal-time structure factor: sum of |v_hat_a(k)|^2 over the samples, layout of spec
+  double* sk_acc = nullptr;
+  long long sk_samples = 0;
 };
 
 namespace {
@@ -267,7 +270,7 @@ void free_ctx(fhd_ctx* c) {
                   c->ex, c->ey, c->ez, c->gx, c->gy, c->gz, c->cell[0], c->cell[1], c->off[0], c->off[1],
                   c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
                   c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
-                  c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo};
+                  c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo, c->sk_acc};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -932,6 +935,7 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
       else if (word == "rfdMode") need(bool(f >> p.rfd_mode));
       else if (word == "rfdDelta") need(bool(f >> p.rfd_delta));
       else if (word == "initialPositions") need(bool(f >> p.initial_positions));
+      else if (word == "structureFactor") need(bool(f >> p.structure_factor));
       else handled = false;
       if (!handled) {
         for (const char* u : unsupported)
@@ -1493,6 +1497,52 @@ int fhd_step(fhd_ctx* c, int nsteps) {
 
 int fhd_set_graph(fhd_ctx* c, int on) {
   CTX_TRY(c, { c->graph_mode = on != 0; });
+  return 1;
+}
+
+// Equal-time structure factor of the velocity (fluam's HydroGrid hook,
+// src/hydroAnalysisIncompressible.cu:35-92, computed in place on the GPU).
+int fhd_sk_reset(fhd_ctx* c) {
+  CTX_TRY(c, {
+    const size_t n = 3 * (size_t)c->p.mz * c->p.my * c->g.P;
+    if (!c->sk_acc) c->sk_acc = dalloc<double>(n);
+    HIPCHK(hipMemsetAsync(c->sk_acc, 0, n * sizeof(double), c->s));
+    c->sk_samples = 0;
+  });
+  return 1;
+}
+
+int fhd_sk_sample(fhd_ctx* c) {
+  if (c && !c->sk_acc && !fhd_sk_reset(c)) return 0;
+  CTX_TRY(c, {
+    // the spectrum buffer is free between steps; the transform is the step's own forward pass
+    fft_forward(c, c->v);
+    fft_ztransform(c, 1);
+    launch_sk_accum(c->spec, c->sk_acc, 3 * (size_t)c->p.mz * c->p.my * c->g.P, c->s);
+    HIPCHK(hipGetLastError());
+    c->sk_samples++;
+  });
+  return 1;
+}
+
+int fhd_sk_read(fhd_ctx* c, double* s, long long* nsamples) {
+  CTX_TRY(c, {
+    const int P = c->g.P, my = c->p.my, mz = c->p.mz;
+    const size_t per = (size_t)mz * my * P, n = 3 * per;
+    std::vector<double> h(n, 0.0);
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->sk_acc) HIPCHK(hipMemcpy(h.data(), c->sk_acc, n * sizeof(double), hipMemcpyDeviceToHost));
+    const double kT = c->p.temperature;
+    const double mult = kT != 0.0 ? c->p.densfluid * c->dV / kT : 1.0;
+    const double norm = c->sk_samples > 0 ? mult / ((double)c->g.N * (double)c->sk_samples) : 0.0;
+    // device [a][ky][z][kx] -> caller [a][z][ky][kx]
+    for (int a = 0; a < 3; ++a)
+      for (int z = 0; z < mz; ++z)
+        for (int ky = 0; ky < my; ++ky)
+          for (int kx = 0; kx < P; ++kx)
+            s[a * per + ((size_t)z * my + ky) * P + kx] = norm * h[a * per + ((size_t)ky * mz + z) * P + kx];
+    if (nsamples) *nsamples = c->sk_samples;
+  });
   return 1;
 }
 

@@ -193,6 +193,7 @@ void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t
 void launch_normals(const Key& key, int stream, int sub, int n, float* out, hipStream_t s);
 void launch_zero_f(float* p, size_t n, hipStream_t s);
 void launch_advance_step(unsigned long long* step, hipStream_t s);
+void launch_sk_accum(const float2* spec, double* acc, size_t n, hipStream_t s);  // acc += |spec|^2
 
 void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per-tile ordering
 void launch_wca(const WcaArgs& a, hipStream_t s);

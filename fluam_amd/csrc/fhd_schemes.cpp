@@ -1,7 +1,7 @@
 // fhd_schemes.cpp -- fluam-style scheme entry points on top of the context API: read the global
 // data.main parameters, run numstepsRelaxation + numsteps steps, and write fluam's output files
 // (<out>.data.main, .seed, .particles / .velocityParticles / .velocityParticlesI,
-// .fluidFinalConfiguration, .temperature, .time).
+// .fluidFinalConfiguration, .temperature, .time; .structureFactor when asked for).
 //
 // Restates schemeIncompressible (src/schemeIncompressible.cu:25-55), schemeQuasiNeutrallyBuoyant
 // (src/schemeQuasiNeutrallyBuoyant.cu:27-96), schemeStokesLimit (src/schemeStokesLimit.cu:27-98)
@@ -132,6 +132,31 @@ struct ParticleFiles {
   }
 };
 
+// <out>.structureFactor: the equal-time structure factor sampled every samplefreq steps (the
+// quantity fluam's HydroGrid hook accumulates, src/hydroAnalysisIncompressible.cu:35-92), over
+// the half spectrum: signed mode numbers n, wavenumbers k = 2 pi n / L, then S_x, S_y, S_z.
+void save_structure_factor(const fhd_params& p, fhd_ctx* c) {
+  const int P = p.mx / 2 + 1;
+  const size_t per = (size_t)p.mz * p.my * P;
+  std::vector<double> sk(3 * per);
+  long long ns = 0;
+  if (!fhd_sk_read(c, sk.data(), &ns)) return;
+  std::ofstream f(out_name(p, ".structureFactor"));
+  f.precision(10);
+  f << "# equal-time structure factor S_a(k) = rho dV / kT <|v_a(k)|^2> / N, samples " << ns << "\n";
+  f << "# nx ny nz kx ky kz S_x S_y S_z\n";
+  const double two_pi = 6.283185307179586;
+  auto sgn = [](int n, int m) { return n <= m / 2 ? n : n - m; };
+  for (int z = 0; z < p.mz; ++z)
+    for (int y = 0; y < p.my; ++y)
+      for (int x = 0; x < P; ++x) {
+        const int nx = x, ny = sgn(y, p.my), nz = sgn(z, p.mz);
+        const size_t i = ((size_t)z * p.my + y) * P + x;
+        f << nx << " " << ny << " " << nz << " " << two_pi * nx / p.lx << " " << two_pi * ny / p.ly << " "
+          << two_pi * nz / p.lz << " " << sk[i] << " " << sk[per + i] << " " << sk[2 * per + i] << "\n";
+      }
+}
+
 bool run_scheme(int scheme, const char* label) {
   fhd_params p;
   if (!fhd_get_params(&p)) return fail("no data.main loaded");
@@ -175,6 +200,7 @@ bool run_scheme(int scheme, const char* label) {
         fhd_get_velocity(c, vs.data());
         ts.sample(p, vs);
       }
+      if (p.structure_factor && !fhd_sk_sample(c)) { ok = fail(fhd_ctx_error(c)); break; }
     }
   }
   if (ok && !fhd_sync(c)) ok = fail(fhd_ctx_error(c));
@@ -183,6 +209,7 @@ bool run_scheme(int scheme, const char* label) {
     fhd_get_velocity(c, v.data());
     save_fluid_final(p, v);
     if (temp) ts.write(p, v);
+    if (p.structure_factor) save_structure_factor(p, c);
     long long t = (long long)(time(nullptr) - t0);
     std::cout << "\nTIME RUN_GPU                       " << t << "\n\n";
     std::ofstream f(out_name(p, ".time"));

@@ -230,4 +230,23 @@ void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t
   hipLaunchKernelGGL(k_divergence, dim3((g.N + 255) / 256), dim3(256), 0, s, g, v, div);
 }
 
+// Equal-time structure factor sample: acc[i] += |spec[i]|^2 in FP64 over the half spectrum of
+// the three velocity components (the in-situ counterpart of the HydroGrid static structure
+// factor that fluam's analysis hook feeds, src/hydroAnalysisIncompressible.cu:35-92).  The
+// phase of the staggering shift drops out of |.|^2, so the plain transform of each component
+// is used.  HBM-bound: 8 B read + 16 B read-modify-write per spectral entry.
+__global__ __launch_bounds__(256) void k_sk_accum(const float2* __restrict__ spec, double* __restrict__ acc,
+                                                 long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const float2 v = spec[i];
+    acc[i] += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+  }
+}
+void launch_sk_accum(const float2* spec, double* acc, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  long long blocks = ((long long)n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_sk_accum, dim3((unsigned)blocks), dim3(256), 0, s, spec, acc, (long long)n);
+}
+
 }  // namespace fhd

@@ -83,6 +83,10 @@ typedef struct fhd_params {
                                  restarted from its saved state draws the same noise */
   char bonded_forces[512];    /* bondedForces: harmonic bonds file (format of bin/inputfiles/README:55-108,
                                  read as src/initializeBondedForces.cpp:29-160); empty = no bonds */
+  int structure_factor;       /* extension keyword `structureFactor 1`: the scheme run samples the
+                                 equal-time structure factor every samplefreq steps and writes
+                                 <outputname>.structureFactor (fluam's HydroGrid hook,
+                                 src/hydroAnalysisIncompressible.cu:35-92) */
 } fhd_params;
 
 /* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
@@ -125,6 +129,17 @@ int fhd_set_graph(fhd_ctx* c, int on);
  * stream); results are identical either way.  Callers that keep many contexts busy at once (a
  * serving loop) already fill the GPU and gain from turning it off. */
 int fhd_set_noise_overlap(fhd_ctx* c, int on);
+
+/* Equal-time structure factor of the velocity, computed on the GPU from the current field
+ * (replaces the HydroGrid library fluam hands its velocities to, src/hydroAnalysisIncompressible.cu:35-92).
+ * fhd_sk_sample adds |v_hat_a(k)|^2 (v_hat: unnormalised DFT) of the current velocity to an FP64
+ * accumulator; fhd_sk_read writes S_a(k) = rho dV / kT * <|v_hat_a(k)|^2> / N (multiplier 1 when
+ * kT = 0) over the half spectrum, s[3][mz][my][mx/2+1] with kx fastest, and the sample count.  At
+ * equilibrium S_a(k) is the diagonal of the discrete projector, 1 - g_a^2/|g|^2.  fhd_sk_reset
+ * clears the accumulator. */
+int fhd_sk_reset(fhd_ctx* c);
+int fhd_sk_sample(fhd_ctx* c);
+int fhd_sk_read(fhd_ctx* c, double* s, long long* nsamples);
 long long fhd_get_step(const fhd_ctx* c);
 /* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
  * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names

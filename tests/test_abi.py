@@ -161,3 +161,9 @@ def test_old_bond_file_layout_is_reported(tmp_path):
     f.write_text("stokesLimit\ncells 8 8 8\nbondedForcesVersion 1\n")
     with pytest.raises(F.FhdError, match="bondedForcesVersion"):
         F.load_datamain(str(f))
+
+
+def test_parse_structure_factor_keyword(tmp_path):
+    f = tmp_path / "d.main"
+    f.write_text("incompressible\ncells 8 8 8\nstructureFactor 1\n")
+    assert F.load_datamain(str(f)).structure_factor == 1

@@ -632,3 +632,105 @@ def test_noise_overlap_switch_changes_no_bits():
     b.step(2)
     assert np.array_equal(a.particles(), b.particles())
     assert np.array_equal(a.velocity(), b.velocity())
+
+
+# ---------------- equal-time structure factor (in situ) ----------------
+def _sk_numpy(v, p):
+    """rho dV / kT |fftn(v_a)|^2 / N over the half spectrum, layout [a][z][ky][kx]."""
+    m = (p.mx, p.my, p.mz)
+    N = p.mx * p.my * p.mz
+    dV = (p.lx / p.mx) * (p.ly / p.my) * (p.lz / p.mz)
+    v = v.reshape(3, p.mz, p.my, p.mx)
+    return np.stack([np.abs(np.fft.rfftn(v[a])) ** 2 for a in range(3)]) * (p.densfluid * dV / p.temperature) / N
+
+
+@pytest.mark.parametrize("scheme,np_", [(SCHEME_INCOMPRESSIBLE, 0), (SCHEME_STOKES_LIMIT, 200)])
+def test_structure_factor_in_situ_matches_numpy(scheme, np_):
+    """The GPU accumulator over 5 snapshots equals numpy's transform of the same snapshots
+    (FP32 transform, FP64 accumulation: 1e-5 relative), on a non-cubic box with dV != 1."""
+    kw = dict(np=np_, kernel_points=4, initial_positions=1, **WCA) if np_ else {}
+    p = make_params(scheme, cells=(16, 24, 32), box=(8.0, 18.0, 40.0), temperature=2e-3, dt=0.05, seed=81,
+                    densfluid=1.5, **kw)
+    _announce(f"sk_in_situ_s{scheme}", p)
+    g = F.Simulation(p)
+    g.step(10)
+    ref = 0.0
+    g.sk_reset()
+    for _ in range(5):
+        g.step(3)
+        g.sk_sample()
+        ref = ref + _sk_numpy(g.velocity(), p)
+    s, n = g.structure_factor()
+    assert n == 5
+    assert rel(s, ref / 5) < 1e-5
+    g.sk_reset()
+    s0, n0 = g.structure_factor()
+    assert n0 == 0 and not s0.any()
+
+
+def test_structure_factor_in_situ_equilibrium():
+    """At equilibrium S_a(k) is the diagonal of the discrete projector, P_aa = 1 - g_a^2/|g|^2
+    with g_a = (2/dx_a) sin(pi n_a/m_a): the in-situ accumulator over four replicas and 120
+    snapshots each gives sum_a S_a = 2 to 1% (mode average), and per component the mean of S_a
+    over the modes with P_aa < 1/4 and over those with P_aa > 3/4 within 0.03 of the mean P_aa
+    there (this separates the components: a swapped axis fails it)."""
+    m = (32, 16, 24)
+    acc, ns = 0.0, 0
+    for seed in (91, 92, 93, 94):
+        p = make_params(SCHEME_INCOMPRESSIBLE, cells=m, box=(32.0, 24.0, 24.0), temperature=1e-3, dt=0.5,
+                        seed=seed)
+        _announce(f"sk_equilibrium_seed{seed}", p)
+        g = F.Simulation(p)
+        g.step(100)
+        for _ in range(120):
+            g.step(5)
+            g.sk_sample()
+        s, n = g.structure_factor()
+        acc = acc + s * n
+        ns += n
+    S = acc / ns
+    nx = np.arange(m[0] // 2 + 1)[None, None, :]
+    ny = np.fft.fftfreq(m[1], 1.0 / m[1])[None, :, None]
+    nz = np.fft.fftfreq(m[2], 1.0 / m[2])[:, None, None]
+    h = (32.0 / m[0], 24.0 / m[1], 24.0 / m[2])
+    g2 = [(2 / h[0] * np.sin(np.pi * nx / m[0])) ** 2 + 0 * ny + 0 * nz,
+          (2 / h[1] * np.sin(np.pi * ny / m[1])) ** 2 + 0 * nx + 0 * nz,
+          (2 / h[2] * np.sin(np.pi * nz / m[2])) ** 2 + 0 * nx + 0 * ny]
+    G = g2[0] + g2[1] + g2[2]
+    nonzero = G > 0
+    tr = S.sum(axis=0)[nonzero]
+    assert abs(tr.mean() - 2.0) < 0.02
+    for a in range(3):
+        P = 1 - g2[a][nonzero] / G[nonzero]
+        Sa = S[a][nonzero]
+        for sel in (P < 0.25, P > 0.75):
+            assert sel.sum() > 100
+            assert abs(Sa[sel].mean() - P[sel].mean()) < 0.03, (a, Sa[sel].mean(), P[sel].mean())
+
+
+def test_scheme_run_writes_structure_factor(tmp_path):
+    """structureFactor 1 in data.main: the run samples every samplefreq steps and writes
+    <outputname>.structureFactor, equal to the context API's accumulator on the same run."""
+    out = tmp_path / "run"
+    dm = tmp_path / "data.main"
+    dm.write_text("incompressible\ndensfluid 1\nshearviscosity 1\ntemperature 0.001\ncells 16 16 8\n"
+                  "celldimension 16 16 8\nseed 9\nnumstepsRelaxation 0\nnumsteps 20\nsamplefreq 5\ndt 0.05\n"
+                  f"structureFactor 1\noutputname {out}\n")
+    p = F.load_datamain(str(dm))
+    assert p.structure_factor == 1
+    _announce("scheme_run_structure_factor", p)
+    assert F.main(str(dm))
+    lines = open(str(out) + ".structureFactor").read().split("\n")
+    assert lines[0].split()[-1] == "4"
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines[2:] if ln.strip()])
+    assert rows.shape == (8 * 16 * 9, 9)
+    sim = F.Simulation(p)
+    for _ in range(4):
+        sim.step(5)
+        sim.sk_sample()
+    s, n = sim.structure_factor()
+    assert n == 4
+    np.testing.assert_allclose(rows[:, 6:9], s.reshape(3, -1).T, rtol=1e-8, atol=1e-12)
+    i = 1 + 9 * 1 + 9 * 16 * 7  # nx 1, ny 1, nz -1 (row z = 7)
+    assert rows[i, :3].tolist() == [1, 1, -1]
+    np.testing.assert_allclose(rows[i, 3:6], [2 * np.pi / 16, 2 * np.pi / 16, -2 * np.pi / 8])
