@@ -252,15 +252,29 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
 FHD_KERNEL_PAIR((template <int MZ, int NT, int TM>), NT, k_fftB, FftPass, body_fftB<MZ, NT, TM>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------
+template <int MX, int MY, int NT>
+static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
+  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(mz, 3), NT, x, s);
+  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(mz, 3), NT, x, s);
+}
+// Lanes per plane workgroup.  A single context's pass has only 3 mz plane workgroups, and more
+// lanes per plane cover more of the plane load and store latency: 1024 from 64^2 up (128^3:
+// 25.4 vs 26.1 / 27.3 us per pass with 512; 58-60 VGPRs, two workgroups per CU by LDS).  A
+// batched dispatch already holds the planes of many contexts, and below 128^2 those pack
+// better as 512-lane workgroups (ensemble step 2.30 vs 2.41 ms).
 template <int MX, int MY>
 static bool launch_AC(const FftArgs& a, int which, const float* in, float2* spec, float* out, const float* aux,
                       float alpha, float beta, hipStream_t s) {
-  // 128^2 planes: 1024 lanes (58-60 VGPRs, two workgroups per CU by LDS) hide more of the plane
-  // load and store latency than 512 (25.4 vs 26.1 / 27.3 us per pass at 128^3)
-  constexpr int NT = MX * MY >= 128 * 128 ? 1024 : MX * MY >= 64 * 64 ? 512 : 256;
   const FftPass x{a, in, spec, out, aux, alpha, beta, 0};
-  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), NT, x, s);
-  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(a.g.mz, 3), NT, x, s);
+  if constexpr (MX * MY >= 64 * 64) {
+    if (MX * MY >= 128 * 128 || t_recorder == nullptr) {
+      launch_AC_nt<MX, MY, 1024>(x, which, a.g.mz, s);
+      return true;
+    }
+    launch_AC_nt<MX, MY, 512>(x, which, a.g.mz, s);
+  } else {
+    launch_AC_nt<MX, MY, 256>(x, which, a.g.mz, s);
+  }
   return true;
 }
 
@@ -309,6 +323,13 @@ static int column_tile(int P, int tmax) {
   return 1;
 }
 
+template <int MZ, int NT>
+static void launch_B_nt(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
+  if (bt == 4) launch(k_fftB<MZ, NT, 5>, k_fftB_b<MZ, NT, 5>, FAM_FFT_Z, grid, NT, x, s);
+  else if (bt == 8) launch(k_fftB<MZ, NT, 9>, k_fftB_b<MZ, NT, 9>, FAM_FFT_Z, grid, NT, x, s);
+  else launch(k_fftB<MZ, NT, 17>, k_fftB_b<MZ, NT, 17>, FAM_FFT_Z, grid, NT, x, s);
+}
+
 // Column-tile width of the z pass: 4, 8 or 16 (chosen per grid shape by the context's
 // schedule), with one extra LDS column for the leftover Nyquist column.
 template <int MZ>
@@ -318,12 +339,15 @@ static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
   const int T = column_tile(P, bt);
   const dim3 grid(P / T, a.g.my);
   const FftPass x{a, nullptr, spec, nullptr, nullptr, 0.0f, 0.0f, T};
-  // 128-point columns: 512 lanes (50 VGPRs) halve each lane's column loads (29.6 vs 30.5 us per
-  // pass at 128^3)
-  constexpr int NT = MZ >= 128 ? 512 : 256;
-  if (bt == 4) launch(k_fftB<MZ, NT, 5>, k_fftB_b<MZ, NT, 5>, FAM_FFT_Z, grid, NT, x, s);
-  else if (bt == 8) launch(k_fftB<MZ, NT, 9>, k_fftB_b<MZ, NT, 9>, FAM_FFT_Z, grid, NT, x, s);
-  else launch(k_fftB<MZ, NT, 17>, k_fftB_b<MZ, NT, 17>, FAM_FFT_Z, grid, NT, x, s);
+  // 512 lanes halve each lane's column loads for a single context's pass from 64-point columns
+  // up (128^3: 29.6 vs 30.5 us per pass; 50 VGPRs); batched dispatches below 128 keep 256
+  if constexpr (MZ >= 64) {
+    if (MZ >= 128 || t_recorder == nullptr) {
+      launch_B_nt<MZ, 512>(x, bt, grid, s);
+      return true;
+    }
+  }
+  launch_B_nt<MZ, 256>(x, bt, grid, s);
   return true;
 }
 
