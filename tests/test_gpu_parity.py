@@ -583,6 +583,37 @@ def test_bonds_without_wca_and_bad_file(tmp_path):
         F.Simulation(pb)
 
 
+def test_bonded_scheme_run_from_datamain(tmp_path):
+    """bondedForces in a fluam data.main, run through fhd_main: a trapped pair, a spring chain
+    and extra mobility; the .particles trajectory equals the context API's on the same
+    parameters, and the trap pulls its particle toward the fixed point."""
+    bonds = tmp_path / "springs.dat"
+    bonds.write_text("3\n0 1 2.0 1.0\n1 2 2.0 1.0\n2 3 2.0 0.0\n1\n4 0.5 0.0 3.0 3.0 3.0\n")
+    out = tmp_path / "bd"
+    dm = tmp_path / "data.main"
+    dm.write_text("stokesLimit\nnumberparticles 20\ncutoff 1.122462048309373\nsigma 1\nkernel 4\n"
+                  "initialPositions 1\nshearviscosity 1\ntemperature 0.0001\ncells 16 16 16\n"
+                  f"celldimension 16 16 16\nseed 12\nnumsteps 40\nsamplefreq 20\ndt 0.05\nextraMobility 0.2\n"
+                  f"bondedForces {bonds}\noutputname {out}\n")
+    p = F.load_datamain(str(dm))
+    assert p.bonded_forces == str(bonds).encode()
+    _announce("scheme_run_bonded", p)
+    assert F.main(str(dm))
+    rows = [ln for ln in open(str(out) + ".particles").read().split("\n")[1:] if ln.strip()]
+    assert len(rows) == 2 * (1 + 20)
+    sim = F.Simulation(p)
+    o = O.Oracle(p)
+    q0 = sim.particles().reshape(3, -1).T.copy()
+    sim.step(40)
+    o.step(40)
+    q = sim.particles().reshape(3, -1).T
+    got = np.array([[float(x) for x in ln.split()] for ln in rows[22:]])
+    assert np.abs(got - q).max() < 1e-12 * max(1.0, np.abs(q).max())
+    assert rel(sim.particles(), o.particles()) < 1e-5
+    x0 = np.array([3.0, 3.0, 3.0])
+    assert np.linalg.norm(q[4] - x0) < np.linalg.norm(q0[4] - x0)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_batch_with_bonds_equals_alone(tmp_path, mode):
     """Replicas with and without bonds in one batch give the same bits as each run alone."""
