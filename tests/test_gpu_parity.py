@@ -39,7 +39,8 @@ def test_normals_match_oracle():
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (6, 12, 16), (32, 32, 32), (24, 48, 16), (16, 96, 128),
-                                   (128, 128, 128), (16, 64, 24), (24, 16, 64), (48, 32, 48), (64, 24, 32)])
+                                   (128, 128, 128), (16, 64, 24), (24, 16, 64), (48, 32, 48), (64, 24, 32),
+                                   (32, 64, 64), (16, 96, 96)])
 def test_fft_r2c_matches_numpy(shape):
     rng = np.random.default_rng(sum(shape))
     a = rng.standard_normal(shape)
