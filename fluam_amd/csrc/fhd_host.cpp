@@ -137,6 +137,7 @@ struct fhd_ctx {
   int gperiod = 0, gphase = 0;
   bool own_stream = true;
   int zchunk = 8;                        // W kernel: z planes per workgroup (schedule)
+  int zchunk_b = 8;                      // the same inside batched dispatches
   hipEvent_t done_ev = nullptr;          // fhd_record / fhd_query
   // Stokes: the stochastic part of W is built on a side stream while the particle kernels run
   hipStream_t s2 = nullptr;
@@ -338,6 +339,7 @@ FluidWArgs base_w(fhd_ctx* c) {
   w.key = make_key(c);
   w.W = c->W;
   w.zchunk = c->zchunk;
+  w.zchunk_b = c->zchunk_b;
   return w;
 }
 
@@ -347,7 +349,7 @@ FluidWArgs base_w(fhd_ctx* c) {
 // process: a replica's schedule depends on its shape only, never on what else runs beside it,
 // and neither choice changes any result bit.
 struct Sched {
-  int btile, zchunk;
+  int btile, zchunk, zchunk_b;
 };
 std::mutex g_sched_mu;
 std::map<std::array<int, 3>, Sched> g_sched;
@@ -374,7 +376,7 @@ void apply_schedule(fhd_ctx* c) {
   const bool tune = !(env && env[0] == '0');
   std::lock_guard<std::mutex> lock(g_sched_mu);
   auto it = g_sched.find(key);
-  Sched sc = {8, 8};
+  Sched sc = {8, 8, 8};
   if (it != g_sched.end()) {
     sc = it->second;
   } else if (tune) {
@@ -385,8 +387,12 @@ void apply_schedule(fhd_ctx* c) {
       const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
       if (t < best) { best = t; sc.btile = bt; }
     }
+    // W kernel: 2 planes per workgroup fill the chip on small grids (64^3: 512 workgroups) when
+    // the context launches alone; a batched dispatch already holds many contexts' workgroups,
+    // where the redundant halo planes of short chunks cost more, so it keeps the best of 4-16
     best = 1e30f;
-    for (int zc : {4, 8, 16}) {
+    float best_b = 1e30f;
+    for (int zc : {2, 4, 8, 16}) {
       c->zchunk = zc;
       FluidWArgs w = base_w(c);
       w.force = c->SF;
@@ -395,11 +401,13 @@ void apply_schedule(fhd_ctx* c) {
       w.f1 = w.f4 = 1.0f;
       const float t = time_launches(c, [&] { launch_fluid_w(w, c->s); }, 4);
       if (t < best) { best = t; sc.zchunk = zc; }
+      if (zc >= 4 && t < best_b) { best_b = t; sc.zchunk_b = zc; }
     }
     g_sched[key] = sc;
   }
   c->fa.btile = sc.btile;
   c->zchunk = sc.zchunk;
+  c->zchunk_b = sc.zchunk_b;
 }
 
 // ---------------- particles ----------------

@@ -42,7 +42,8 @@ struct FluidWArgs {
   float f1, f4;            // diagonal / off-diagonal noise prefactors
   Key key;
   float* W;
-  int zchunk;
+  int zchunk;           // z planes per workgroup
+  int zchunk_b;         // the same when the launch is recorded into a batched dispatch
 };
 
 struct FillArgs {

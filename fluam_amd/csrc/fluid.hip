@@ -222,8 +222,10 @@ void launch_advance_step(unsigned long long* step, hipStream_t s) {
 }
 
 void launch_fluid_w(const FluidWArgs& a, hipStream_t s) {
-  dim3 grid((a.g.mx + TX - 1) / TX, (a.g.my + TY - 1) / TY, (a.g.mz + a.zchunk - 1) / a.zchunk);
-  launch(k_fluid_w, k_fluid_w_b, FAM_FLUID_W, grid, 256, a, s);
+  FluidWArgs x = a;
+  if (t_recorder && x.zchunk_b > 0) x.zchunk = x.zchunk_b;
+  dim3 grid((x.g.mx + TX - 1) / TX, (x.g.my + TY - 1) / TY, (x.g.mz + x.zchunk - 1) / x.zchunk);
+  launch(k_fluid_w, k_fluid_w_b, FAM_FLUID_W, grid, 256, x, s);
 }
 
 void launch_divergence(const GridDev& g, const float* v, float* div, hipStream_t s) {
