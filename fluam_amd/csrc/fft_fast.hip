@@ -325,17 +325,18 @@ static int column_tile(int P, int tmax) {
 
 template <int MZ, int NT>
 static void launch_B_nt(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
-  if (bt == 4) launch(k_fftB<MZ, NT, 5>, k_fftB_b<MZ, NT, 5>, FAM_FFT_Z, grid, NT, x, s);
+  if (bt == 2) launch(k_fftB<MZ, NT, 3>, k_fftB_b<MZ, NT, 3>, FAM_FFT_Z, grid, NT, x, s);
+  else if (bt == 4) launch(k_fftB<MZ, NT, 5>, k_fftB_b<MZ, NT, 5>, FAM_FFT_Z, grid, NT, x, s);
   else if (bt == 8) launch(k_fftB<MZ, NT, 9>, k_fftB_b<MZ, NT, 9>, FAM_FFT_Z, grid, NT, x, s);
   else launch(k_fftB<MZ, NT, 17>, k_fftB_b<MZ, NT, 17>, FAM_FFT_Z, grid, NT, x, s);
 }
 
-// Column-tile width of the z pass: 4, 8 or 16 (chosen per grid shape by the context's
+// Column-tile width of the z pass: 2, 4, 8 or 16 (chosen per grid shape by the context's
 // schedule), with one extra LDS column for the leftover Nyquist column.
 template <int MZ>
 static bool launch_B2(const FftArgs& a, float2* spec, hipStream_t s) {
   const int P = a.g.P;
-  const int bt = a.btile == 4 || a.btile == 16 ? a.btile : 8;
+  const int bt = a.btile == 2 || a.btile == 4 || a.btile == 16 ? a.btile : 8;
   const int T = column_tile(P, bt);
   const dim3 grid(P / T, a.g.my);
   const FftPass x{a, nullptr, spec, nullptr, nullptr, 0.0f, 0.0f, T};

@@ -382,7 +382,7 @@ void apply_schedule(fhd_ctx* c) {
   } else if (tune) {
     // scratch runs on the context's own buffers before its state exists (contents irrelevant)
     float best = 1e30f;
-    for (int bt : {4, 8, 16}) {
+    for (int bt : {2, 4, 8, 16}) {  // 2: more column workgroups on small grids
       c->fa.btile = bt;
       const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
       if (t < best) { best = t; sc.btile = bt; }
@@ -408,6 +408,17 @@ void apply_schedule(fhd_ctx* c) {
   c->fa.btile = sc.btile;
   c->zchunk = sc.zchunk;
   c->zchunk_b = sc.zchunk_b;
+  // explicit schedule (tests and experiments): FHD_BTILE in {2, 4, 8, 16}, FHD_ZCHUNK >= 1
+  if (const char* e = getenv("FHD_BTILE")) {
+    const int v = atoi(e);
+    if (v != 2 && v != 4 && v != 8 && v != 16) throw std::runtime_error("FHD_BTILE must be 2, 4, 8 or 16");
+    c->fa.btile = v;
+  }
+  if (const char* e = getenv("FHD_ZCHUNK")) {
+    const int v = atoi(e);
+    if (v < 1) throw std::runtime_error("FHD_ZCHUNK must be at least 1");
+    c->zchunk = c->zchunk_b = v;
+  }
 }
 
 // ---------------- particles ----------------

@@ -17,7 +17,7 @@ struct FftArgs {
   uint32_t plan_h, plan_y, plan_z;  // radices, 4 bits per stage
   int nst_h, nst_y, nst_z;
   int ztile;                      // kx columns per pass-B workgroup (generic kernels)
-  int btile;                      // kx columns per pass-B workgroup (size-specialised: 4, 8 or 16)
+  int btile;                      // kx columns per pass-B workgroup (size-specialised: 2, 4, 8 or 16)
   int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
   int generic;                    // 1: force the generic (runtime-size) kernels
   int stokes;                     // 1: v = P W / (eta |g|^2), mode 0 -> 0

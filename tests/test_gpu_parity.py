@@ -363,6 +363,31 @@ def test_autotuned_schedule_changes_no_bits():
 
 
 
+@pytest.mark.parametrize("btile,zchunk", [(2, 1), (2, 2), (4, 3), (16, 16)])
+def test_every_schedule_changes_no_bits(btile, zchunk):
+    """Forced schedules (z-pass column tiles 2-16, W-kernel chunks of 1-16 planes, including
+    ones that do not divide the grid) give the bits of the fixed default schedule, for an
+    incompressible 32^3 run and a Stokes run with particles."""
+    cases = [make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=5),
+             _replica_params(8, (48, 48, 24), 200)]
+    for p in cases:
+        _announce(f"schedule_bt{btile}_zc{zchunk}", p)
+        os.environ["FHD_AUTOTUNE"] = "0"
+        fixed = F.Simulation(p)
+        os.environ["FHD_BTILE"] = str(btile)
+        os.environ["FHD_ZCHUNK"] = str(zchunk)
+        try:
+            forced = F.Simulation(p)
+        finally:
+            del os.environ["FHD_BTILE"], os.environ["FHD_ZCHUNK"]
+            os.environ["FHD_AUTOTUNE"] = "1"
+        fixed.step(6)
+        forced.step(6)
+        assert np.array_equal(fixed.velocity(), forced.velocity())
+        if p.np:
+            assert np.array_equal(fixed.particles(), forced.particles())
+
+
 def test_blown_up_start_moves_no_coordinate_by_2p24_cells():
     """A dense uniform-random start (pairs deep inside the WCA core) asks for displacements of
     ~1e16 cells.  Such a coordinate stays where it was, so no coordinate moves by 2^24 cells or
