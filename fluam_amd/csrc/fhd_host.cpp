@@ -354,20 +354,26 @@ struct Sched {
 std::mutex g_sched_mu;
 std::map<std::array<int, 3>, Sched> g_sched;
 
+// Shortest of three timed rounds of reps launches (one warm-up launch first), so one slow
+// round does not decide the schedule.
 float time_launches(fhd_ctx* c, const std::function<void()>& f, int reps) {
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   f();  // warm-up
-  HIPCHK(hipEventRecord(a, c->s));
-  for (int k = 0; k < reps; ++k) f();
-  HIPCHK(hipEventRecord(b, c->s));
-  HIPCHK(hipEventSynchronize(b));
-  float ms = 0.0f;
-  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  float best = 1e30f;
+  for (int round = 0; round < 3; ++round) {
+    HIPCHK(hipEventRecord(a, c->s));
+    for (int k = 0; k < reps; ++k) f();
+    HIPCHK(hipEventRecord(b, c->s));
+    HIPCHK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    best = std::min(best, ms);
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  return ms;
+  return best;
 }
 
 void apply_schedule(fhd_ctx* c) {
@@ -394,11 +400,17 @@ void apply_schedule(fhd_ctx* c) {
     float best_b = 1e30f;
     for (int zc : {2, 4, 8, 16}) {
       c->zchunk = zc;
+      // the form the scheme launches: noise only (Stokes), or the full stencil with advection
       FluidWArgs w = base_w(c);
-      w.force = c->SF;
-      w.force_coef = 1.0f;
-      w.noise = 2;
+      w.noise = 1;
       w.f1 = w.f4 = 1.0f;
+      if (c->p.scheme != FHD_SCHEME_STOKES_LIMIT) {
+        w.v = c->v;
+        w.vadv = c->v;
+        w.lap_coef = 1.0f;
+        w.adv_coef = 1.0f;
+        w.trace_free = 1;
+      }
       const float t = time_launches(c, [&] { launch_fluid_w(w, c->s); }, 4);
       if (t < best) { best = t; sc.zchunk = zc; }
       if (zc >= 4 && t < best_b) { best_b = t; sc.zchunk_b = zc; }
