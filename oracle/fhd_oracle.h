@@ -22,6 +22,8 @@
  *   incompressible step   dense-matrix brute force (I - dt nu L/2)^-1 P on a 4^3 grid,
  *                         divergence, momentum conservation, equal-time S(k)
  *   Stokes solve          dense saddle-point solve on a 4^3 grid, zero mean
+ *   Stokes step           at kT = 0: the midpoint rule over the pinned trap force, spread,
+ *                         Stokes solve and interpolation
  *   QNB step              mass=0 reduces to the incompressible step with no particles;
  *                         particle velocities = J v; divergence
  */

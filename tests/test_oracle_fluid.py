@@ -292,3 +292,30 @@ def test_oracle_restart_reads_fluid_file_and_step(tmp_path):
     # the restarted run projects its loaded field once more (as a fresh run does), so it agrees
     # to round-off rather than bit for bit
     assert np.abs(a.velocity() - b.velocity()).max() < 1e-12 * max(1.0, np.abs(v3).max())
+
+
+def test_stokes_midpoint_step_composes_pinned_blocks(tmp_path):
+    """A Stokes-limit step of one particle in a harmonic trap at kT = 0 (no noise, no RFD drift)
+    is the midpoint rule over the separately pinned blocks (src/runSchemeStokesLimit.cu:163-276):
+    u(q) = J(q) (eta L)^-1 P S(q) F(q), q_half = q + dt/2 u(q), q_new = q + dt u(q_half), with F the
+    trap force, S the spread (1/dV included), the Stokes solve and J the interpolation."""
+    bonds = tmp_path / "trap.dat"
+    bonds.write_text("0\n1\n0 0.7 0.0 3.0 -2.0 1.5\n")
+    for kernel in (3, 4):
+        p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 12), box=(16.0, 12.0, 9.0), np=1, kernel_points=kernel,
+                        nonbonded=0, temperature=0.0, dt=0.2, seed=4, initial_positions=0,
+                        bonded_forces=str(bonds))
+        o = O.Oracle(p)
+        q0 = np.array([-4.3, 2.6, -1.2])
+        o.set_particles(q0)
+
+        def u(q):
+            W = o.spread(q, o.bonded(q))
+            return o.interp(q, o.solve_stokes(W))
+
+        qh = q0 + 0.5 * p.dt * u(q0)
+        q1 = q0 + p.dt * u(qh)
+        o.step(1)
+        got = o.particles()
+        assert np.abs(got - q1).max() < 1e-12 * np.abs(q1).max()
+        assert np.linalg.norm(q1 - np.array([3.0, -2.0, 1.5])) < np.linalg.norm(q0 - np.array([3.0, -2.0, 1.5]))
