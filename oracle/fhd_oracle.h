@@ -22,8 +22,9 @@
  *   incompressible step   dense-matrix brute force (I - dt nu L/2)^-1 P on a 4^3 grid,
  *                         divergence, momentum conservation, equal-time S(k)
  *   Stokes solve          dense saddle-point solve on a 4^3 grid, zero mean
- *   Stokes step           at kT = 0: the midpoint rule over the pinned trap force, spread,
- *                         Stokes solve and interpolation
+ *   Stokes step           the midpoint rule over the pinned trap force, spread, Stokes solve,
+ *                         interpolation, noise slots, noise divergence and RFD spreads
+ *                         (kT = 0 and kT > 0)
  *   QNB step              mass=0 reduces to the incompressible step with no particles;
  *                         particle velocities = J v; divergence
  */

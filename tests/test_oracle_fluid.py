@@ -319,3 +319,42 @@ def test_stokes_midpoint_step_composes_pinned_blocks(tmp_path):
         got = o.particles()
         assert np.abs(got - q1).max() < 1e-12 * np.abs(q1).max()
         assert np.linalg.norm(q1 - np.array([3.0, -2.0, 1.5])) < np.linalg.norm(q0 - np.array([3.0, -2.0, 1.5]))
+
+
+def test_stokes_stochastic_step_composes_pinned_blocks(tmp_path):
+    """The full Stokes-limit step at kT > 0 (src/runSchemeStokesLimit.cu:163-276): solve 1 takes
+    W = S F(q) + sqrt(2) D.Sigma_1; solve 2 takes W = S F(q_half) + (kT/delta)[S(q + delta/2 w) w
+    - S(q - delta/2 w) w] + sqrt(1/2) D.(Sigma_1 + Sigma_2), with the RFD direction w and the
+    noise slots drawn from the step's counters.  Rebuilt here from the separately pinned noise,
+    noise divergence, spread, Stokes solve and interpolation; it equals the oracle's step."""
+    bonds = tmp_path / "traps.dat"
+    bonds.write_text("0\n3\n0 0.7 0.0 3.0 -2.0 1.5\n1 0.4 0.5 -1.0 1.0 0.0\n2 1.1 0.0 0.0 0.0 -2.5\n")
+    kT, eta, dt = 1e-3, 1.0, 0.2
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 12), box=(16.0, 12.0, 9.0), np=3, kernel_points=4,
+                    nonbonded=0, temperature=kT, shearviscosity=eta, dt=dt, seed=11, initial_positions=0,
+                    bonded_forces=str(bonds))
+    o = O.Oracle(p)
+    q0 = np.array([-4.3, 2.6, 1.0, 2.6, -1.2, 0.4, -1.2, 3.0, -2.0])  # [x0 x1 x2 y0 y1 y2 z0 z1 z2]
+    o.set_particles(q0)
+    N = 16 * 16 * 12
+    dV = 1.0 * 0.75 * 0.75
+    f1 = np.sqrt(4 * kT * eta / (dV * dt))
+    f4 = np.sqrt(2 * kT * eta / (dV * dt))
+    Z1 = O.fluid_noise(p.seed, 0, 0, N)
+    Z2 = O.fluid_noise(p.seed, 0, 1, N)
+    w = np.array([O.normals4(p.seed, 0, 3, 0, i)[:3] for i in range(3)]).T.ravel()  # RFD directions
+    delta = p.rfd_delta
+
+    def vel(q, W):
+        return o.interp(q, o.solve_stokes(W))
+
+    W1 = o.spread(q0, o.bonded(q0)) + O.noise_divergence(o, Z1, np.sqrt(2) * f1, np.sqrt(2) * f4, False)
+    qh = q0 + 0.5 * dt * vel(q0, W1)
+    rfd = o.spread(q0 + 0.5 * delta * w, kT / delta * w) - o.spread(q0 - 0.5 * delta * w, kT / delta * w)
+    W2 = (o.spread(qh, o.bonded(qh)) + rfd + O.noise_divergence(o, Z1, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False)
+          + O.noise_divergence(o, Z2, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False))
+    q1 = q0 + dt * vel(qh, W2)
+    o.step(1)
+    got = o.particles()
+    assert np.abs(got - q1).max() < 1e-11 * np.abs(q1).max()
+    assert np.abs(q1 - q0).max() > 1e-4  # the particles moved
