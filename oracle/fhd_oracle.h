@@ -26,7 +26,8 @@
  *                         interpolation, noise slots, noise divergence and RFD spreads
  *                         (kT = 0 and kT > 0)
  *   QNB step              mass=0 reduces to the incompressible step with no particles;
- *                         particle velocities = J v; divergence
+ *                         particle velocities = J v; divergence; at kT = 0 the bootstrap and
+ *                         Adams-Bashforth steps over the pinned blocks
  */
 #ifndef FHD_ORACLE_H
 #define FHD_ORACLE_H

@@ -358,3 +358,40 @@ def test_stokes_stochastic_step_composes_pinned_blocks(tmp_path):
     got = o.particles()
     assert np.abs(got - q1).max() < 1e-11 * np.abs(q1).max()
     assert np.abs(q1 - q0).max() > 1e-4  # the particles moved
+
+
+def test_qnb_steps_compose_pinned_blocks(tmp_path):
+    """Three QNB (mass 0) steps of trapped particles at kT = 0 (no noise, no thermal drift), from
+    rest: the bootstrap predictor-corrector (src/firstStepQuasiNeutrallyBuoyant.cu:35-1152), then
+    Adams-Bashforth steps (src/runSchemeQuasiNeutrallyBuoyant.cu:178-677), rebuilt from the
+    separately pinned Laplacian and advection terms, spread, Crank-Nicolson solve and
+    interpolation: q_half = q + dt/2 J(q) v, b = v + lap(v) + dt S F(q_half) / rho; first step
+    v~ = solve(b - A(v_half)) with v_half from a predictor solve(b - A(v)); later steps
+    v~ = solve(b - 3/2 A(v) + 1/2 A(v_prev)); q_new = q + dt J(q_half) (v + v~) / 2."""
+    bonds = tmp_path / "traps.dat"
+    bonds.write_text("0\n2\n0 0.9 0.0 2.0 -1.0 1.5\n1 0.6 0.3 -2.0 2.0 -1.0\n")
+    rho, dt = 1.3, 0.1
+    p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(16, 12, 12), box=(16.0, 12.0, 9.0), np=2,
+                    kernel_points=3, nonbonded=0, temperature=0.0, densfluid=rho, dt=dt, seed=13,
+                    initial_positions=0, bonded_forces=str(bonds))
+    o = O.Oracle(p)
+    q = np.array([-3.1, 3.4, 1.7, -0.6, 0.9, 2.2])
+    o.set_particles(q)
+    v = np.zeros(3 * o.N)
+    a_prev = None
+    for n in range(3):
+        qh = q + 0.5 * dt * o.interp(q, v)
+        b = v + O.laplacian_term(o, v) + dt * o.spread(qh, o.bonded(qh)) / rho
+        A = O.advection_term(o, v)
+        if n == 0:
+            vp = o.solve_incompressible(b - A)
+            vt = o.solve_incompressible(b - O.advection_term(o, 0.5 * (v + vp)))
+        else:
+            vt = o.solve_incompressible(b - 1.5 * A + 0.5 * a_prev)
+        a_prev = A
+        q = q + dt * o.interp(qh, 0.5 * (v + vt))
+        v = vt
+        o.step(1)
+        assert np.abs(o.particles() - q).max() < 1e-11 * np.abs(q).max(), n
+        assert np.abs(o.velocity() - v).max() < 1e-11 * max(np.abs(v).max(), 1e-300), n
+    assert np.abs(v).max() > 1e-4  # the trapped particles drove the fluid
