@@ -640,6 +640,17 @@ def test_bonded_scheme_run_from_datamain(tmp_path):
     assert np.linalg.norm(q[4] - x0) < np.linalg.norm(q0[4] - x0)
 
 
+@pytest.mark.parametrize("text", ["2\n0 1 1.0 0.0\n", "1\n0 1 1.0 0.0\n1\n0 1.0 0.0 1.0 2.0\n", "-1\n0\n", ""])
+def test_malformed_bond_file_is_an_error(tmp_path, text):
+    """A truncated bond file or a negative count is reported at creation, nothing is run."""
+    path = tmp_path / "bonds.dat"
+    path.write_text(text)
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=2, kernel_points=3, initial_positions=1,
+                    bonded_forces=str(path), **WCA)
+    with pytest.raises(F.FhdError, match="bonds file"):
+        F.Simulation(p)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_batch_with_bonds_equals_alone(tmp_path, mode):
     """Replicas with and without bonds in one batch give the same bits as each run alone."""

@@ -108,3 +108,13 @@ def test_coincident_particles(tmp_path):
 def test_bad_bond_file_is_rejected(tmp_path):
     with pytest.raises(RuntimeError):
         _oracle(tmp_path, 2, pairs=[(0, 2, 1.0, 0.0)])  # index outside 0..np-1
+
+
+@pytest.mark.parametrize("text", ["2\n0 1 1.0 0.0\n", "1\n0 1 1.0 0.0\n1\n0 1.0 0.0 1.0 2.0\n", "-1\n0\n", ""])
+def test_truncated_or_malformed_bond_file_is_rejected(tmp_path, text):
+    path = tmp_path / "bonds.dat"
+    path.write_text(text)
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=2, nonbonded=0, initial_positions=0, seed=3,
+                    bonded_forces=str(path))
+    with pytest.raises(RuntimeError):
+        O.Oracle(p)
