@@ -86,6 +86,7 @@ def lib():
     L.fhd_batch_time_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.fhd_get_step.argtypes = [vp]
     L.fhd_get_step.restype = C.c_longlong
+    L.fhd_get_np.argtypes = [vp]
     L.fhd_time_steps.argtypes = [vp, C.c_int, dp]
     L.fhd_profile_kernels.argtypes = [vp, C.c_int, dp, C.c_char_p, C.c_int]
     L.fhd_fft_r2c.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
@@ -180,6 +181,7 @@ class Simulation:
         self.h = lib().fhd_create(C.byref(self.p)) if handle is None else handle
         if not self.h:
             raise FhdError(_err())
+        self.p.np = lib().fhd_get_np(self.h)  # a coordinates file sets the count
         self.N = self.p.mx * self.p.my * self.p.mz
 
     def close(self):

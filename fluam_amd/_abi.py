@@ -47,6 +47,7 @@ class FhdParams(C.Structure):
         ("initial_step", C.c_longlong),
         ("bonded_forces", C.c_char * 512),
         ("structure_factor", C.c_int),
+        ("velocities", C.c_char * 512),
     ]
 
     def copy(self):
@@ -96,7 +97,7 @@ def make_params(scheme=SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), box=None, **kw
         if k == "background":
             for a in range(3):
                 p.background[a] = v[a]
-        elif k in ("outputname", "coordinates", "fluid", "bonded_forces"):
+        elif k in ("outputname", "coordinates", "fluid", "bonded_forces", "velocities"):
             setattr(p, k, v.encode() if isinstance(v, str) else v)
         else:
             if not hasattr(p, k):

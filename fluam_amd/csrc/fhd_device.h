@@ -17,7 +17,7 @@ FHD_DEV int xcd_item(int b, int n) {
 
 // Noise streams (counter word 1 = stream << 4 | sub).
 enum { FHD_STREAM_FLUID = 1, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4, FHD_STREAM_PART2 = 5,
-       FHD_STREAM_INIT_POS = 8 };
+       FHD_STREAM_INIT_POS = 8, FHD_STREAM_INIT_FLUID = 9 };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products map to

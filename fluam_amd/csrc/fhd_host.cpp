@@ -891,6 +891,19 @@ extern "C" void fhd_default_params(fhd_params* p) {
 
 namespace {
 
+// The particle count a coordinates file declares (its first number): with loadparticles 1 it
+// replaces numberparticles, as src/createParticles.cpp:88 reads np from the file.
+bool coordinates_count(const char* path, int* np, std::string& err) {
+  std::ifstream f(path);
+  long long n = -1;
+  if (!f || !(f >> n) || n < 0 || n > (1LL << 30)) {
+    err = std::string("cannot read the particle count of coordinates file ") + path;
+    return false;
+  }
+  *np = (int)n;
+  return true;
+}
+
 bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
   std::ifstream f(path);
   if (!f) {
@@ -950,7 +963,7 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
       else if (word == "coordinates") { std::string s; need(bool(f >> s)); snprintf(p.coordinates, sizeof(p.coordinates), "%s", s.c_str()); }
       else if (word == "fluid") { std::string s; need(bool(f >> s)); snprintf(p.fluid, sizeof(p.fluid), "%s", s.c_str()); }
       else if (word == "initialStep") need(bool(f >> p.initial_step));
-      else if (word == "velocities") { std::string s; need(bool(f >> s)); }
+      else if (word == "velocities") { std::string s; need(bool(f >> s)); snprintf(p.velocities, sizeof(p.velocities), "%s", s.c_str()); }
       else if (word == "computeNonBondedForces") need(bool(f >> p.nonbonded));
       else if (word == "extraMobility") need(bool(f >> p.extra_mobility));
       else if (word == "bondedForces") { std::string s; need(bool(f >> s)); snprintf(p.bonded_forces, sizeof(p.bonded_forces), "%s", s.c_str()); }
@@ -1003,6 +1016,7 @@ bool parse_datamain(const char* path, fhd_params* out, std::string& err) {
     err = "loadparticles 1 needs a coordinates file";
     return false;
   }
+  if (p.initial_positions == 2 && !coordinates_count(p.coordinates, &p.np, err)) return false;
   *out = p;
   return true;
 }
@@ -1054,13 +1068,35 @@ bool read_coordinates(const fhd_params& p, std::vector<double>& q, std::string& 
   int n = 0;
   f >> n;
   if (n != p.np) {
-    err = "coordinates file particle count differs from numberparticles";
+    err = "coordinates file particle count changed while it was read";
     return false;
   }
   q.assign(3 * (size_t)n, 0.0);
   for (int i = 0; i < n; ++i)
     if (!(f >> q[i] >> q[n + i] >> q[2 * (size_t)n + i])) {
       err = "coordinates file truncated";
+      return false;
+    }
+  return true;
+}
+
+// velocities file: "np" then np lines "vx vy vz"; its count must equal the coordinates file's
+// (src/createParticles.cpp:103-120).  Returned component-major, like positions.
+bool read_velocities(const fhd_params& p, std::vector<double>& u, std::string& err) {
+  std::ifstream f(p.velocities);
+  if (!f) {
+    err = std::string("cannot open velocities file ") + p.velocities;
+    return false;
+  }
+  int n = -1;
+  if (!(f >> n) || n != p.np) {
+    err = "ERROR NP DIFFERENT IN PARTICLESCOOR AND PARTICLESVEL (velocities file count differs from the coordinates)";
+    return false;
+  }
+  u.assign(3 * (size_t)n, 0.0);
+  for (int i = 0; i < n; ++i)
+    if (!(f >> u[i] >> u[n + i] >> u[2 * (size_t)n + i])) {
+      err = "velocities file truncated";
       return false;
     }
   return true;
@@ -1147,6 +1183,63 @@ void philox_host(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) 
     k1 += 0xBB67AE85u;
   }
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Four standard normals of one host draw (Box-Muller on 24-bit uniforms, FP64), the same
+// construction as the device generator's.
+void host_normals4(uint64_t seed, uint32_t index, uint32_t stream, double out[4]) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  const uint32_t ctr[4] = {index, stream << 4, 0u, 0u};
+  uint32_t w[4];
+  philox_host(ctr, key, w);
+  for (int k = 0; k < 2; ++k) {
+    const double u1 = (double)((w[2 * k] >> 8) + 1u) * (1.0 / 16777216.0);
+    const double u2 = (double)(w[2 * k + 1] >> 8) * (1.0 / 16777216.0);
+    const double r = sqrt(-2.0 * log(u1));
+    out[2 * k] = r * cos(2.0 * M_PI * u2);
+    out[2 * k + 1] = r * sin(2.0 * M_PI * u2);
+  }
+}
+
+// Initial velocity for initfluid 0-4 (src/initializeFluid.cpp:31-160): 0 the background velocity;
+// 1 equilibrium fluctuations sqrt(kT / (rho dV)) N(0,1) per face (cell i draws its three normals
+// from stream FHD_STREAM_INIT_FLUID at index i in place of fluam's host gauss()), minus their
+// mean, plus the background; 2 a transverse wave vy = vy0 + 0.04 sin(2 pi z / lz); 3 a
+// longitudinal wave vz = 0.004 sin(16 pi (z + dz/2) / lz); 4 vx = 0.425.  z is the cell centre,
+// (iz + 1/2) dz - lz/2 (src/createCells.cpp:66-69).
+void initial_fluid(const fhd_params& p, std::vector<double>& v) {
+  const int mx = p.mx, my = p.my, mz = p.mz;
+  const long long N = (long long)mx * my * mz;
+  const double dx = p.lx / mx, dy = p.ly / my, dz = p.lz / mz, dV = dx * dy * dz;
+  v.assign(3 * (size_t)N, 0.0);
+  const double* b = p.background;
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (long long i = 0; i < N; ++i) {
+    const double z = (double)(i / ((long long)mx * my) + 0.5) * dz - 0.5 * p.lz;
+    double c[3] = {b[0], b[1], b[2]};
+    switch (p.initfluid) {
+      case 1: {
+        double n[4];
+        host_normals4(p.seed, (uint32_t)i, FHD_STREAM_INIT_FLUID, n);
+        const double s = sqrt(p.temperature / (p.densfluid * dV));
+        for (int a = 0; a < 3; ++a) {
+          c[a] = s * n[a];
+          mean[a] += c[a];
+        }
+        break;
+      }
+      case 2: c[1] = b[1] + 0.04 * sin(2.0 * M_PI / p.lz * z); break;
+      case 3: c[0] = 0.0; c[1] = 0.0; c[2] = 0.004 * sin(2.0 * M_PI / p.lz * 8.0 * (z + 0.5 * p.lz / mz)); break;
+      case 4: c[0] = 0.425; c[1] = 0.0; c[2] = 0.0; break;
+      default: break;
+    }
+    for (int a = 0; a < 3; ++a) v[(size_t)a * N + i] = c[a];
+  }
+  if (p.initfluid == 1)
+    for (int a = 0; a < 3; ++a) {
+      const double m = mean[a] / (double)N;
+      for (long long i = 0; i < N; ++i) v[(size_t)a * N + i] = v[(size_t)a * N + i] - m + b[a];
+    }
 }
 
 void initial_positions(const fhd_params& p, std::vector<double>& q) {
@@ -1289,6 +1382,13 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
   fhd_ctx* c = new fhd_ctx();
   try {
     c->p = *pin;
+    if (c->p.initial_positions == 2) {
+      // loadparticles 1: the coordinates file's count is the particle count
+      std::string err;
+      if (!coordinates_count(c->p.coordinates, &c->p.np, err)) throw std::runtime_error(err);
+    }
+    if (c->p.initfluid < 0 || c->p.initfluid > 4)
+      throw std::runtime_error("initfluid must be 0-4 (the options of src/initializeFluid.cpp for this path)");
     if (c->p.wca_sigma < 0) c->p.wca_sigma = 2.0 * c->p.lx / c->p.mx;
     if (c->p.wca_epsilon < 0) c->p.wca_epsilon = c->p.temperature;
     validate(c->p);
@@ -1324,10 +1424,14 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
     c->errd = dalloc<int>(8);
     HIPCHK(hipMemset(c->errd, 0, 8 * sizeof(int)));
     build_tables(c);
-    // initial velocity: initfluid 0 -> background velocity (src/initializeFluid.cpp:31-60)
+    // initial velocity (initfluid, src/initializeFluid.cpp:31-160); the Stokes limit has no
+    // velocity state
     std::vector<float> v0(N3, 0.0f);
-    if (p.scheme != FHD_SCHEME_STOKES_LIMIT)
-      for (int a = 0; a < 3; ++a) std::fill(v0.begin() + (size_t)a * g.N, v0.begin() + (size_t)(a + 1) * g.N, (float)p.background[a]);
+    if (p.scheme != FHD_SCHEME_STOKES_LIMIT) {
+      std::vector<double> vi;
+      initial_fluid(p, vi);
+      for (size_t i = 0; i < N3; ++i) v0[i] = (float)vi[i];
+    }
     if (p.fluid[0]) {
       // restart: velocity and density from a saved configuration (fluid keyword)
       std::vector<double> vf;
@@ -1374,6 +1478,16 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
         initial_positions(p, q);
       }
       if (!fhd_set_particles(c, q.data())) throw std::runtime_error(c->err);
+      if (p.initial_positions == 2 && p.velocities[0]) {
+        // initial particle velocities (reported until the first step replaces them)
+        std::vector<double> uv;
+        std::string err;
+        if (!read_velocities(p, uv, err)) throw std::runtime_error(err);
+        std::vector<float4> hu(c->np);
+        for (int i = 0; i < c->np; ++i)
+          hu[i] = make_float4((float)uv[i], (float)uv[c->np + i], (float)uv[2 * (size_t)c->np + i], 0.0f);
+        HIPCHK(hipMemcpy(c->u, hu.data(), c->np * sizeof(float4), hipMemcpyHostToDevice));
+      }
       if (p.bonded_forces[0]) {
         HostBonds hb;
         std::string err;
@@ -1606,6 +1720,7 @@ int fhd_sync(fhd_ctx* c) {
 }
 
 long long fhd_get_step(const fhd_ctx* c) { return c ? c->step : -1; }
+int fhd_get_np(const fhd_ctx* c) { return c ? c->np : -1; }
 
 int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms) {
   CTX_TRY(c, {
@@ -1832,7 +1947,16 @@ long long ctx_bytes(const fhd_params& p) {
 
 extern "C" {
 
-long long fhd_bytes_needed(const fhd_params* p) { return p ? ctx_bytes(*p) : -1; }
+long long fhd_bytes_needed(const fhd_params* p) {
+  if (!p) return -1;
+  fhd_params q = *p;
+  std::string err;
+  if (q.initial_positions == 2 && !coordinates_count(q.coordinates, &q.np, err)) {
+    g_last_error = err;
+    return -1;
+  }
+  return ctx_bytes(q);
+}
 
 int fhd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes) {
   size_t f = 0, t = 0;

@@ -57,7 +57,11 @@ typedef struct fhd_params {
   long long numsteps;         /* numsteps */
   long long numsteps_relaxation; /* numstepsRelaxation */
   long long samplefreq;       /* samplefreq */
-  int initfluid;              /* initfluid (0: background velocity) */
+  int initfluid;              /* initfluid (src/initializeFluid.cpp:31-160): 0 background velocity;
+                                 1 background + equilibrium fluctuations sqrt(kT/(rho dV)) N(0,1) per
+                                 face with the mean removed (normals from the library's counter
+                                 generator, stream 9); 2 transverse wave vy += 0.04 sin(2 pi z/lz);
+                                 3 longitudinal wave vz = 0.004 sin(16 pi (z + dz/2)/lz); 4 vx = 0.425 */
   double background[3];       /* backgroundvelocity */
   int np;                     /* numberparticles */
   int kernel_points;          /* extension keyword `kernel` 3|4 (Peskin 3- or 4-point) */
@@ -76,7 +80,8 @@ typedef struct fhd_params {
                                  2: coordinates file, 3: seeded uniform-random without pairs closer
                                  than the WCA sigma (random sequential addition) */
   char outputname[512];       /* outputname */
-  char coordinates[512];      /* coordinates file (loadparticles 1) */
+  char coordinates[512];      /* coordinates file (loadparticles 1): its first number is the particle
+                                 count, which replaces numberparticles (src/createParticles.cpp:81-100) */
   char fluid[512];            /* fluid: initial velocity and density from a .fluidFinalConfiguration
                                  file (src/loadDataMain.cu:443-445, src/initializeFluid.cpp:168-192) */
   long long initial_step;     /* initialStep (extension): step counter of the first step, so a run
@@ -87,6 +92,8 @@ typedef struct fhd_params {
                                  equal-time structure factor every samplefreq steps and writes
                                  <outputname>.structureFactor (fluam's HydroGrid hook,
                                  src/hydroAnalysisIncompressible.cu:35-92) */
+  char velocities[512];       /* velocities: initial particle velocities, "np" then np lines "vx vy vz";
+                                 np must equal the coordinates file's (src/createParticles.cpp:103-120) */
 } fhd_params;
 
 /* Fill *p with the defaults of src/loadDataMain.cu:177-299 (restated). */
@@ -141,6 +148,9 @@ int fhd_sk_reset(fhd_ctx* c);
 int fhd_sk_sample(fhd_ctx* c);
 int fhd_sk_read(fhd_ctx* c, double* s, long long* nsamples);
 long long fhd_get_step(const fhd_ctx* c);
+/* Particles the context holds: numberparticles, or the count its coordinates file declares
+ * (loadparticles 1, src/createParticles.cpp:88); -1 for a null context. */
+int fhd_get_np(const fhd_ctx* c);
 /* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
  * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names
  * into kern_names (each up to 47 chars + NUL, 48-byte stride); returns kernels reported. */

@@ -85,7 +85,7 @@ static void uniforms4(uint64_t seed, int64_t step, int stream, int sub, uint32_t
 }
 
 enum { STREAM_FLUID1 = 1, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
-       STREAM_INIT_POS = 8 };
+       STREAM_INIT_POS = 8, STREAM_INIT_FLUID = 9 };
 
 // Stress noise of one cell and step: 12 normals n_k, k = 0..11, where n_k is word k%4 of the
 // Philox draw with sub = k/4 (stream STREAM_FLUID1).  Sigma_1 = n_0..n_5 and Sigma_2 = n_6..n_11
@@ -220,7 +220,7 @@ extern "C" double orc_delta4(double r) {
   if (x <= 1.0) return (3.0 - 2.0 * x + sqrt(1.0 + 4.0 * x - 4.0 * x * x)) / 8.0;
   return (5.0 - 2.0 * x - sqrt(-7.0 + 12.0 * x - 4.0 * x * x)) / 8.0;
 }
-static double ddelta4(double r) {
+extern "C" double orc_ddelta4(double r) {
   double x = fabs(r);
   double s = (r > 0) - (r < 0);
   if (x >= 2.0) return 0.0;
@@ -440,7 +440,7 @@ static void axis_weights(const orc_ctx* c, int ax, int comp, double qpos, AxisW*
     for (int k = 0; k < 4; ++k) {
       double r = t - (double)(j0 - 1 + k);
       aw->w[k] = orc_delta4(r);
-      aw->dw[k] = ddelta4(r) / h;
+      aw->dw[k] = orc_ddelta4(r) / h;
     }
   } else {
     int j0 = (int)floor(t + 0.5);
@@ -762,26 +762,90 @@ static bool orc_read_fluid(orc_ctx* c, const char* path) {
   return ok;
 }
 
+// Initial fluid velocity, initfluid 0-4 (src/initializeFluid.cpp:31-160): background; equilibrium
+// fluctuations sqrt(kT/(rho dV)) N(0,1) per face (cell i: normals 0-2 of draw (seed, step 0,
+// STREAM_INIT_FLUID, sub 0, index i)) minus their mean plus the background; a transverse wave
+// vy0 + 0.04 sin(2 pi z/lz); a longitudinal wave 0.004 sin(16 pi (z + dz/2)/lz); vx = 0.425.
+static void initial_fluid(orc_ctx* c) {
+  const fhd_params& p = c->p;
+  const int N = c->N;
+  const double* b = p.background;
+  double sum[3] = {0, 0, 0};
+  for (int i = 0; i < N; ++i) {
+    const int iz = i / (c->mx * c->my);
+    const double z = (iz + 0.5) * c->dz - 0.5 * p.lz;
+    double v[3] = {b[0], b[1], b[2]};
+    if (p.initfluid == 1) {
+      double n[4];
+      orc_normals4(p.seed, 0, STREAM_INIT_FLUID, 0, (uint32_t)i, n);
+      const double s = sqrt(p.temperature / (p.densfluid * c->dV));
+      for (int a = 0; a < 3; ++a) { v[a] = s * n[a]; sum[a] += v[a]; }
+    } else if (p.initfluid == 2) {
+      v[1] = b[1] + 0.04 * sin(2.0 * M_PI / p.lz * z);
+    } else if (p.initfluid == 3) {
+      v[0] = 0.0; v[1] = 0.0;
+      v[2] = 0.004 * sin(2.0 * M_PI / p.lz * 8.0 * (z + 0.5 * p.lz / c->mz));
+    } else if (p.initfluid == 4) {
+      v[0] = 0.425; v[1] = 0.0; v[2] = 0.0;
+    }
+    for (int a = 0; a < 3; ++a) c->v[a][i] = v[a];
+  }
+  if (p.initfluid == 1)
+    for (int a = 0; a < 3; ++a) {
+      const double m = sum[a] / (double)N;
+      for (int i = 0; i < N; ++i) c->v[a][i] = c->v[a][i] - m + b[a];
+    }
+}
+
+// Particle positions (and optional velocities) from the coordinates / velocities files
+// (src/createParticles.cpp:81-120): "np" then np lines of three numbers each.
+static bool read_triples(const char* path, int* np, std::vector<double>* out) {
+  FILE* f = fopen(path, "r");
+  if (!f) return false;
+  int n = -1;
+  bool ok = fscanf(f, "%d", &n) == 1 && n >= 0 && (*np < 0 || n == *np);
+  if (ok) {
+    *np = n;
+    for (int a = 0; a < 3; ++a) out[a].assign(n, 0.0);
+    for (int i = 0; i < n && ok; ++i) ok = fscanf(f, "%lf %lf %lf", &out[0][i], &out[1][i], &out[2][i]) == 3;
+  }
+  fclose(f);
+  return ok;
+}
+
 extern "C" orc_ctx* orc_create(const fhd_params* p) {
   if (!p || p->mx <= 0 || p->my <= 0 || p->mz <= 0) return nullptr;
   orc_ctx* c = new orc_ctx();
   c->p = *p;
+  std::vector<double> qf[3], uf[3];
+  if (p->initial_positions == 2) {
+    int n = -1;
+    if (!read_triples(p->coordinates, &n, qf) || (p->velocities[0] && !read_triples(p->velocities, &n, uf))) {
+      delete c;
+      return nullptr;
+    }
+    c->p.np = n;
+  }
   c->mx = p->mx; c->my = p->my; c->mz = p->mz;
   c->N = p->mx * p->my * p->mz;
   c->dx = p->lx / p->mx; c->dy = p->ly / p->my; c->dz = p->lz / p->mz;
   c->dV = c->dx * c->dy * c->dz;
   c->fft.init(c->mx, c->my, c->mz);
-  for (int a = 0; a < 3; ++a) c->v[a].assign(c->N, p->initfluid == 0 ? p->background[a] : 0.0);
-  if (p->scheme == FHD_SCHEME_STOKES_LIMIT)
-    for (int a = 0; a < 3; ++a) c->v[a].assign(c->N, 0.0);
+  for (int a = 0; a < 3; ++a) c->v[a].assign(c->N, 0.0);
+  if (p->scheme != FHD_SCHEME_STOKES_LIMIT) initial_fluid(c);
   c->step = p->initial_step;
   if (p->fluid[0] && !orc_read_fluid(c, p->fluid)) {
     delete c;
     return nullptr;
   }
-  const int np = p->np;
+  const int np = c->p.np;
   for (int a = 0; a < 3; ++a) { c->q[a].assign(np, 0.0); c->F[a].assign(np, 0.0); c->u[a].assign(np, 0.0); }
-  if (np > 0) {
+  if (p->initial_positions == 2) {
+    for (int a = 0; a < 3; ++a) {
+      c->q[a] = qf[a];
+      if (!uf[a].empty()) c->u[a] = uf[a];
+    }
+  } else if (np > 0) {
     std::vector<double> q0((size_t)3 * np);
     orc_initial_positions(p, q0.data());
     for (int a = 0; a < 3; ++a)
@@ -796,6 +860,7 @@ extern "C" orc_ctx* orc_create(const fhd_params* p) {
 
 extern "C" void orc_destroy(orc_ctx* c) { delete c; }
 extern "C" long long orc_get_step(orc_ctx* c) { return c ? c->step : -1; }
+extern "C" int orc_get_np(orc_ctx* c) { return c ? c->p.np : -1; }
 
 extern "C" int orc_set_velocity(orc_ctx* c, const double* v) {
   for (int a = 0; a < 3; ++a)
@@ -873,6 +938,33 @@ extern "C" int orc_interp(orc_ctx* c, const double* q, const double* field, doub
     interp_one(c, qp, fld, up);
     for (int a = 0; a < 3; ++a) u[(size_t)a * np + i] = up[a];
   }
+  return 1;
+}
+
+extern "C" int orc_spread_drift(orc_ctx* c, const double* q, double scale, double* field) {
+  std::vector<double> fld[3];
+  for (int a = 0; a < 3; ++a) fld[a].assign(field + (size_t)a * c->N, field + (size_t)(a + 1) * c->N);
+  const int np = c->p.np;
+  for (int i = 0; i < np; ++i) {
+    double qp[3] = {q[i], q[np + i], q[2 * np + i]};
+    spread_drift_one(c, qp, scale, fld);
+  }
+  for (int a = 0; a < 3; ++a) memcpy(field + (size_t)a * c->N, fld[a].data(), sizeof(double) * c->N);
+  return 1;
+}
+
+extern "C" int orc_spread_quirk(orc_ctx* c, const double* q, const double* qd, const double* Fin, double scale,
+                                double* field) {
+  std::vector<double> fld[3];
+  for (int a = 0; a < 3; ++a) fld[a].assign(field + (size_t)a * c->N, field + (size_t)(a + 1) * c->N);
+  const int np = c->p.np;
+  for (int i = 0; i < np; ++i) {
+    double q0[3] = {q[i], q[np + i], q[2 * np + i]};
+    double qq[3] = {qd[i], qd[np + i], qd[2 * np + i]};
+    double Fp[3] = {Fin[i], Fin[np + i], Fin[2 * np + i]};
+    spread_quirk_one(c, q0, qq, Fp, scale, fld);
+  }
+  for (int a = 0; a < 3; ++a) memcpy(field + (size_t)a * c->N, fld[a].data(), sizeof(double) * c->N);
   return 1;
 }
 

@@ -51,6 +51,7 @@ int orc_fft3d(int mx, int my, int mz, double* data, int sign);
 double orc_delta3(double r);   /* 3-point Roma-Peskin kernel, r in grid units */
 double orc_delta4(double r);   /* 4-point Peskin kernel, r in grid units */
 double orc_ddelta3(double r);  /* derivative of the 3-point kernel */
+double orc_ddelta4(double r);  /* derivative of the 4-point kernel */
 
 orc_ctx* orc_create(const fhd_params* p);
 void orc_destroy(orc_ctx* c);
@@ -62,6 +63,7 @@ int orc_get_particle_velocities(orc_ctx* c, double* u);
 int orc_get_forces(orc_ctx* c, double* f);
 int orc_step(orc_ctx* c, int nsteps);
 long long orc_get_step(orc_ctx* c);
+int orc_get_np(orc_ctx* c);   /* numberparticles, or the count of the coordinates file */
 int orc_set_threads(int n);
 
 /* Building blocks on the context's grid / particles. */
@@ -69,6 +71,13 @@ int orc_wca(orc_ctx* c, const double* q, double* f);               /* q,f: 3*np 
 int orc_bonded(orc_ctx* c, const double* q, double* f);            /* harmonic bonds only, f = bond forces */
 int orc_spread(orc_ctx* c, const double* q, const double* F, double* field); /* field += S F (3*N) */
 int orc_interp(orc_ctx* c, const double* q, const double* field, double* u);  /* u = J field */
+/* field_c += scale * d/dq_c [phi phi phi] summed over the particles (the QNB thermal-drift
+ * spread without its kT m_f/(m_e+m_f)/dV factor, which the caller passes in scale). */
+int orc_spread_drift(orc_ctx* c, const double* q, double scale, double* field);
+/* field += scale * S_quirk F: stencil nodes chosen from the displaced positions qd, kernel
+ * weights evaluated at the undisplaced q (the reference's RFD spread, rfdMode 1). */
+int orc_spread_quirk(orc_ctx* c, const double* q, const double* qd, const double* F, double scale,
+                     double* field);
 int orc_divergence(orc_ctx* c, const double* v, double* div);     /* div: N */
 /* One incompressible solve: v_out = shift^-1 FFT^-1 [ M(k) shift FFT W ] / N with
  * M = (I - g g^T/|g|^2)/(1 - dt nu L/2) (mode 0 identity), as kernelUpdateVIncompressible. */

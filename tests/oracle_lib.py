@@ -30,13 +30,16 @@ def lib():
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.orc_interp.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_spread.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_spread_drift.argtypes = [C.c_void_p, dp, C.c_double, dp]
+        L.orc_spread_quirk.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, dp]
         L.orc_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_step.restype = C.c_longlong
         L.orc_get_step.argtypes = [C.c_void_p]
+        L.orc_get_np.argtypes = [C.c_void_p]
         L.orc_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_normals4.argtypes = [C.c_uint64, C.c_int64, C.c_int, C.c_int, C.c_uint32, dp]
         L.orc_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int]
-        for name in ("orc_delta3", "orc_delta4", "orc_ddelta3"):
+        for name in ("orc_delta3", "orc_delta4", "orc_ddelta3", "orc_ddelta4"):
             getattr(L, name).restype = C.c_double
             getattr(L, name).argtypes = [C.c_double]
         L.orc_initial_positions.argtypes = [C.POINTER(FhdParams), dp]
@@ -83,6 +86,10 @@ def ddelta3(r):
     return lib().orc_ddelta3(float(r))
 
 
+def ddelta4(r):
+    return lib().orc_ddelta4(float(r))
+
+
 def initial_positions(params):
     q = np.zeros(3 * params.np)
     lib().orc_initial_positions(C.byref(params), _p(q))
@@ -95,6 +102,7 @@ class Oracle:
         self.h = lib().orc_create(C.byref(self.p))
         if not self.h:
             raise RuntimeError("orc_create failed")
+        self.p.np = lib().orc_get_np(self.h)  # a coordinates file sets the count
         self.N = self.p.mx * self.p.my * self.p.mz
 
     def close(self):
@@ -157,6 +165,22 @@ class Oracle:
         F = np.ascontiguousarray(F, dtype=np.float64)
         out = np.zeros(3 * self.N) if field is None else np.ascontiguousarray(field, dtype=np.float64).copy()
         lib().orc_spread(self.h, _p(q), _p(F), _p(out))
+        return out
+
+    def spread_drift(self, q, scale):
+        """field_c = scale * sum_p d/dq_c [phi phi phi] (thermal-drift spread)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros(3 * self.N)
+        lib().orc_spread_drift(self.h, _p(q), float(scale), _p(out))
+        return out
+
+    def spread_quirk(self, q, qd, F, scale):
+        """scale * sum_p F_p phi(q - node) over the nodes the displaced qd selects."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        qd = np.ascontiguousarray(qd, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        out = np.zeros(3 * self.N)
+        lib().orc_spread_quirk(self.h, _p(q), _p(qd), _p(F), float(scale), _p(out))
         return out
 
     def interp(self, q, field):
