@@ -139,6 +139,8 @@ struct fhd_ctx {
   int zchunk = 8;                        // W kernel: z planes per workgroup (schedule)
   int zchunk_b = 8;                      // the same inside batched dispatches
   hipEvent_t done_ev = nullptr;          // fhd_record / fhd_query
+  bool done_checked = false;             // error word read since the last fhd_record completed
+  bool dev_error = false;                // fhd_query found a device error flag
   // Stokes: the stochastic part of W is built on a side stream while the particle kernels run
   hipStream_t s2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_noise = nullptr;
@@ -352,7 +354,7 @@ struct Sched {
   int btile, zchunk, zchunk_b;
 };
 std::mutex g_sched_mu;
-std::map<std::array<int, 3>, Sched> g_sched;
+std::map<std::array<int, 4>, Sched> g_sched;
 
 // Shortest of three timed rounds of reps launches (one warm-up launch first), so one slow
 // round does not decide the schedule.
@@ -377,7 +379,8 @@ float time_launches(fhd_ctx* c, const std::function<void()>& f, int reps) {
 }
 
 void apply_schedule(fhd_ctx* c) {
-  const std::array<int, 3> key = {c->p.mx, c->p.my, c->p.mz};
+  // keyed by shape and scheme: the W kernel is timed in the form the scheme launches
+  const std::array<int, 4> key = {c->p.mx, c->p.my, c->p.mz, c->p.scheme};
   const char* env = getenv("FHD_AUTOTUNE");  // 0: fixed defaults (8 columns, 8 planes)
   const bool tune = !(env && env[0] == '0');
   std::lock_guard<std::mutex> lock(g_sched_mu);
@@ -848,6 +851,19 @@ void graph_step(fhd_ctx* c) {
   if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) qnb_after_step(c);
   c->gphase = (c->gphase + 1) % c->gperiod;
   c->step++;
+}
+
+// The context's device error word (fluam's errorKernel, checked by its gpuToHost* calls,
+// src/allocateErrorArray.cu:22-32, src/gpuToHostStokesLimit.cu:23-30).  The flags are sticky: a
+// context that has flagged an error keeps reporting it.
+void check_device_flags(fhd_ctx* c) {
+  int e[8];
+  HIPCHK(hipMemcpy(e, c->errd, sizeof(e), hipMemcpyDeviceToHost));
+  if (e[0] & kErrNonFinite)
+    throw std::runtime_error("device error: non-finite particle displacement (a force or velocity overflowed)");
+  if (e[0] & kErrClamped)
+    throw std::runtime_error("device error: a particle displacement of 2^24 cells or more was not applied "
+                             "(blown-up step)");
 }
 
 void advance(fhd_ctx* c) {
@@ -1512,7 +1528,9 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
     } else if (p.bonded_forces[0]) {
       throw std::runtime_error("bondedForces needs particles");
     }
-    HIPCHK(hipDeviceSynchronize());
+    // the uploads above are synchronous; wait for the context's own stream (the schedule timing
+    // runs there), not for other contexts' queued work
+    HIPCHK(hipStreamSynchronize(c->s));
   } catch (const std::exception& e) {
     g_last_error = e.what();
     free_ctx(c);
@@ -1712,9 +1730,7 @@ int fhd_set_noise_overlap(fhd_ctx* c, int on) {
 int fhd_sync(fhd_ctx* c) {
   CTX_TRY(c, {
     HIPCHK(hipStreamSynchronize(c->s));
-    int e[8];
-    HIPCHK(hipMemcpy(e, c->errd, sizeof(e), hipMemcpyDeviceToHost));
-    if (e[0]) throw std::runtime_error("non-finite particle displacement detected on the device");
+    check_device_flags(c);
   });
   return 1;
 }
@@ -1973,17 +1989,33 @@ int fhd_record(fhd_ctx* c) {
   CTX_TRY(c, {
     if (!c->done_ev) HIPCHK(hipEventCreateWithFlags(&c->done_ev, hipEventDisableTiming));
     HIPCHK(hipEventRecord(c->done_ev, c->s));
+    c->done_checked = false;
   });
   return 1;
 }
 
+// Once the marked work has completed, the device error word is read once (a few bytes): a
+// context whose steps flagged an error reports -1 from then on, so a serving loop can evict it.
 int fhd_query(fhd_ctx* c) {
   if (!c || !c->done_ev) return 1;
+  if (c->dev_error) return -1;
   const hipError_t e = hipEventQuery(c->done_ev);
-  if (e == hipSuccess) return 1;
   if (e == hipErrorNotReady) return 0;
-  c->err = hipGetErrorString(e);
-  return -1;
+  if (e != hipSuccess) {
+    c->err = hipGetErrorString(e);
+    return -1;
+  }
+  if (!c->done_checked) {
+    c->done_checked = true;
+    try {
+      check_device_flags(c);
+    } catch (const std::exception& ex) {
+      c->err = ex.what();
+      c->dev_error = true;
+      return -1;
+    }
+  }
+  return 1;
 }
 
 fhd_batch* fhd_batch_create(const fhd_params* p, int n) {
@@ -2321,8 +2353,10 @@ void batch_advance(fhd_batch* b) {
       c->dev_step_valid = true;
       joined = true;
     }
-  if (b->mode == 2 && !b->timer_ahead && !joined) join_replicas_to_timer(b);
-  if (b->mode == 2) b->timer_ahead = true;
+  // the batch graph runs on the timer stream: order it after work queued on the replica streams
+  // (an eager step, a structure-factor sample, a velocity upload) and those streams after it
+  if (!b->timer_ahead && !joined) join_replicas_to_timer(b);
+  b->timer_ahead = true;
   HIPCHK(hipGraphLaunch(b->mode == 2 ? b->mexec[b->phase] : b->exec[b->phase], b->timer));
   for (fhd_ctx* c : b->reps) {
     if (c->p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) qnb_after_step(c);

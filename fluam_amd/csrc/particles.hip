@@ -552,8 +552,12 @@ FHD_KERNEL_PAIR((template <int NPTS>), 256, k_src_prep, SpreadArgs, body_src_pre
 // addend.  Integer sums are exact, so the accumulated field does not
 // depend on the order in which the contributions arrive.
 FHD_DEV unsigned long long fixed64(float x) {
+  // Finite sources scale to |x| <= 2^30.  A non-finite source (an overflowed force) saturates:
+  // +-inf to +-2^30 and NaN to -2^30 (fmaxf returns its non-NaN operand), so the conversion is
+  // always defined; the field it lands in overflows and the interpolation pass flags the step.
+  const float c = fminf(fmaxf(x, -1073741824.0f), 1073741824.0f);
   // truncation toward zero: at the 2^30 scale its bias (< 1 unit) is far below FP32 rounding
-  const int q = (int)x;
+  const int q = (int)c;
   return (unsigned long long)(long long)q;
 }
 
@@ -868,14 +872,20 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
     float4 bo = a.base_off[id];
     bool ok = isfinite(disp[0]) && isfinite(disp[1]) && isfinite(disp[2]);
     if (!ok) {
-      atomicOr(a.err, 1);
+      atomicOr(a.err, kErrNonFinite);
       disp[0] = disp[1] = disp[2] = 0.0f;
     }
     // a displacement of 2^24 cells or more along an axis (a blown-up step, e.g. two particles
-    // started on top of each other under the WCA core) leaves that coordinate where it was
-    if (!(fabsf(disp[0]) < 16777216.0f * g.dx)) disp[0] = 0.0f;
-    if (!(fabsf(disp[1]) < 16777216.0f * g.dy)) disp[1] = 0.0f;
-    if (!(fabsf(disp[2]) < 16777216.0f * g.dz)) disp[2] = 0.0f;
+    // started on top of each other under the WCA core) leaves that coordinate where it was, and
+    // the step is flagged
+    const bool big = !(fabsf(disp[0]) < 16777216.0f * g.dx) || !(fabsf(disp[1]) < 16777216.0f * g.dy) ||
+                     !(fabsf(disp[2]) < 16777216.0f * g.dz);
+    if (big) {
+      atomicOr(a.err, kErrClamped);
+      if (!(fabsf(disp[0]) < 16777216.0f * g.dx)) disp[0] = 0.0f;
+      if (!(fabsf(disp[1]) < 16777216.0f * g.dy)) disp[1] = 0.0f;
+      if (!(fabsf(disp[2]) < 16777216.0f * g.dz)) disp[2] = 0.0f;
+    }
     float ox = bo.x + disp[0], oy = bo.y + disp[1], oz = bo.z + disp[2];
     renorm_axis(bc.x, ox, g.dx, g.idx);
     renorm_axis(bc.y, oy, g.dy, g.idy);

@@ -16,7 +16,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import Simulation, bytes_needed, mem_info
+from . import FhdError, Simulation, bytes_needed, mem_info
 from ._abi import FhdParams
 
 
@@ -31,6 +31,7 @@ class Job:
     finished: Optional[float] = None
     velocity: Optional[np.ndarray] = None
     particles: Optional[np.ndarray] = None
+    error: Optional[str] = None          # set when the job was evicted on a device error
     _sim: Optional[Simulation] = field(default=None, repr=False)
     _issued: int = 0
     _mark: int = 0
@@ -89,26 +90,40 @@ class Server:
             # the others) until the GPU has reached that marker.
             progressed = False
             for j in active:
-                if j._issued >= j.steps:
+                if j._issued >= j.steps or j.error:
                     continue
-                if j._issued - j._mark >= self.max_inflight:
-                    if not j._sim.done():
-                        continue
-                    j._sim.record()
-                    j._mark = j._issued
-                j._sim.step(1)
+                try:
+                    if j._issued - j._mark >= self.max_inflight:
+                        if not j._sim.done():
+                            continue
+                        j._sim.record()
+                        j._mark = j._issued
+                    j._sim.step(1)
+                except FhdError as e:
+                    # a device error flagged by this job's steps (fluam's errorKernel): the job is
+                    # evicted at once; the others keep running
+                    j.error = str(e)
+                    continue
                 j._issued += 1
                 progressed = True
                 if j._issued == j.steps:
                     j._sim.record()
                     j._mark = j._issued
-            # retire jobs whose last step has completed
+            # retire jobs whose last step has completed, and evicted ones
             still = []
             for j in active:
-                if j._issued == j.steps and j._sim.done():
-                    j._sim.sync()
+                finished = j.error is not None
+                if not finished and j._issued == j.steps:
+                    try:
+                        finished = j._sim.done()
+                        if finished:
+                            j._sim.sync()
+                    except FhdError as e:
+                        j.error = str(e)
+                        finished = True
+                if finished:
                     j.finished = time.perf_counter() - t0
-                    if self.keep_results:
+                    if self.keep_results and not j.error:
                         j.velocity = j._sim.velocity()
                         if j.params.np > 0:
                             j.particles = j._sim.particles()
@@ -125,9 +140,10 @@ class Server:
             if not active and pending:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
         wall = time.perf_counter() - t0
-        lat = np.array([j.latency for j in done])
-        total = sum(j.cell_updates for j in done)
-        return {"jobs": len(done), "wall_s": wall, "cell_updates_per_s": total / wall,
+        ok = [j for j in done if not j.error]
+        lat = np.array([j.latency for j in ok]) if ok else np.array([0.0])
+        total = sum(j.cell_updates for j in ok)
+        return {"jobs": len(ok), "evicted": len(done) - len(ok), "wall_s": wall, "cell_updates_per_s": total / wall,
                 "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
                 "mean_latency_s": float(lat.mean())}
 

@@ -127,7 +127,10 @@ int fhd_get_particles(fhd_ctx* c, double* q);
 int fhd_get_particle_velocities(fhd_ctx* c, double* u);  /* J v at the last step (QNB) */
 int fhd_get_forces(fhd_ctx* c, double* f);              /* last particle forces, 3*np */
 int fhd_step(fhd_ctx* c, int nsteps);                    /* enqueue nsteps steps (async) */
-int fhd_sync(fhd_ctx* c);                                /* wait, then check device error flags */
+/* Wait for the context's queued work, then check its device error flags (fluam's errorKernel):
+ * 0 with the reason in fhd_ctx_error when a step produced a non-finite particle displacement or
+ * one of 2^24 cells or more (not applied).  The flags are sticky. */
+int fhd_sync(fhd_ctx* c);
 /* Replay the steady-state step from captured HIP graphs (on != 0) instead of launching each
  * kernel.  No fluam counterpart (fluam launches every kernel each step); results are identical. */
 int fhd_set_graph(fhd_ctx* c, int on);
@@ -152,8 +155,10 @@ long long fhd_get_step(const fhd_ctx* c);
  * (loadparticles 1, src/createParticles.cpp:88); -1 for a null context. */
 int fhd_get_np(const fhd_ctx* c);
 /* Time nsteps steps with HIP events on the context's stream: total ms into *ms.
- * Per-kernel average durations (ms) for up to nkern kernels into kern_ms, their names
- * into kern_names (each up to 47 chars + NUL, 48-byte stride); returns kernels reported. */
+ * fhd_profile_kernels runs nsteps steps with HIP events around each kernel group and reports up
+ * to nkern groups: kern_ms must hold 2*nkern doubles, filled with (mean ms per launch, launch
+ * count) pairs, kern_ms[2j] and kern_ms[2j+1] for group j; kern_names must hold 48*nkern bytes,
+ * group j's NUL-terminated name at kern_names + 48*j.  Returns the number of groups reported. */
 int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms);
 int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern);
 
@@ -169,7 +174,8 @@ long long fhd_bytes_needed(const fhd_params* p);
 /* Free and total device memory of the current GPU. */
 int fhd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
 /* Mark the work enqueued so far on the context; fhd_query returns 1 once it has completed,
- * 0 while it runs, -1 on a device error. */
+ * 0 while it runs, -1 on a device error (a HIP error, or a device error flag of fhd_sync's
+ * kinds raised by the completed work; sticky). */
 int fhd_record(fhd_ctx* c);
 int fhd_query(fhd_ctx* c);
 /* n replicas (own parameters, shapes and seeds), each on its own stream with graph-replayed

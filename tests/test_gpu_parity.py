@@ -397,15 +397,21 @@ def test_blown_up_start_moves_no_coordinate_by_2p24_cells():
     _announce("blown_up_start", p)
     g = F.Simulation(p)
     q = g.particles()
+    flagged = False
     for _ in range(4):
         g.step(1)
-        g.sync()
+        try:
+            g.sync()
+        except F.FhdError as e:  # the clamped displacement is reported, as fluam's errorKernel would
+            assert "2^24" in str(e)
+            flagged = True
         qn = g.particles()
         assert np.all(np.isfinite(qn))
         assert np.abs(qn - q).max() < 2.0 ** 24
         q = qn
     pos = np.stack([q[:2048], q[2048:4096], q[4096:]], axis=1)
     assert len(np.unique(pos, axis=0)) == 2048
+    assert flagged
 
 
 def test_overlap_free_random_start_matches_oracle():

@@ -105,6 +105,27 @@ def test_jobs_overlap_continuous_batching(fake):
     assert short.finished < long_.finished
 
 
+def test_device_error_evicts_only_that_job(fake, monkeypatch):
+    """A job whose steps raise a device error flag (fhd_query reports -1, raised as FhdError by
+    done()) is evicted with the reason recorded; the other jobs run to completion."""
+    class Failing(FakeSim):
+        def done(self):
+            if self.p.mx == 24 and self.steps >= 8:
+                raise F.FhdError("device error: non-finite particle displacement")
+            return super().done()
+
+    monkeypatch.setattr(S, "Simulation", Failing)
+    jobs = [_job("a", 16, 40, 0.0), _job("bad", 24, 40, 0.0), _job("c", 16, 30, 0.001)]
+    r = S.Server(budget=10 ** 9, keep_results=True, max_inflight=2).run(jobs)
+    a, bad, c = jobs
+    assert bad.error and "non-finite" in bad.error
+    assert bad.velocity is None and bad.finished is not None
+    assert a.error is None and c.error is None
+    assert a.velocity[0] == 40 and c.velocity[0] == 30
+    assert r["jobs"] == 2 and r["evicted"] == 1
+    assert Failing.live == 0
+
+
 def test_poisson_arrivals_are_increasing():
     jobs = S.poisson_jobs(50.0, 20, 3, lambda i, rng: (make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16)), 5))
     t = [j.arrival for j in jobs]
