@@ -26,21 +26,26 @@ def algorithmic_bytes(N, npart):
 
 
 def kernel_bytes(name, mx, my, mz, npart):
-    """Algorithmic HBM bytes of one launch of a kernel (FP32, 3 components)."""
+    """Algorithmic HBM bytes of one launch of a kernel group: the per-pass figures of the
+    whole-step formula (FP32, 3 components, per Stokes solve), split by the kernel that does the
+    pass.  Per cell: spread writes 12 B; the W + xy-forward pass reads 12 and writes 12 (its noise
+    is generated in registers); the z pass reads and writes the half spectrum, 24; the xy-inverse
+    pass 24; the interpolation reads 12.  Per particle: sort 32, force 24, spread 24,
+    interpolation and update 24.  Passes the formula does not charge (the separate noise pass,
+    the spreading records) get 0 B; their cost is time the fused pipeline would not spend."""
     N = mx * my * mz
-    P = mx // 2 + 1
-    spec = 3 * mz * my * P * 8
-    if name == "fft_z_solve":
-        return 2 * spec                      # read + write the half spectrum
-    if name == "fft_xy_fwd":
-        return 3 * N * 4 + spec              # read real W, write half spectrum
-    if name == "fft_xy_inv":
-        return spec + 3 * N * 4              # read half spectrum, write real v
-    if name == "fluid_w":
-        return 3 * N * 4                     # Stokes: write W = D.Sigma (noise in registers)
-    if name == "spread":
-        return 2 * 3 * N * 4 + 32 * npart    # read W, write W + S F, read sorted particles
-    return None
+    per = {
+        "spread": 12 * N + 24 * npart,
+        "fft_xy_fwd": 24 * N,
+        "fft_z_solve": 24 * N,
+        "fft_xy_inv": 24 * N,
+        "interp": 12 * N + 24 * npart,
+        "sort": 32 * npart,
+        "wca": 24 * npart,
+        "fluid_w": 0,
+        "src_prep": 0,
+    }
+    return per.get(name)
 
 
 def hbm_traffic(name):
@@ -56,29 +61,69 @@ def hbm_traffic(name):
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(args):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    threads = O.lib().orc_set_threads(0)
-    p = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
-                    dt=0.05, seed=3, initial_positions=3, cutoff=2 ** (1 / 6), wca_sigma=1.0)
+
+def _time_oracle(O, p, threads, max_steps, budget_s, all_threads):
+    """Seconds per step of the FP64 oracle on `threads` OpenMP threads: one untimed step (on all
+    threads), then steps until max_steps or budget_s."""
+    O.lib().orc_set_threads(all_threads)
     o = O.Oracle(p)
-    o.step(1)  # warm-up (page-in, plans)
+    o.step(1)
+    O.lib().orc_set_threads(threads)
     t0 = time.perf_counter()
     n = 0
-    while True:
+    while n < max_steps and (n == 0 or time.perf_counter() - t0 < budget_s):
         o.step(1)
         n += 1
-        if time.perf_counter() - t0 > args.cpu_seconds or n >= 4:
-            break
-    dt = (time.perf_counter() - t0) / n
+    return (time.perf_counter() - t0) / n, n
+
+
+def cpu_baseline(args):
+    """The FP64 oracle (the reference's discretisation restated on the CPU) timed on this host:
+    the benchmarked configuration on all host threads (the reported value) and on one thread,
+    and the incompressible 32^3 and inertial-coupling 64^3 configurations likewise.  Bounded
+    samples of a few steps each (about 25 s in all)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    from fluam_amd._abi import make_params, SCHEME_STOKES_LIMIT, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_NEUTRALLY_BUOYANT
+
+    all_threads = O.lib().orc_set_threads(0)
+    wca = dict(cutoff=2 ** (1 / 6), wca_sigma=1.0)
+    p3 = make_params(SCHEME_STOKES_LIMIT, cells=(128, 128, 128), np=65536, kernel_points=4, temperature=1e-3,
+                     dt=0.05, seed=3, initial_positions=3, **wca)
+    p1 = make_params(SCHEME_INCOMPRESSIBLE, cells=(32, 32, 32), temperature=1e-3, dt=0.05, seed=11)
+    p2 = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(64, 64, 64), np=4096, kernel_points=3, temperature=1e-3,
+                     dt=0.05, seed=5, initial_positions=3, **wca)
+    budget = args.cpu_seconds
+    dt_all, n_all = _time_oracle(O, p3, all_threads, 4, 0.25 * budget, all_threads)
+    dt_one, n_one = _time_oracle(O, p3, 1, 1, 0.0, all_threads)
+    others = {}
+    for name, p, steps in (("incompressible 32^3", p1, 40), ("inertial coupling (mass 0) 64^3, 4096 particles", p2, 10)):
+        N = p.mx * p.my * p.mz
+        a, na = _time_oracle(O, p, all_threads, steps, 0.05 * budget, all_threads)
+        s1, n1 = _time_oracle(O, p, 1, max(1, steps // 4), 0.05 * budget, all_threads)
+        others[name] = {"cell_updates_per_s_all_threads": N / a, "cell_updates_per_s_1_thread": N / s1,
+                        "s_per_step_all_threads": a, "s_per_step_1_thread": s1, "timed_steps": [na, n1]}
+    O.lib().orc_set_threads(all_threads)
     N = 128 ** 3
-    return {"value": N / dt, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
-            "particle_steps_per_s": 65536 / dt,
-            "sample": f"FP64 oracle, Stokes 128^3 + 65536 particles, {n} timed steps after 1 warm-up, "
-                      f"OpenMP {threads} threads, {dt:.2f} s/step"}
+    return {"value": N / dt_all, "unit": "cell-updates/s", "cores": int(all_threads), "kind": "port",
+            "particle_steps_per_s": 65536 / dt_all,
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "host_cpus_usable": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "one_thread": {"value": N / dt_one, "s_per_step": dt_one, "timed_steps": n_one},
+            "other_configs": others,
+            "sample": f"FP64 oracle, Stokes 128^3 + 65536 particles, {n_all} timed steps after 1 warm-up on "
+                      f"{all_threads} OpenMP threads ({dt_all:.2f} s/step) and {n_one} on 1 thread "
+                      f"({dt_one:.1f} s/step)"}
 
 
 def plain_back_to_back(F, params_steps):
@@ -294,10 +339,20 @@ def main():
     psps = npart * args.steps / (ev_ms * 1e-3)
     # per-kernel times (HIP events around each launch group, same stream) over a short window
     prof = sim.profile(min(args.steps, 10))
-    dom = max((k for k in prof if kernel_bytes(k, mx, my, mz, npart) is not None), key=lambda k: prof[k][0] * prof[k][1])
+    # the dominant kernel: the largest share of the step among the kernels the formula charges
+    dom = max((k for k in prof if kernel_bytes(k, mx, my, mz, npart)), key=lambda k: prof[k][0] * prof[k][1])
     kb = kernel_bytes(dom, mx, my, mz, npart)
     kms = prof[dom][0]
     achieved = kb / (kms * 1e-3) / 1e9
+    per_kernel = {}
+    for k, (k_ms, k_calls) in prof.items():
+        b = kernel_bytes(k, mx, my, mz, npart)
+        t = hbm_traffic(k)
+        per_kernel[k] = {"avg_launch_ms": round(k_ms, 5), "launches_per_step": k_calls / min(args.steps, 10),
+                         "algorithmic_bytes": b,
+                         "frac": None if not b else round(b / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic_bytes": t,
+                         "traffic_over_algorithmic": None if (t is None or not b) else round(t / b, 2)}
     step_gbs = algorithmic_bytes(N, npart) / (ms_per_step * 1e-3) / 1e9
     line = {
         "metric": "fluid cell-updates/s + IB particle-steps/s; batched-ensemble p50/p99 latency",
@@ -318,13 +373,15 @@ def main():
         "config": {"workload": "stokesLimit midpoint step, 128^3 periodic staggered grid, 65536 particles, "
                                "4-point Peskin kernel, WCA sigma=1 rc=2^(1/6) eps=kT, kT=1e-3, eta=1, dt=0.05",
                    "cells": [mx, my, mz], "particles": npart,
+                   "initial_positions": "seeded uniform-random by random sequential addition (no pair closer "
+                                        "than sigma; data.main initialPositions 3)",
                    "launch": "HIP graph replay of the step" if args.graph else "eager launches"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic(dom),
                      "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},
         "step_roofline": {"algorithmic_bytes": algorithmic_bytes(N, npart), "achieved_GBs": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
-        "kernels_ms": {k: round(v[0], 5) for k, v in prof.items()},
+        "kernels": per_kernel,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args),
     }
     print(json.dumps(line))
