@@ -161,7 +161,7 @@ FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, cons
     ezs[i] = a.ez[i];
     gzs[i] = a.gz[i];
   }
-  const int tot = 3 * mz * T;
+  const int tot = a.ncomp * mz * T;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int c = e / (mz * T), r = e - c * mz * T, z = r / T, t = r - z * T;
     int kx = kx0 + t;
@@ -170,7 +170,7 @@ FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, cons
     buf[e] = val;
   }
   __syncthreads();
-  LineMap zl = {3 * T, T, mz * T, 1, T};
+  LineMap zl = {a.ncomp * T, T, mz * T, 1, T};
   if (a.zmode != 2) lds_fft(buf, zl, mz, a.plan_z, a.nst_z, twz, 1, -1.0f);
   if (a.zmode != 0) {
     // transform-only modes (building blocks for the parity tests): no operator, no scaling
@@ -279,7 +279,7 @@ FHD_KERNEL_PAIR((), kThreads, k_fft_xy_inv, FftPass, body_fft_xy_inv(a.f, a.spec
 void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s) {
   if (!a.generic && fast_fft_xy_fwd(a, in, spec, s)) return;
   const FftPass x{a, in, spec, nullptr, nullptr, 0.0f, 0.0f, 0};
-  launch(k_fft_xy_fwd, k_fft_xy_fwd_b, FAM_FFT_XY_FWD, dim3(a.g.mz, 3), kThreads, x, s);
+  launch(k_fft_xy_fwd, k_fft_xy_fwd_b, FAM_FFT_XY_FWD, dim3(a.g.mz, a.ncomp), kThreads, x, s);
 }
 void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s) {
   if (!a.generic && fast_fft_z_solve(a, spec, s)) return;
@@ -290,7 +290,7 @@ void launch_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const f
                        hipStream_t s) {
   if (!a.generic && fast_fft_xy_inv(a, spec, out, aux, alpha, beta, s)) return;
   const FftPass x{a, nullptr, const_cast<float2*>(spec), out, aux, alpha, beta, 0};
-  launch(k_fft_xy_inv, k_fft_xy_inv_b, FAM_FFT_XY_INV, dim3(a.g.mz, 3), kThreads, x, s);
+  launch(k_fft_xy_inv, k_fft_xy_inv_b, FAM_FFT_XY_INV, dim3(a.g.mz, a.ncomp), kThreads, x, s);
 }
 
 }  // namespace fhd

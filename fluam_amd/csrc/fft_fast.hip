@@ -197,14 +197,14 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
   }
   const size_t cstride = (size_t)my * MZ * P;
   float2* base = spec + (size_t)ky * MZ * P;
-  const int tot = 3 * MZ * TC;
+  const int tot = a.ncomp * MZ * TC;
   for (int e = threadIdx.x; e < tot; e += NT) {
     const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
     const int kx = t < T ? tile * T + t : ntile * T + tile;
     buf[(c * TM + t) * LS + z] = base[c * cstride + (size_t)z * P + kx];
   }
   __syncthreads();
-  const int lines = 3 * TM;
+  const int lines = a.ncomp * TM;
   if (a.zmode != 2) ffft<MZ, 1, NT, 3 * TM, 1>(buf, lines, LS, twz, -1.0f);
   if (a.zmode == 0) {
     const float gy = a.gy[ky];
@@ -254,8 +254,8 @@ FHD_KERNEL_PAIR((template <int MZ, int NT, int TM>), NT, k_fftB, FftPass, body_f
 // ---------------- dispatch ----------------
 template <int MX, int MY, int NT>
 static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
-  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(mz, 3), NT, x, s);
-  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(mz, 3), NT, x, s);
+  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(mz, x.f.ncomp), NT, x, s);
+  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
 }
 // Lanes per plane workgroup.  A single context's pass has only 3 mz plane workgroups, and more
 // lanes per plane cover more of the plane load and store latency: 1024 from 64^2 up (128^3:

@@ -83,6 +83,68 @@ T* dalloc(size_t n) {
 }  // namespace
 
 // ------------------------------------------------------------------------------------------
+// Device scratch of the no-allocation forward transform (fhd_fft_r2c_noalloc).
+// ------------------------------------------------------------------------------------------
+namespace {
+// one component of a 128^3 half spectrum (8.5 MB), then the FFT tables
+constexpr size_t kNoAllocSpecMax = (size_t)128 * 128 * 65;
+constexpr size_t kNoAllocTab = 8 * (size_t)kMaxFftLen;  // 6 complex tables + 3 real ones (float2 units)
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) float2 scratch[kNoAllocSpecMax + kNoAllocTab];
+
+namespace {
+std::mutex g_noalloc_mu;
+float2* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Address of this GPU's copy of `scratch`, taken on the first call on each device (with that
+// device current).  Host code uses this address, never the array's name.
+float2* noalloc_scratch() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw std::runtime_error("hipGetSymbolAddress returned a null address");
+    g_noalloc_ptr[dev] = static_cast<float2*>(ptr);
+  }
+  return g_noalloc_ptr[dev];
+}
+
+// The transform tables of build_tables, computed on the device from the same FP64 formulas:
+// twiddles e^{-2 pi i k/m} (x, y, z), shifts e^{i pi k/m} (x, y, z), then the real symbols
+// g(k) = 2 sin(pi k/m) (unit cells), kMaxFftLen entries per table.
+__global__ void k_noalloc_tables(float2* tab, int mx, int my, int mz) {
+  const int k = threadIdx.x;
+  const int m[3] = {mx, my, mz};
+  float* gt = reinterpret_cast<float*>(tab + 6 * kMaxFftLen);
+  for (int ax = 0; ax < 3; ++ax) {
+    if (k >= m[ax]) continue;
+    const double a = -2.0 * M_PI * (double)k / (double)m[ax];
+    tab[ax * kMaxFftLen + k] = make_float2((float)cos(a), (float)sin(a));
+    const double e = M_PI * (double)k / (double)m[ax];
+    tab[(3 + ax) * kMaxFftLen + k] = make_float2((float)cos(e), (float)sin(e));
+    gt[ax * kMaxFftLen + k] = (float)(2.0 / 1.0 * sin(M_PI * (double)k / (double)m[ax]));
+  }
+}
+
+// Half spectrum from the passes' layout [ky][z][kx] to the caller's [z][ky][kx].
+__global__ void k_noalloc_to_host_layout(const float2* __restrict__ spec, float2* __restrict__ out, int my, int mz,
+                                         int P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ns = (long long)mz * my * P;
+  if (i >= ns) return;
+  const int kx = (int)(i % P);
+  const long long r = i / P;
+  const int ky = (int)(r % my), z = (int)(r / my);
+  out[i] = spec[((long long)ky * mz + z) * P + kx];
+}
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
 // Context.
 // ------------------------------------------------------------------------------------------
 struct fhd_ctx {
@@ -260,6 +322,7 @@ void build_tables(fhd_ctx* c) {
   a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
   a.btile = 8;
   a.zmode = 0;
+  a.ncomp = 3;
   const char* gen = getenv("FHD_FFT_GENERIC");  // 1: use the runtime-size FFT kernels everywhere
   a.generic = (gen && gen[0] == '1') ? 1 : 0;
   a.stokes = 0;
@@ -1877,6 +1940,68 @@ int fhd_fft_c2r(int mx, int my, int mz, const double* in_spec, double* out) {
   }
   fhd_destroy(c);
   return ok;
+}
+
+// ---------------- the no-allocation forward transform ----------------
+// fhd_fft_r2c_noalloc works in a file-scope device array instead of allocated memory: the half
+// spectrum in the passes' [ky][z][kx] layout, then the transform tables.  Each GPU holds its own
+// copy until the process ends; its address is taken once per device with hipGetSymbolAddress
+// and only that address is handed to kernels.
+int fhd_fft_r2c_noalloc(int mx, int my, int mz, const float* in, float* out_spec, void* stream) {
+  try {
+    if (!in || !out_spec) throw std::runtime_error("fhd_fft_r2c_noalloc: null buffer");
+    fhd_params p;
+    fhd_default_params(&p);
+    p.scheme = FHD_SCHEME_STOKES_LIMIT;
+    p.mx = mx; p.my = my; p.mz = mz;
+    p.lx = mx; p.ly = my; p.lz = mz;
+    p.dt = 1.0;
+    validate(p);
+    const int P = mx / 2 + 1;
+    if ((size_t)mz * my * P > kNoAllocSpecMax)
+      throw std::runtime_error("fhd_fft_r2c_noalloc: mz * my * (mx/2 + 1) exceeds " +
+                               std::to_string(kNoAllocSpecMax) + " (a 128^3 grid)");
+    float2* sc = noalloc_scratch();
+    float2* spec = sc;
+    float2* tab = sc + kNoAllocSpecMax;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_noalloc_tables, dim3(1), dim3(kMaxFftLen), 0, s, tab, mx, my, mz);
+    HIPCHK(hipGetLastError());
+    FftArgs a;
+    memset(&a, 0, sizeof(a));
+    GridDev& g = a.g;
+    g.mx = mx; g.my = my; g.mz = mz;
+    g.N = mx * my * mz;
+    g.P = P;
+    g.dx = g.dy = g.dz = 1.0f;
+    g.idx = g.idy = g.idz = 1.0f;
+    g.lx = (float)mx; g.ly = (float)my; g.lz = (float)mz;
+    a.twx = tab; a.twy = tab + kMaxFftLen; a.twz = tab + 2 * kMaxFftLen;
+    a.ex = tab + 3 * kMaxFftLen; a.ey = tab + 4 * kMaxFftLen; a.ez = tab + 5 * kMaxFftLen;
+    const float* gt = reinterpret_cast<const float*>(tab + 6 * kMaxFftLen);
+    a.gx = gt; a.gy = gt + kMaxFftLen; a.gz = gt + 2 * kMaxFftLen;
+    make_plan(mx / 2, &a.plan_h, &a.nst_h);
+    make_plan(my, &a.plan_y, &a.nst_y);
+    make_plan(mz, &a.plan_z, &a.nst_z);
+    a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
+    a.btile = 8;
+    a.zmode = 1;
+    a.ncomp = 1;
+    const char* gen = getenv("FHD_FFT_GENERIC");
+    a.generic = (gen && gen[0] == '1') ? 1 : 0;
+    a.eta = 1.0f;
+    a.inv_n = (float)(1.0 / (double)g.N);
+    launch_fft_xy_fwd(a, in, spec, s);
+    launch_fft_z_solve(a, spec, s);
+    const long long ns = (long long)mz * my * P;
+    hipLaunchKernelGGL(k_noalloc_to_host_layout, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s,
+                       (const float2*)spec, reinterpret_cast<float2*>(out_spec), my, mz, P);
+    HIPCHK(hipGetLastError());
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 0;
+  }
+  return 1;
 }
 
 int fhd_normals(unsigned long long seed, long long step, int stream, int sub, int n, double* out) {

@@ -19,6 +19,7 @@ struct FftArgs {
   int ztile;                      // kx columns per pass-B workgroup (generic kernels)
   int btile;                      // kx columns per pass-B workgroup (size-specialised: 2, 4, 8 or 16)
   int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
+  int ncomp;                      // velocity components transformed (3; 1 for a single field, transform-only modes)
   int generic;                    // 1: force the generic (runtime-size) kernels
   int stokes;                     // 1: v = P W / (eta |g|^2), mode 0 -> 0
   float eta;

@@ -202,6 +202,17 @@ int fhd_batch_time_steps(fhd_batch* b, int nsteps, double* ms);
  * (mz,my,mx/2+1) complex (interleaved re,im) -> back; unnormalised both ways. */
 int fhd_fft_r2c(int mx, int my, int mz, const double* in, double* out_spec);
 int fhd_fft_c2r(int mx, int my, int mz, const double* in_spec, double* out);
+/* No-allocation variant of fhd_fft_r2c.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  in: device pointer to a real FP32 field (mz,my,mx), x fastest;
+ * out_spec: device pointer to (mz,my,mx/2+1) complex FP32 values, interleaved re,im;
+ * unnormalised, the transform of fhd_fft_r2c.  Enqueued on `stream` (a hipStream_t; NULL is the
+ * null stream) without waiting for it.  It allocates nothing: the work space is a file-scope
+ * device array of 128*128*65 complex values plus tables (about 8.5 MB, held by each GPU it runs
+ * on until the process ends), so mz*my*(mx/2+1) must not exceed 128*128*65 (a 128^3 grid) or the
+ * call fails.  Every call on one GPU uses that GPU's one array: calls that overlap on a GPU (on
+ * different streams or host threads) race.  1 on success, 0 with fhd_last_error() on failure. */
+int fhd_fft_r2c_noalloc(int mx, int my, int mz, const float* in, float* out_spec, void* stream);
 /* Counter-based normal generator used for every noise term (Philox4x32-10 + Box-Muller).
  * Writes n normals of (seed, step, stream, index=0..n/4, sub) as the kernels draw them. */
 int fhd_normals(unsigned long long seed, long long step, int stream, int sub, int n, double* out);
