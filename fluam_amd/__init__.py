@@ -20,7 +20,8 @@ __all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "ge
            "read_fluid_file", "write_fluid_file"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "lib", "libfhd.so")
+# FLUAM_AMD_LIB names another build of the library (for example one installed elsewhere)
+LIB_PATH = os.environ.get("FLUAM_AMD_LIB") or os.path.join(PKG, "lib", "libfhd.so")
 _lib = None
 
 

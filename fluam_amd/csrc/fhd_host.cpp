@@ -168,7 +168,14 @@ struct fhd_ctx {
   float4* off[2] = {nullptr, nullptr};
   float4* F[2] = {nullptr, nullptr};
   float4* u = nullptr;
-  int *count = nullptr, *slot = nullptr, *tmp = nullptr;
+  int *count = nullptr, *slot = nullptr, *tmp = nullptr;  // count: 2 x ntiles (one per position set)
+  int* slot_of = nullptr;  // sorted slot of each particle in set 0
+  int* sflag = nullptr;    // [k]: position set k must be sorted before its next use
+  // inertial coupling with excess mass: a field of work space and per-particle vectors by id
+  float* M1 = nullptr;     // the projected correction dv~, then 1/2 dt nu L dv
+  float4 *dulag = nullptr, *du = nullptr, *dp = nullptr, *jvt = nullptr, *ms = nullptr;
+  float skin[3] = {-1, -1, -1};  // cells a listed particle may stray outside its tile (-1: any)
+  float skin_len = 0.0f;
   int* start[2] = {nullptr, nullptr};
   int* sid[2] = {nullptr, nullptr};
   int4* scell[2] = {nullptr, nullptr};
@@ -255,8 +262,13 @@ void validate(const fhd_params& p) {
     if (p.kernel_points != 3 && p.kernel_points != 4) throw std::runtime_error("kernel must be 3 or 4");
     if (tile_edge(p.mx) < 0 || tile_edge(p.my) < 0 || tile_edge(p.mz) < 0)
       throw std::runtime_error("with particles, cells per axis must be a multiple of 4 or at most 8");
-    if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT && p.mass != 0.0)
-      throw std::runtime_error("quasiNeutrallyBuoyant supports mass 0 (neutrally buoyant) only");
+    if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT) {
+      // the particle's total mass m_e + m_f, m_f = 1.5 dV volumeParticle rho, must be positive
+      const double mf = 1.5 * (p.lx / p.mx) * (p.ly / p.my) * (p.lz / p.mz) * p.volume_particle * p.densfluid;
+      if (!(p.volume_particle > 0.0)) throw std::runtime_error("volumeParticle must be positive");
+      if (!std::isfinite(p.mass) || !(p.mass + mf > 0.0))
+        throw std::runtime_error("mass (the excess mass) must be finite and above -1.5 dV volumeParticle rho");
+    }
     if (p.nonbonded && p.cutoff > 0) {
       double L[3] = {p.lx, p.ly, p.lz};
       int m[3] = {p.mx, p.my, p.mz};
@@ -336,7 +348,8 @@ void free_ctx(fhd_ctx* c) {
                   c->ex, c->ey, c->ez, c->gx, c->gy, c->gz, c->cell[0], c->cell[1], c->off[0], c->off[1],
                   c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
                   c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
-                  c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo, c->sk_acc};
+                  c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo, c->sk_acc,
+                  c->slot_of, c->sflag, c->M1, c->dulag, c->du, c->dp, c->jvt, c->ms};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -507,7 +520,10 @@ void sort_set(fhd_ctx* c, int k) {
   a.t = c->t;
   a.cell = c->cell[k];
   a.off = c->off[k];
-  a.count = c->count;
+  a.count = c->count + (size_t)k * c->t.n;
+  a.count_other = c->count + (size_t)(1 - k) * c->t.n;
+  a.gate = c->sflag + k;
+  a.slot_of = k == 0 ? c->slot_of : nullptr;
   a.slot = c->slot;
   a.start = c->start[k];
   a.tmp = c->tmp;
@@ -557,6 +573,7 @@ void wca(fhd_ctx* c, int k, float4* F) {
   a.rc2 = (float)(c->p.cutoff * c->p.cutoff);
   a.sig2 = (float)(c->p.wca_sigma * c->p.wca_sigma);
   a.eps48 = (float)(48.0 * c->p.wca_epsilon);
+  a.skin_len = c->skin_len;
   a.F = F;
   a.np = c->np;
   launch_wca(a, c->s);
@@ -598,6 +615,36 @@ InterpArgs interp_base(fhd_ctx* c, int eval_set) {
   a.err = c->errd;
   a.np = c->np;
   return a;
+}
+
+// The kept sort.  Interpolation passes that move particles also keep the sorted copies of the
+// position set they fill: from q^n (set 0) to q^{n+1/2} (set 1) the slot order of set 0 is
+// reused, and the next update writes q^{n+1} back into set 0's slots.  A particle that leaves
+// its tile's skin raises the set's flag, and the set is sorted again (the sort kernels are
+// gated on that flag).
+void skin_to_half(fhd_ctx* c, InterpArgs& ia) {
+  SkinArgs& k = ia.sk;
+  k.s_cell = c->scell[1];
+  k.s_off = c->soff[1];
+  k.s_slot = nullptr;
+  k.skin_flag = c->sflag + 1;
+  k.clear_flag = c->sflag;
+  k.sid_dst = c->sid[1];
+  k.start_src = c->start[0];
+  k.start_dst = c->start[1];
+  for (int a = 0; a < 3; ++a) k.skin[a] = c->skin[a];
+}
+void skin_to_full(fhd_ctx* c, InterpArgs& ia) {
+  SkinArgs& k = ia.sk;
+  k.s_cell = c->scell[0];
+  k.s_off = c->soff[0];
+  k.s_slot = c->slot_of;
+  k.skin_flag = c->sflag;
+  k.clear_flag = c->sflag + 1;
+  k.sid_dst = nullptr;
+  k.start_src = nullptr;
+  k.start_dst = nullptr;
+  for (int a = 0; a < 3; ++a) k.skin[a] = c->skin[a];
 }
 
 // ---------------- schemes ----------------
@@ -696,6 +743,7 @@ void step_stokes(fhd_ctx* c) {
     ia.base_off = c->off[0];
     ia.out_cell = c->cell[1];
     ia.out_off = c->off[1];
+    skin_to_half(c, ia);
     ia.f1 = c->v;
     ia.a1 = (float)(0.5 * dt);
     if (xm) {
@@ -736,6 +784,7 @@ void step_stokes(fhd_ctx* c) {
     ia.base_off = c->off[0];
     ia.out_cell = c->cell[0];
     ia.out_off = c->off[0];
+    skin_to_full(c, ia);
     ia.f1 = c->v;
     ia.a1 = (float)dt;
     ia.u_out = c->u;
@@ -749,6 +798,105 @@ void step_stokes(fhd_ctx* c) {
     mark(c, "interp");
     c->last_force_set = 1;
   }
+}
+
+// ---------------- inertial coupling with excess mass ----------------
+// The corrections of one QNB step with m_e != 0 after the unperturbed prediction v~ (in v2),
+// at q^{n+1/2} (sorted set 1) with q^n by id (src/runSchemeQuasiNeutrallyBuoyant.cu:336-656):
+//   du  = lag + J(q^{n+1/2}) v^n - J(q^n) v^n                     kernelCalculateDeltau
+//   dp  = m_e (u - J v~ - du)                                      kernelCalculateDeltap
+//   dv~ = P [sum phi f dp], f = 1.5 V_p / (1.5 V_p dV rho + m_e)  -> M1
+//   s   = dp - m_e J dv~;  u' = J v~ + J dv~ + du (update_u)       updateParticleVelocityme
+//   dv  = (I - dt nu/2 L)^-1 P S(s) / rho                          -> W
+// `lag` false: the bootstrap predictor, which has no lagged term yet.
+PVecArgs pvec_base(fhd_ctx* c) {
+  PVecArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = c->g;
+  a.kernel_points = c->p.kernel_points;
+  a.eval = sset(c, 1);
+  a.np = c->np;
+  return a;
+}
+void pvec(fhd_ctx* c, const PVecArgs& a) {
+  launch_pvec(a, c->s);
+  mark(c, "mass");
+}
+void qnb_mass_correction(fhd_ctx* c, bool lag, bool update_u) {
+  const fhd_params& p = c->p;
+  const double me = p.mass, rho = p.densfluid, dV = c->dV, Vp = p.volume_particle;
+  const float cn = (float)(0.5 * p.dt * p.shearviscosity / rho);
+  PVecArgs a = pvec_base(c);
+  a.f1 = c->v;
+  a.c1 = 1.0f;
+  a.f0 = c->v;
+  a.c0 = -1.0f;
+  a.cell0 = c->cell[0];
+  a.off0 = c->off[0];
+  a.A = lag ? c->dulag : nullptr;
+  a.a = 1.0f;
+  a.out = c->du;
+  pvec(c, a);
+  a = pvec_base(c);
+  a.f1 = c->v2;
+  a.c1 = (float)-me;
+  a.d1 = 1.0f;
+  a.A = c->u;
+  a.a = (float)me;
+  a.B = c->du;
+  a.b = (float)-me;
+  a.out = c->dp;
+  a.out2 = c->jvt;
+  pvec(c, a);
+  const double f = 1.5 * Vp / (1.5 * Vp * dV * rho + me);  // src/quasiNeutrallyBuoyantFunctions2.cu:1332-1335
+  SpreadArgs sa = spread_base(c);
+  sa.ngroups = 1;
+  sa.grp[0].set = sset(c, 1);
+  sa.grp[0].mode = SRC_VALUE;
+  sa.grp[0].val = c->dp;
+  sa.grp[0].coef = (float)(f * dV);  // the pass scales by 1/dV: sum phi f dp
+  sa.out = c->W;
+  sa.out_add = nullptr;
+  run_spread(c, sa);
+  fft_forward(c, c->W);
+  fft_solve(c, 0, 0.0f);  // projectionDivergenceFree (mode 0 kept)
+  fft_inverse(c, c->M1, nullptr, 1.0f, 0.0f);
+  a = pvec_base(c);
+  a.f1 = c->M1;
+  a.c1 = (float)-me;
+  a.A = c->dp;
+  a.a = 1.0f;
+  a.out = c->ms;
+  if (update_u) {
+    a.d1 = 1.0f;
+    a.A2 = c->jvt;
+    a.a2 = 1.0f;
+    a.B2 = c->du;
+    a.b2 = 1.0f;
+    a.out2 = c->u;
+  }
+  pvec(c, a);
+  sa.grp[0].val = c->ms;
+  sa.grp[0].coef = (float)(1.0 / rho);  // S(s) / rho = sum phi s / (rho dV)
+  run_spread(c, sa);
+  fft_forward(c, c->W);
+  fft_solve(c, 0, cn);
+  fft_inverse(c, c->W, nullptr, 1.0f, 0.0f);
+}
+// M1 = 1/2 dt nu L dv (dv in W) and v2 = al v + be v2 + ga dv.
+void qnb_mass_fluid(fhd_ctx* c, float al, float be, float ga) {
+  MassFluidArgs m;
+  m.g = c->g;
+  m.dv = c->W;
+  m.lap = c->M1;
+  m.coef = (float)(0.5 * c->p.dt * c->p.shearviscosity / c->p.densfluid);
+  m.v = c->v;
+  m.v2 = c->v2;
+  m.al = al;
+  m.be = be;
+  m.ga = ga;
+  launch_mass_fluid(m, c->s);
+  mark(c, "mass");
 }
 
 // Host-side state a QNB step leaves behind (also applied after each replay of a captured step):
@@ -780,6 +928,7 @@ void step_qnb(fhd_ctx* c) {
     ia.base_off = c->off[0];
     ia.out_cell = c->cell[1];
     ia.out_off = c->off[1];
+    skin_to_half(c, ia);
     ia.f1 = c->v;
     ia.a1 = (float)(0.5 * dt);
     launch_interp(ia, c->s);
@@ -802,6 +951,7 @@ void step_qnb(fhd_ctx* c) {
     w.force_coef = (float)(dt / rho);
     c->last_force_set = 1;
   }
+  const bool mass = c->M1 != nullptr;
   if (!c->qnb_started) {
     // predictor with first-order advection; stores A(v^n) as the Adams-Bashforth history
     w.vadv = c->v;
@@ -811,7 +961,29 @@ void step_qnb(fhd_ctx* c) {
     mark(c, "fluid_w");
     fft_forward(c, c->W);
     fft_solve(c, 0, cn);
-    fft_inverse(c, c->v2, c->v, 0.5f, 0.5f);  // v^{n+1/2}
+    if (!mass) {
+      fft_inverse(c, c->v2, c->v, 0.5f, 0.5f);  // v^{n+1/2}
+    } else {
+      // the predictor's excess-mass corrections (no lagged term yet, u kept), v^{n+1/2} =
+      // (v^n + v~ + dv) / 2, and the corrector's lagged term J(q') 1/2 dt nu L dv at
+      // q' = q^n + dt/2 J(q^{n+1/2}) v^{n+1/2} (src/firstStepQuasiNeutrallyBuoyant.cu:84,229-560)
+      fft_inverse(c, c->v2, nullptr, 1.0f, 0.0f);
+      qnb_mass_correction(c, false, false);
+      qnb_mass_fluid(c, 0.5f, 0.5f, 0.5f);
+      Lag2Args l;
+      l.g = c->g;
+      l.kernel_points = p.kernel_points;
+      l.eval = sset(c, 1);
+      l.cell0 = c->cell[0];
+      l.off0 = c->off[0];
+      l.fv = c->v2;
+      l.hdt = (float)(0.5 * dt);
+      l.fl = c->M1;
+      l.out = c->dulag;
+      l.np = c->np;
+      launch_lag2(l, c->s);
+      mark(c, "mass");
+    }
     // corrector: advection at v^{n+1/2}
     w.vadv = c->v2;
     w.adv_store = nullptr;
@@ -830,18 +1002,30 @@ void step_qnb(fhd_ctx* c) {
   fft_forward(c, c->W);
   fft_solve(c, 0, cn);
   fft_inverse(c, c->v2, nullptr, 1.0f, 0.0f);  // v~^{n+1}
+  if (mass) {
+    // excess mass: u' and dv, v^{n+1} = v~ + dv, and the next step's lagged term
+    // J(q^{n+1/2}) 1/2 dt nu L dv (interpolateLaplacianDeltaV)
+    qnb_mass_correction(c, true, true);
+    qnb_mass_fluid(c, 0.0f, 1.0f, 1.0f);
+    PVecArgs a = pvec_base(c);
+    a.f1 = c->M1;
+    a.c1 = 1.0f;
+    a.out = c->dulag;
+    pvec(c, a);
+  }
   if (c->np > 0) {
-    // q^{n+1} = q^n + dt/2 J(q^{n+1/2}) (v^n + v~); u = J(q^{n+1/2}) v~
+    // q^{n+1} = q^n + dt/2 J(q^{n+1/2}) (v^n + v^{n+1}); u = J(q^{n+1/2}) v~ (mass 0)
     InterpArgs ia = interp_base(c, 1);
     ia.base_cell = c->cell[0];
     ia.base_off = c->off[0];
     ia.out_cell = c->cell[0];
     ia.out_off = c->off[0];
+    skin_to_full(c, ia);
     ia.f1 = c->v;
     ia.a1 = (float)(0.5 * dt);
     ia.f2 = c->v2;
     ia.a2 = (float)(0.5 * dt);
-    ia.u_out = c->u;
+    ia.u_out = mass ? nullptr : c->u;
     launch_interp(ia, c->s);
     mark(c, "interp");
   }
@@ -1545,8 +1729,39 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
       }
       c->u = dalloc<float4>(c->np);
       HIPCHK(hipMemset(c->u, 0, sizeof(float4) * c->np));
-      c->count = dalloc<int>(t.n);
-      HIPCHK(hipMemset(c->count, 0, sizeof(int) * t.n));
+      if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT && p.mass != 0.0) {
+        c->M1 = dalloc<float>(N3);
+        float4** pv[5] = {&c->dulag, &c->du, &c->dp, &c->jvt, &c->ms};
+        for (float4** q : pv) {
+          *q = dalloc<float4>(c->np);
+          HIPCHK(hipMemset(*q, 0, sizeof(float4) * c->np));
+        }
+      }
+      c->count = dalloc<int>(2 * (size_t)t.n);
+      HIPCHK(hipMemset(c->count, 0, 2 * sizeof(int) * t.n));
+      c->slot_of = dalloc<int>(c->np);
+      c->sflag = dalloc<int>(2);
+      const int flags0[2] = {1, 0};
+      HIPCHK(hipMemcpy(c->sflag, flags0, sizeof(flags0), hipMemcpyHostToDevice));
+      // skin of the kept sort, per axis with three or more tiles: half a cell (the WCA pass
+      // culls from the tile grown by it, so it is kept small; particles move ~1e-3 cells per
+      // step, so a sort is rarely needed).  The spreading pass finds a source in its tile's
+      // neighbours while its stencil (two cells each way, one more for an RFD displacement)
+      // stays within them, s <= T - 3; the WCA pass finds every pair while no tile beyond the
+      // neighbours can hold a partner, 2 s h + rc <= T h.
+      {
+        const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz};
+        const double h[3] = {c->dx, c->dy, c->dz};
+        const bool wca_on = p.nonbonded && p.cutoff > 0;
+        c->skin_len = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+          if (NT[a] <= 2) { c->skin[a] = -1.0f; continue; }
+          double sk = std::min(0.5, (double)(T[a] - 3));
+          if (wca_on) sk = std::min(sk, (T[a] * h[a] - p.cutoff) / (2.0 * h[a]));
+          c->skin[a] = (float)std::max(sk, 0.0);
+          c->skin_len = std::max(c->skin_len, (float)(c->skin[a] * h[a]));
+        }
+      }
       c->slot = dalloc<int>(c->np);
       c->tmp = dalloc<int>(c->np);
       std::vector<double> q;
@@ -1671,6 +1886,8 @@ int fhd_set_particles(fhd_ctx* c, const double* q) {
     if (np > 0) {
       HIPCHK(hipMemcpy(c->cell[0], hc.data(), np * sizeof(int4), hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->off[0], ho.data(), np * sizeof(float4), hipMemcpyHostToDevice));
+      const int one = 1;  // the sorted copies are stale: sort again before the next use
+      HIPCHK(hipMemcpy(c->sflag, &one, sizeof(int), hipMemcpyHostToDevice));
     }
   });
   return 1;
@@ -2080,6 +2297,7 @@ long long ctx_bytes(const fhd_params& p) {
     b += 2 * np * (16 + 16 + 16 + 4 + 16 + 16) + 2 * (nt + 1) * 4;  // positions, forces, sorted sets
     b += np * 16 + nt * 4 + 2 * np * 4;                               // u, counts, slots, scratch
     b += 2 * 2 * np * ((long long)sizeof(SrcRec) + 16);               // spreading records
+    if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT && p.mass != 0.0) b += N3 * 4 + 5 * np * 16;
   }
   return b;
 }

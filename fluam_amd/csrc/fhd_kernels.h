@@ -74,19 +74,24 @@ struct Tiles {
   int n;                // total tiles
 };
 
-// Counting sort of one position set by tile (bin, scan + scatter, per-tile ordering).
+// Counting sort of one position set by tile (bin, then scan + scatter).  The order inside a
+// tile is the order of the binning atomics: every consumer is order-independent (integer
+// accumulation in the spread and WCA passes, per-particle interpolation).
 struct SortArgs {
   GridDev g;
   Tiles t;
   const int4* cell;     // unwrapped cells (by id)
   const float4* off;
-  int* count;           // per-tile counts (zero on entry, cleared again by the ordering pass)
+  int* count;           // per-tile counts of this set (zero on entry)
+  int* count_other;     // the other set's counts, cleared by the scatter pass for that set's next sort
   int* slot;            // rank of each particle within its tile, in atomic order
   int* start;           // tile_start[ntiles + 1]
-  int* tmp;             // ids grouped by tile
+  int* tmp;             // ids grouped by tile (large-grid path)
   int* sid;             // sorted outputs
-  int4* scell;
+  int4* scell;          // wrapped cell, w = the tile the slot is listed in
   float4* soff;
+  int* slot_of;         // optional: sorted slot of each particle id
+  const int* gate;      // optional: the sort runs only when *gate != 0 (the count clearing always runs)
   int np;
 };
 
@@ -134,6 +139,22 @@ struct SpreadArgs {
   int nrec0, nrec_tot;  // records of group 0 and of both groups (set by launch_src_prep)
 };
 
+// The sorted copies of an updated position set are kept in place between sorts: a particle
+// stays listed in its tile while it lies within `skin` cells of the tile along every axis that
+// has three or more tiles; beyond that the pass raises *skin_flag and the set is sorted again
+// before its next use.
+struct SkinArgs {
+  int4* s_cell;          // sorted copy to update (wrapped cell; w = listed tile, kept)
+  float4* s_off;
+  const int* s_slot;     // slot of each id in s_cell (null: the evaluated slot itself)
+  int* skin_flag;        // set to 1 when a particle leaves its tile's skin
+  int* clear_flag;       // set to 0 by the pass (a flag consumed by the sort before it)
+  int* sid_dst;          // the new set's slot ids (its slot order is the evaluated set's), null: none
+  const int* start_src;  // copied to start_dst (ntiles + 1 entries)
+  int* start_dst;
+  float skin[3];         // cells per axis; a negative value: the axis is never checked
+};
+
 struct InterpArgs {
   GridDev g;
   Tiles t;
@@ -143,6 +164,7 @@ struct InterpArgs {
   const float4* base_off;
   int4* out_cell;        // out = base + a1 J f1 + a2 J f2 + extra (by id)
   float4* out_off;
+  SkinArgs sk;           // sorted copies of out, kept current (s_cell null: none)
   const float* f1; float a1;
   const float* f2; float a2;
   float4* u_out;         // J f2 (or J f1 when f2 is null), optional
@@ -153,11 +175,58 @@ struct InterpArgs {
   int np;
 };
 
+// Per-particle combinations of interpolated fields (the inertial-coupling step with excess mass,
+// src/quasiNeutrallyBuoyantFunctions2.cu:5-2290,2679-3149):
+//   out[id]  = a A[id] + b B[id] + c1 J(f1) + c0 J0(f0)
+//   out2[id] = a2 A2[id] + b2 B2[id] + d1 J(f1)           (when out2 is set)
+// J interpolates at the sorted set `eval`, J0 at the by-id positions (cell0, off0).  Null
+// vectors and fields count as zero.
+struct PVecArgs {
+  GridDev g;
+  int kernel_points;
+  SortedSet eval;
+  const float* f1; float c1, d1;
+  const float* f0; float c0;
+  const int4* cell0; const float4* off0;
+  const float4* A; float a;
+  const float4* B; float b;
+  float4* out;
+  const float4* A2; float a2;
+  const float4* B2; float b2;
+  float4* out2;
+  int np;
+};
+
+// Lagged term of the bootstrap step: q' = q0(id) + hdt J(fv) at `eval`, then out[id] = J(q') fl
+// (findNeighborParticlesQuasiNeutrallyBuoyantTEST4_3 and interpolateLaplacianDeltaV_2,
+// src/quasiNeutrallyBuoyantFunctions.cu:4872-5258,5879-6281).
+struct Lag2Args {
+  GridDev g;
+  int kernel_points;
+  SortedSet eval;
+  const int4* cell0; const float4* off0;  // q^n by id (unwrapped cells)
+  const float* fv; float hdt;
+  const float* fl;
+  float4* out;
+  int np;
+};
+
+// lap = coef L dv, and v2 = al v + be v2 + ga dv (laplacianDeltaV, calculateVelocityAtHalfTimeStep,
+// updateFluidQuasiNeutrallyBuoyantSemiImplicit).
+struct MassFluidArgs {
+  GridDev g;
+  const float* dv;
+  float* lap; float coef;
+  const float* v; float* v2;
+  float al, be, ga;
+};
+
 struct WcaArgs {
   GridDev g;
   Tiles t;
   SortedSet set;
   float rc2, sig2, eps48;
+  float skin_len;  // how far (length) a listed particle may lie outside its tile
   float4* F;   // by id
   int np;
 };
@@ -203,6 +272,9 @@ void launch_bonded(const BondArgs& a, hipStream_t s);  // F += bond forces (afte
 void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
 void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation
 void launch_interp(const InterpArgs& a, hipStream_t s);
+void launch_pvec(const PVecArgs& a, hipStream_t s);
+void launch_lag2(const Lag2Args& a, hipStream_t s);
+void launch_mass_fluid(const MassFluidArgs& a, hipStream_t s);
 inline void launch_zero_f4(float4* p, int n, hipStream_t s) {
   if (n > 0) launch_zero_f(reinterpret_cast<float*>(p), 4 * (size_t)n, s);
 }

@@ -187,6 +187,31 @@ __global__ void k_divergence(GridDev g, const float* __restrict__ v, float* __re
            (VZ[i] - VZ[x + g.mx * (y + g.my * zm)]) * g.idz;
 }
 
+// Excess-mass fluid update: lap = coef L dv (seven-point staggered Laplacian per component,
+// laplacianDeltaV, src/quasiNeutrallyBuoyantFunctions.cu:5294-5386) and v2 = al v + be v2 + ga dv
+// (calculateVelocityAtHalfTimeStep / updateFluidQuasiNeutrallyBuoyantSemiImplicit).
+FHD_BODY void body_mass_fluid(const MassFluidArgs& a, const Blk& b) {
+  const GridDev& g = a.g;
+  const size_t e = (size_t)b.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * (size_t)g.N) return;
+  const int C = (int)(e / g.N);
+  const int i = (int)(e - (size_t)C * g.N);
+  const int x = i % g.mx, y = (i / g.mx) % g.my, z = i / (g.mx * g.my);
+  const float* f = a.dv + (size_t)C * g.N;
+  auto at = [&](int xx, int yy, int zz) { return f[wrapi(xx, g.mx) + g.mx * (wrapi(yy, g.my) + g.my * wrapi(zz, g.mz))]; };
+  const float c0 = f[i];
+  const float L = g.idx * g.idx * (at(x + 1, y, z) - 2.0f * c0 + at(x - 1, y, z)) +
+                  g.idy * g.idy * (at(x, y + 1, z) - 2.0f * c0 + at(x, y - 1, z)) +
+                  g.idz * g.idz * (at(x, y, z + 1) - 2.0f * c0 + at(x, y, z - 1));
+  a.lap[e] = a.coef * L;
+  if (a.v2) a.v2[e] = a.al * a.v[e] + a.be * a.v2[e] + a.ga * c0;
+}
+FHD_KERNEL_PAIR((), 256, k_mass_fluid, MassFluidArgs, body_mass_fluid(a, b))
+
+void launch_mass_fluid(const MassFluidArgs& a, hipStream_t s) {
+  launch(k_mass_fluid, k_mass_fluid_b, FAM_MASS_FLUID, dim3((unsigned)((3 * (size_t)a.g.N + 255) / 256)), 256, a, s);
+}
+
 // Normals of the shared generator, for parity tests: out[4 i + k] = draw(index i)[k].
 __global__ void k_normals(Key key, int stream, int sub, int n, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -28,16 +28,29 @@ FHD_DEV int tile_of(const GridDev& g, const Tiles& t, int4 c) {
   return x + t.ntx * (y + t.nty * z);
 }
 
+FHD_DEV bool sort_on(const SortArgs& a) { return a.gate == nullptr || *a.gate != 0; }
+
 FHD_BODY void body_bin(const SortArgs& a, const Blk& b) {
   int i = b.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
+  if (i >= a.np || !sort_on(a)) return;
   int tile = tile_of(a.g, a.t, a.cell[i]);
   a.slot[i] = atomicAdd(&a.count[tile], 1);
 }
 FHD_KERNEL_PAIR((), 256, k_bin, SortArgs, body_bin(a, b))
 
-// Exclusive scan of the tile counts (single workgroup); resets the counts for the next binning.
+// Sorted copy of particle i at slot pos of tile `tile`.
+FHD_DEV void place(const SortArgs& a, int i, int pos, int tile) {
+  const int4 c = a.cell[i];
+  a.sid[pos] = i;
+  a.scell[pos] = make_int4(wrapi(c.x, a.g.mx), wrapi(c.y, a.g.my), wrapi(c.z, a.g.mz), tile);
+  a.soff[pos] = a.off[i];
+  if (a.slot_of) a.slot_of[i] = pos;
+}
+
+// Exclusive scan of the tile counts (single workgroup, large grids); resets the counts for the
+// next binning.
 FHD_BODY void body_scan(const SortArgs& a) {
+  if (!sort_on(a)) return;
   const int n = a.t.n;
   int* __restrict__ count = a.count;
   int* __restrict__ start = a.start;
@@ -68,15 +81,17 @@ FHD_KERNEL_PAIR((), 1024, k_scan, SortArgs, body_scan(a))
 
 FHD_BODY void body_scatter(const SortArgs& a, const Blk& b) {
   int i = b.x * blockDim.x + threadIdx.x;
-  if (i >= a.np) return;
-  a.tmp[a.start[tile_of(a.g, a.t, a.cell[i])] + a.slot[i]] = i;
+  if (i >= a.np || !sort_on(a)) return;
+  const int tile = tile_of(a.g, a.t, a.cell[i]);
+  place(a, i, a.start[tile] + a.slot[i], tile);
 }
 FHD_KERNEL_PAIR((), 256, k_scatter, SortArgs, body_scatter(a, b))
 
 // Scan + scatter in one launch for up to kScanTilesMax tiles: every workgroup forms the whole
 // exclusive prefix of the tile counts (a few KB, read from L2) in LDS, workgroup 0 stores it as
-// the tile starts, and each thread places its particle.  The counts are cleared later, by the
-// per-tile ordering pass.
+// the tile starts, and each thread places its particle's sorted copy.  This set's counts are
+// still being read, so the launch clears the other set's counts instead (their next binning
+// comes after this set's): the clearing runs whether or not the sort itself is gated off.
 constexpr int kScanTilesMax = 8192;
 FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
   const GridDev& g = a.g;
@@ -85,11 +100,13 @@ FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
   const int* __restrict__ slot = a.slot;
   const int* __restrict__ count = a.count;
   int* __restrict__ start = a.start;
-  int* __restrict__ tmp = a.tmp;
   const int np = a.np;
   __shared__ int pre[kScanTilesMax + 1];
   __shared__ int wsum[4];
   const int n = t.n, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int gtid = b.x * blockDim.x + tid, gsz = b.nx * blockDim.x;
+  for (int k = gtid; k < n; k += gsz) a.count_other[k] = 0;
+  if (!sort_on(a)) return;  // uniform over the grid
   const int ch = (n + 255) / 256;
   const int lo = min(n, tid * ch), hi = min(n, lo + ch);
   int sum = 0;
@@ -108,57 +125,12 @@ FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
   if (b.x == 0)
     for (int k = tid; k <= n; k += 256) start[k] = pre[k];
   const int i = b.x * blockDim.x + tid;
-  if (i < np) tmp[pre[tile_of(g, t, cell[i])] + slot[i]] = i;
+  if (i < np) {
+    const int tile = tile_of(g, t, cell[i]);
+    place(a, i, pre[tile] + slot[i], tile);
+  }
 }
 FHD_KERNEL_PAIR((), 256, k_scan_scatter, SortArgs, body_scan_scatter(a, b))
-
-constexpr int kSortStage = 256;  // per-wave LDS staging of a tile's ids
-
-// One wavefront per tile (four per workgroup): rank each particle by its index within the
-// tile, then gather.  The
-// sorted copies carry the cell wrapped into the box (0 <= c < m), which is all the spatial
-// passes need; the unwrapped cell stays in the per-particle arrays.
-FHD_BODY void body_tile_sort(const SortArgs& a, const Blk& b) {
-  const GridDev& g = a.g;
-  const int ntiles = a.t.n;
-  const int* __restrict__ start = a.start;
-  const int* __restrict__ tmp = a.tmp;
-  const int4* __restrict__ cell = a.cell;
-  const float4* __restrict__ off = a.off;
-  int* __restrict__ sid = a.sid;
-  int4* __restrict__ scell = a.scell;
-  float4* __restrict__ soff = a.soff;
-  int* __restrict__ count = a.count;
-  __shared__ int ids[4][kSortStage];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int tile = b.x * 4 + w;
-  const bool valid = tile < ntiles;
-  int s0 = 0, n = 0;
-  if (valid) {
-    s0 = start[tile];
-    n = start[tile + 1] - s0;
-    if (lane == 0) count[tile] = 0;  // ready for the next binning
-  }
-  // the tile's ids in LDS (the rank loop reads each one n times)
-  const bool staged = n <= kSortStage;
-  if (staged)
-    for (int e = lane; e < n; e += 64) ids[w][e] = tmp[s0 + e];
-  __syncthreads();
-  if (!valid) return;
-  for (int e = lane; e < n; e += 64) {
-    const int id = staged ? ids[w][e] : tmp[s0 + e];
-    int rank = 0;
-    if (staged)
-      for (int f = 0; f < n; ++f) rank += (ids[w][f] < id);
-    else
-      for (int f = 0; f < n; ++f) rank += (tmp[s0 + f] < id);
-    int4 c = cell[id];
-    sid[s0 + rank] = id;
-    scell[s0 + rank] = make_int4(wrapi(c.x, g.mx), wrapi(c.y, g.my), wrapi(c.z, g.mz), 0);
-    soff[s0 + rank] = off[id];
-  }
-}
-FHD_KERNEL_PAIR((), 256, k_tile_sort, SortArgs, body_tile_sort(a, b))
 
 // d = difference of two wrapped cell coordinates, |d| < m: nearest periodic image.
 FHD_DEV int min_image(int d, int m) {
@@ -181,6 +153,13 @@ FHD_DEV int wave_incl_scan(int v) {
     if (lane >= o) v += t;
   }
   return v;
+}
+
+// A pair term scaled to |x| < 2^40 as a 64-bit integer; non-finite values saturate (NaN to the
+// negative bound), so the conversion is always defined.
+FHD_DEV long long fixed_term(float x) {
+  const float c = fminf(fmaxf(x, -2199023255552.0f), 2199023255552.0f);  // +-2^41
+  return (long long)c;
 }
 
 constexpr int kWcaThreads = 256;
@@ -230,14 +209,16 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   }
   __syncthreads();
   const int nall = nb_pref[nnb];
-  // conservative cull radius (the pair test below is exact)
+  // conservative cull radius (the pair test below is exact), measured from the tile grown by
+  // the skin: a listed particle may lie up to skin_len outside its tile along each axis
   const float rcc = sqrtf(a.rc2) * 1.001f + 1e-6f;
+  const float skin = a.skin_len;
   const float Tx = t.tx * g.dx, Ty = t.ty * g.dy, Tz = t.tz * g.dz;
   // The distance to the tile is measured from the candidate's image nearest the tile centre,
   // folded on whole cells, so it can be off by up to one cell near the half-box point.  Along
   // an axis where that point lies farther than rc + one cell outside the tile, the cull is
   // exact; along an axis with one tile every candidate is inside.  Otherwise keep everything.
-  auto axis_ok = [&](int nt_a, int m, int T, float d) { return nt_a == 1 || 0.5f * (m - T) * d > rcc + d; };
+  auto axis_ok = [&](int nt_a, int m, int T, float d) { return nt_a == 1 || 0.5f * (m - T) * d > rcc + skin + d; };
   const bool cull = axis_ok(t.ntx, g.mx, t.tx, g.dx) && axis_ok(t.nty, g.my, t.ty, g.dy) &&
                     axis_ok(t.ntz, g.mz, t.tz, g.dz);
   // Compacts the near candidates with rank in [p0, p0 + kWcaCap) into LDS; returns how many
@@ -266,9 +247,9 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
           const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
           const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
           const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
-          const float ex = t.ntx == 1 ? 0.0f : fmaxf(0.0f, fabsf(px) - 0.5f * Tx);
-          const float ey = t.nty == 1 ? 0.0f : fmaxf(0.0f, fabsf(py) - 0.5f * Ty);
-          const float ez = t.ntz == 1 ? 0.0f : fmaxf(0.0f, fabsf(pz) - 0.5f * Tz);
+          const float ex = t.ntx == 1 ? 0.0f : fmaxf(0.0f, fabsf(px) - 0.5f * Tx - skin);
+          const float ey = t.nty == 1 ? 0.0f : fmaxf(0.0f, fabsf(py) - 0.5f * Ty - skin);
+          const float ez = t.ntz == 1 ? 0.0f : fmaxf(0.0f, fabsf(pz) - 0.5f * Tz - skin);
           near = ex * ex + ey * ey + ez * ez <= rcc * rcc;
         }
       }
@@ -309,42 +290,80 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
       ci = a.set.cell[si];
       di = a.set.off[si];
     }
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int p = 0; p < npass; ++p) {
-      if (npass > 1) {
-        __syncthreads();  // the previous pass's candidates consumed
-        stage(p * kWcaCap);
-      }
-      const int nk = min(kWcaCap, nnear - p * kWcaCap);
-      if (own) {
-        for (int k = sl; k < nk; k += S) {
-          const int4 cc = ccell[k];
-          if (cc.w == si) continue;
-          const float4 dd = coff[k];
-          float rx = (float)min_image(ci.x - cc.x, g.mx) * g.dx + (di.x - dd.x);
-          float ry = (float)min_image(ci.y - cc.y, g.my) * g.dy + (di.y - dd.y);
-          float rz = (float)min_image(ci.z - cc.z, g.mz) * g.dz + (di.z - dd.z);
-          float r2 = rx * rx + ry * ry + rz * rz;
-          if (r2 < a.rc2 && r2 > 0.0f) {
-            // separations below sigma/100 are evaluated at sigma/100, so the force stays finite
-            // (fluam's tabulated force is bounded near r = 0 too, src/initForcesNonBonded.cu:59)
-            r2 = fmaxf(r2, 1.0e-4f * a.sig2);
-            const float s2 = a.sig2 / r2;
-            const float s6 = s2 * s2 * s2;
-            const float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
-            fx += f * rx;
-            fy += f * ry;
-            fz += f * rz;
+    // The force is summed as 64-bit integers at a scale set by the largest force component of
+    // any of the particle's pair terms: each term keeps its FP32 value and integer sums are
+    // exact, so the force does not depend on the order of the candidates (the order inside a
+    // tile is that of the sort's atomics).  A first sweep finds the largest component and keeps
+    // up to four terms per lane in registers (pairs within rc are few); a second sweep
+    // recomputes the terms only when some lane had more, or the candidates came in passes.
+    float m = 0.0f, scale = 0.0f;
+    long long sx = 0, sy = 0, sz = 0;
+    float t0x = 0, t0y = 0, t0z = 0, t1x = 0, t1y = 0, t1z = 0, t2x = 0, t2y = 0, t2z = 0, t3x = 0, t3y = 0,
+          t3z = 0;
+    int nterm = 0;
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      for (int p = 0; p < npass; ++p) {
+        if (npass > 1) {
+          __syncthreads();  // the previous pass's candidates consumed
+          stage(p * kWcaCap);
+        }
+        const int nk = min(kWcaCap, nnear - p * kWcaCap);
+        if (own) {
+          for (int k = sl; k < nk; k += S) {
+            const int4 cc = ccell[k];
+            if (cc.w == si) continue;
+            const float4 dd = coff[k];
+            float rx = (float)min_image(ci.x - cc.x, g.mx) * g.dx + (di.x - dd.x);
+            float ry = (float)min_image(ci.y - cc.y, g.my) * g.dy + (di.y - dd.y);
+            float rz = (float)min_image(ci.z - cc.z, g.mz) * g.dz + (di.z - dd.z);
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 < a.rc2 && r2 > 0.0f) {
+              // separations below sigma/100 are evaluated at sigma/100, so the force stays finite
+              // (fluam's tabulated force is bounded near r = 0 too, src/initForcesNonBonded.cu:59)
+              r2 = fmaxf(r2, 1.0e-4f * a.sig2);
+              const float s2 = a.sig2 / r2;
+              const float s6 = s2 * s2 * s2;
+              const float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
+              const float tx = f * rx, ty = f * ry, tz = f * rz;
+              if (sweep == 0) {
+                m = fmaxf(m, fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz))));
+                if (nterm == 0) { t0x = tx; t0y = ty; t0z = tz; }
+                else if (nterm == 1) { t1x = tx; t1y = ty; t1z = tz; }
+                else if (nterm == 2) { t2x = tx; t2y = ty; t2z = tz; }
+                else if (nterm == 3) { t3x = tx; t3y = ty; t3z = tz; }
+                ++nterm;
+              } else {
+                sx += fixed_term(tx * scale);
+                sy += fixed_term(ty * scale);
+                sz += fixed_term(tz * scale);
+              }
+            }
           }
+        }
+      }
+      if (sweep == 0) {
+        for (int o = 1; o < S; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (!(m <= 3.0e38f)) m = 3.0e38f;  // an overflowed term saturates
+        int e = 0;
+        if (m > 0.0f) frexpf(m, &e);  // m < 2^e
+        scale = ldexpf(1.0f, 40 - e);
+        m = ldexpf(1.0f, e - 40);  // from here on: the inverse scale
+        const bool again = __syncthreads_or(nterm > 4) || npass > 1;  // uniform over the group
+        if (!again) {
+          if (nterm > 0) { sx += fixed_term(t0x * scale); sy += fixed_term(t0y * scale); sz += fixed_term(t0z * scale); }
+          if (nterm > 1) { sx += fixed_term(t1x * scale); sy += fixed_term(t1y * scale); sz += fixed_term(t1z * scale); }
+          if (nterm > 2) { sx += fixed_term(t2x * scale); sy += fixed_term(t2y * scale); sz += fixed_term(t2z * scale); }
+          if (nterm > 3) { sx += fixed_term(t3x * scale); sy += fixed_term(t3y * scale); sz += fixed_term(t3z * scale); }
+          break;
         }
       }
     }
     for (int o = 1; o < S; o <<= 1) {
-      fx += __shfl_xor(fx, o);
-      fy += __shfl_xor(fy, o);
-      fz += __shfl_xor(fz, o);
+      sx += __shfl_xor(sx, o);
+      sy += __shfl_xor(sy, o);
+      sz += __shfl_xor(sz, o);
     }
-    if (own && sl == 0) a.F[a.set.id[si]] = make_float4(fx, fy, fz, 0.0f);
+    if (own && sl == 0) a.F[a.set.id[si]] = make_float4((float)sx * m, (float)sy * m, (float)sz * m, 0.0f);
   }
 }
 FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
@@ -532,11 +551,14 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
                      min(w.cen[2].base, w.stg[2].base)};
   const Tiles& t = a.t;
   const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz}, MM[3] = {g.mx, g.my, g.mz};
-  const int cc[3] = {c.x, c.y, c.z};  // sorted copies hold wrapped cells
+  // the tile the source is listed in (the sorted copy's w): the spreading pass looks for it in
+  // the lists of that tile's neighbours, which its stencil may reach while it stays within the
+  // skin of that tile
+  const int lt[3] = {c.w % t.ntx, (c.w / t.ntx) % t.nty, c.w / (t.ntx * t.nty)};
   int mask = 0;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    const int own = T[ax] == 8 ? cc[ax] >> 3 : cc[ax] / T[ax];
+    const int own = lt[ax];
 #pragma unroll
     for (int e = -1; e <= 1; ++e) {
       int tt = own + e;  // in [-1, NT]
@@ -892,6 +914,32 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
     renorm_axis(bc.z, oz, g.dz, g.idz);
     a.out_cell[id] = bc;
     a.out_off[id] = make_float4(ox, oy, oz, 0.0f);
+    if (a.sk.s_cell) {
+      // the sorted copy of the new position, in the slot of the set it replaces; the particle
+      // stays listed in that slot's tile unless it left the tile's skin
+      const int slot = a.sk.s_slot ? a.sk.s_slot[id] : s;
+      const int lt = (a.sk.s_slot ? a.sk.s_cell[slot] : a.eval.cell[s]).w;
+      const int wc[3] = {wrapi(bc.x, g.mx), wrapi(bc.y, g.my), wrapi(bc.z, g.mz)};
+      const int lto[3] = {(lt % a.t.ntx) * a.t.tx, ((lt / a.t.ntx) % a.t.nty) * a.t.ty,
+                          (lt / (a.t.ntx * a.t.nty)) * a.t.tz};
+      const int T[3] = {a.t.tx, a.t.ty, a.t.tz}, M[3] = {g.mx, g.my, g.mz};
+      const float fo[3] = {ox * g.idx, oy * g.idy, oz * g.idz};
+      bool out = false;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        const float sk = a.sk.skin[ax];
+        if (sk < 0.0f) continue;
+        int d = wc[ax] - lto[ax];  // in (-m, m): fold to [-m/2, m/2)
+        if (2 * d >= M[ax]) d -= M[ax];
+        else if (2 * d < -M[ax]) d += M[ax];
+        const float x = (float)d + 0.5f + fo[ax];  // cells from the tile's lower face
+        out |= !(x >= -sk && x <= (float)T[ax] + sk);
+      }
+      if (out) *a.sk.skin_flag = 1;
+      a.sk.s_cell[slot] = make_int4(wc[0], wc[1], wc[2], lt);
+      a.sk.s_off[slot] = make_float4(ox, oy, oz, 0.0f);
+      if (a.sk.sid_dst) a.sk.sid_dst[s] = id;
+    }
   }
 }
 
@@ -919,11 +967,103 @@ FHD_BODY void body_interp(const InterpArgs& a, const Blk& b) {
     u1[comp] = combine_rows(r1, wz);
     u2[comp] = combine_rows(r2, wz);
   }
+  if (a.sk.s_cell) {
+    // bookkeeping of the kept sort: the tile starts of the set this pass fills in, and the flag
+    // that the sort before this pass consumed
+    const int nth = b.nx * blockDim.x;
+    if (a.sk.start_dst)
+      for (int k = gid; k <= a.t.n; k += nth) a.sk.start_dst[k] = a.sk.start_src[k];
+    if (gid == 0 && a.sk.clear_flag) *a.sk.clear_flag = 0;
+  }
   if (s >= a.np || q != 0) return;
   interp_finish(a, s, u1, u2);
 }
 FHD_KERNEL_PAIR((template <int NPTS>), 256, k_interp, InterpArgs, body_interp<NPTS>(a, b))
 
+
+// ---------------- inertial coupling with excess mass ----------------
+// Interpolation of the three components of f at one stencil (weights w): every lane of the
+// particle's four gets the full sums (combine_rows broadcasts).
+template <int NPTS>
+FHD_DEV void interp3(const float* f, const GridDev& g, const SrcWeights& w, bool act, int q, float out[3]) {
+#pragma unroll
+  for (int comp = 0; comp < 3; ++comp) {
+    const AxisWeights& wx = comp == 0 ? w.stg[0] : w.cen[0];
+    const AxisWeights& wy = comp == 1 ? w.stg[1] : w.cen[1];
+    const AxisWeights& wz = comp == 2 ? w.stg[2] : w.cen[2];
+    float r = 0.0f;
+    if (act && f) r = interp_row<NPTS>(f + (size_t)comp * g.N, g, wx, wy, wz, q);
+    out[comp] = combine_rows(r, wz);
+  }
+}
+
+template <int NPTS>
+FHD_DEV void weights_at(const GridDev& g, int4 c, float4 d, SrcWeights& w) {
+  const int4 cw = make_int4(wrapi(c.x, g.mx), wrapi(c.y, g.my), wrapi(c.z, g.mz), 0);
+  source_weights<NPTS>(cw, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
+}
+
+FHD_DEV float4 f4_or_zero(const float4* p, int i) { return p ? p[i] : make_float4(0.f, 0.f, 0.f, 0.f); }
+
+template <int NPTS>
+FHD_BODY void body_pvec(const PVecArgs& a, const Blk& b) {
+  const int gid = xcd_item(b.x, b.nx) * blockDim.x + threadIdx.x;
+  const int s = gid >> 2, q = gid & 3;
+  const GridDev& g = a.g;
+  const bool act = s < a.np && q < NPTS;
+  SrcWeights w, w0;
+  int id = 0;
+  if (s < a.np) {
+    id = a.eval.id[s];
+    weights_at<NPTS>(g, a.eval.cell[s], a.eval.off[s], w);
+    if (a.f0) weights_at<NPTS>(g, a.cell0[id], a.off0[id], w0);
+  }
+  float j1[3], j0[3] = {0.f, 0.f, 0.f};
+  interp3<NPTS>(a.f1, g, w, act, q, j1);
+  if (a.f0) interp3<NPTS>(a.f0, g, w0, act, q, j0);
+  if (s >= a.np || q != 0) return;
+  const float4 A = f4_or_zero(a.A, id), B = f4_or_zero(a.B, id);
+  a.out[id] = make_float4(a.a * A.x + a.b * B.x + a.c1 * j1[0] + a.c0 * j0[0],
+                          a.a * A.y + a.b * B.y + a.c1 * j1[1] + a.c0 * j0[1],
+                          a.a * A.z + a.b * B.z + a.c1 * j1[2] + a.c0 * j0[2], 0.0f);
+  if (a.out2) {
+    const float4 A2 = f4_or_zero(a.A2, id), B2 = f4_or_zero(a.B2, id);
+    a.out2[id] = make_float4(a.a2 * A2.x + a.b2 * B2.x + a.d1 * j1[0], a.a2 * A2.y + a.b2 * B2.y + a.d1 * j1[1],
+                             a.a2 * A2.z + a.b2 * B2.z + a.d1 * j1[2], 0.0f);
+  }
+}
+FHD_KERNEL_PAIR((template <int NPTS>), 256, k_pvec, PVecArgs, body_pvec<NPTS>(a, b))
+
+template <int NPTS>
+FHD_BODY void body_lag2(const Lag2Args& a, const Blk& b) {
+  const int gid = xcd_item(b.x, b.nx) * blockDim.x + threadIdx.x;
+  const int s = gid >> 2, q = gid & 3;
+  const GridDev& g = a.g;
+  const bool act = s < a.np && q < NPTS;
+  SrcWeights w;
+  int id = 0;
+  if (s < a.np) {
+    id = a.eval.id[s];
+    weights_at<NPTS>(g, a.eval.cell[s], a.eval.off[s], w);
+  }
+  float uh[3];
+  interp3<NPTS>(a.fv, g, w, act, q, uh);
+  // q' = q^n + hdt J(q^{n+1/2}) v_half, as (cell, offset)
+  if (s < a.np) {
+    int4 c = a.cell0[id];
+    const float4 o = a.off0[id];
+    float ox = o.x + a.hdt * uh[0], oy = o.y + a.hdt * uh[1], oz = o.z + a.hdt * uh[2];
+    renorm_axis(c.x, ox, g.dx, g.idx);
+    renorm_axis(c.y, oy, g.dy, g.idy);
+    renorm_axis(c.z, oz, g.dz, g.idz);
+    weights_at<NPTS>(g, c, make_float4(ox, oy, oz, 0.0f), w);
+  }
+  float l[3];
+  interp3<NPTS>(a.fl, g, w, act, q, l);
+  if (s >= a.np || q != 0) return;
+  a.out[id] = make_float4(l[0], l[1], l[2], 0.0f);
+}
+FHD_KERNEL_PAIR((template <int NPTS>), 256, k_lag2, Lag2Args, body_lag2<NPTS>(a, b))
 
 // ------------------------------- launchers -------------------------------
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
@@ -933,10 +1073,10 @@ void launch_sort(const SortArgs& a, hipStream_t s) {
   if (a.t.n <= kScanTilesMax) {
     launch(k_scan_scatter, k_scan_scatter_b, FAM_SCAN, dim3(nblk(a.np > 0 ? a.np : 1, 256)), 256, a, s);
   } else {
+    // large grids: the single-workgroup scan clears this set's counts itself
     launch(k_scan, k_scan_b, FAM_SCAN, dim3(1), 1024, a, s);
     if (a.np > 0) launch(k_scatter, k_scatter_b, FAM_SCATTER, dim3(nblk(a.np, 256)), 256, a, s);
   }
-  launch(k_tile_sort, k_tile_sort_b, FAM_TILE_SORT, dim3(nblk(a.t.n, 4)), 256, a, s);
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n), kWcaThreads, a, s);
@@ -966,6 +1106,18 @@ void launch_spread(const SpreadArgs& a, hipStream_t s) {
     if (fast) launch(k_spread<3, true>, k_spread_b<3, true>, FAM_SPREAD, grid, kSpreadThreads, a, s);
     else launch(k_spread<3, false>, k_spread_b<3, false>, FAM_SPREAD, grid, kSpreadThreads, a, s);
   }
+}
+void launch_pvec(const PVecArgs& a, hipStream_t s) {
+  if (a.np <= 0) return;
+  const dim3 grid(nblk(4 * a.np, 256));
+  if (a.kernel_points == 4) launch(k_pvec<4>, k_pvec_b<4>, FAM_PVEC, grid, 256, a, s);
+  else launch(k_pvec<3>, k_pvec_b<3>, FAM_PVEC, grid, 256, a, s);
+}
+void launch_lag2(const Lag2Args& a, hipStream_t s) {
+  if (a.np <= 0) return;
+  const dim3 grid(nblk(4 * a.np, 256));
+  if (a.kernel_points == 4) launch(k_lag2<4>, k_lag2_b<4>, FAM_LAG2, grid, 256, a, s);
+  else launch(k_lag2<3>, k_lag2_b<3>, FAM_LAG2, grid, 256, a, s);
 }
 void launch_interp(const InterpArgs& a, hipStream_t s) {
   if (a.np <= 0) return;

@@ -69,7 +69,8 @@ typedef struct fhd_params {
   double cutoff;              /* cutoff */
   double wca_sigma;           /* extension keyword `sigma` (default 2*dx, src/initForcesNonBonded.cu:24) */
   double wca_epsilon;         /* extension keyword `epsilon` (default kT, src/initForcesNonBonded.cu:25) */
-  double mass;                /* mass (excess mass; the build supports 0) */
+  double mass;                /* mass: the particles' excess mass m_e over the fluid they carry,
+                                 m_f = 1.5 dV volumeParticle rho (quasiNeutrallyBuoyant; m_e + m_f > 0) */
   double volume_particle;     /* volumeParticle */
   double extra_mobility;      /* extraMobility (0 = off) */
   double rfd_delta;           /* RFD displacement (0.001, src/stokesLimitFunctions.cu:1719) */

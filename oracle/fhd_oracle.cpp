@@ -241,6 +241,7 @@ struct orc_ctx {
   std::vector<double> F[3];     // last particle forces
   std::vector<double> u[3];     // last particle velocities (QNB: J v~)
   std::vector<double> advPrev[3];  // QNB Adams-Bashforth history
+  std::vector<double> dulag[3];    // QNB, excess mass: lagged 1/2 dt nu J L Delta v of the last step
   bool first_step_done = false;
   int64_t step = 0;
   bool projected = false;
@@ -988,6 +989,15 @@ extern "C" int orc_solve_incompressible(orc_ctx* c, const double* Win, double* v
   for (int a = 0; a < 3; ++a) memcpy(vout + (size_t)a * c->N, V[a].data(), sizeof(double) * c->N);
   return 1;
 }
+// Projection onto discretely divergence-free fields, mode 0 kept (projectionDivergenceFree,
+// src/projectionDivergenceFree.cu:21-72).
+extern "C" int orc_project(orc_ctx* c, const double* Win, double* vout) {
+  std::vector<double> W[3], V[3];
+  for (int a = 0; a < 3; ++a) W[a].assign(Win + (size_t)a * c->N, Win + (size_t)(a + 1) * c->N);
+  solve(c, W, V, false, true);
+  for (int a = 0; a < 3; ++a) memcpy(vout + (size_t)a * c->N, V[a].data(), sizeof(double) * c->N);
+  return 1;
+}
 extern "C" int orc_solve_stokes(orc_ctx* c, const double* Win, double* vout) {
   std::vector<double> W[3], V[3];
   for (int a = 0; a < 3; ++a) W[a].assign(Win + (size_t)a * c->N, Win + (size_t)(a + 1) * c->N);
@@ -1151,14 +1161,83 @@ static void qnb_half_positions(orc_ctx* c, const std::vector<double>* v, double 
   }
 }
 
+// Interpolation of a field at every particle of q: out[a][i] = (J(q) field)_a(i).
+static void interp_all(const orc_ctx* c, const std::vector<double>* q, const std::vector<double>* field,
+                       std::vector<double>* out) {
+  const int np = c->p.np;
+  for (int a = 0; a < 3; ++a) out[a].assign(np, 0.0);
+  for (int i = 0; i < np; ++i) {
+    double qp[3] = {q[0][i], q[1][i], q[2][i]}, up[3];
+    interp_one(c, qp, field, up);
+    for (int a = 0; a < 3; ++a) out[a][i] = up[a];
+  }
+}
+// field = scale * sum_p phi(q_p) val_p (S with the 1/dV of the caller's scale).
+static void spread_all(const orc_ctx* c, const std::vector<double>* q, const std::vector<double>* val,
+                       double scale, std::vector<double>* field) {
+  for (int a = 0; a < 3; ++a) field[a].assign(c->N, 0.0);
+  for (int i = 0; i < c->p.np; ++i) {
+    double qp[3] = {q[0][i], q[1][i], q[2][i]}, vp[3] = {val[0][i], val[1][i], val[2][i]};
+    spread_one(c, qp, vp, scale, field);
+  }
+}
+
+// Excess-mass corrections of one QNB step (mass != 0), after the unperturbed prediction vt
+// (src/runSchemeQuasiNeutrallyBuoyant.cu:336-656, STEPS 4-9 and 11):
+//   du  = dulag + (J(qh) - J(q)) v                      kernelCalculateDeltau
+//   dp  = m_e (u - J(qh) vt - du)                       kernelCalculateDeltap
+//   dvt = P [sum_p phi(qh) f dp], f = 1.5 V_p / (1.5 V_p dV rho + m_e)
+//                                                       kernelSpreadDeltap, projectionDivergenceFree
+//   g   = sum_p phi(qh) (dp - m_e J(qh) dvt) / (rho dV) kernelSpreadDeltapMinusJTildev
+//   u'  = J(qh) vt + J(qh) dvt + du                     updateParticleVelocityme
+//   dv  = (I - dt nu/2 L)^-1 P g                        kernelUpdateVIncompressible
+// (src/quasiNeutrallyBuoyantFunctions2.cu:5-2290,2679-3149).  Returns dv; u_new receives u'.
+// With `first`, the particle velocity is left as it is (the bootstrap predictor skips STEP 8,
+// src/firstStepQuasiNeutrallyBuoyant.cu:365-388).
+static void qnb_mass_correction(orc_ctx* c, const std::vector<double>* qh, const std::vector<double>* vt,
+                                const std::vector<double>* dulag, std::vector<double>* dv,
+                                std::vector<double>* u_new) {
+  const int np = c->p.np;
+  const double me = c->p.mass, rho = c->p.densfluid, dV = c->dV, Vp = c->p.volume_particle;
+  std::vector<double> jh[3], j0[3], du[3], dp[3], jvt[3], dvt[3], jdvt[3], s[3], g[3];
+  interp_all(c, qh, c->v, jh);
+  interp_all(c, c->q, c->v, j0);
+  interp_all(c, qh, vt, jvt);
+  for (int a = 0; a < 3; ++a) {
+    du[a].resize(np);
+    dp[a].resize(np);
+    for (int i = 0; i < np; ++i) {
+      du[a][i] = dulag[a][i] + jh[a][i] - j0[a][i];
+      dp[a][i] = me * (c->u[a][i] - jvt[a][i] - du[a][i]);
+    }
+  }
+  const double f = 1.5 * Vp / (1.5 * Vp * dV * rho + me);  // src/quasiNeutrallyBuoyantFunctions2.cu:1332-1335
+  spread_all(c, qh, dp, f, s);
+  solve(c, s, dvt, false, true);
+  interp_all(c, qh, dvt, jdvt);
+  for (int a = 0; a < 3; ++a) {
+    s[a].resize(np);
+    u_new[a].resize(np);
+    for (int i = 0; i < np; ++i) {
+      s[a][i] = dp[a][i] - me * jdvt[a][i];
+      u_new[a][i] = jvt[a][i] + jdvt[a][i] + du[a][i];
+    }
+  }
+  spread_all(c, qh, s, 1.0 / (rho * dV), g);
+  solve(c, g, dv, false, false);
+}
+
 static void step_qnb(orc_ctx* c) {
   const int N = c->N, np = c->p.np;
   const double dV = c->dV, rho = c->p.densfluid, dt = c->p.dt;
+  const bool mass = c->p.mass != 0.0 && np > 0;
   const double fact1 = sqrt(4.0 * c->p.temperature * c->p.shearviscosity * dt / (dV * rho * rho));
   const double fact4 = sqrt(2.0 * c->p.temperature * c->p.shearviscosity * dt / (dV * rho * rho));
   std::vector<double> Z;
   fluid_noise(c->p.seed, c->step, 0, N, Z);
   std::vector<double> qh[3], f[3], W[3], A[3], vt[3], vh[3];
+  if (mass && c->dulag[0].size() != (size_t)np)
+    for (int a = 0; a < 3; ++a) c->dulag[a].assign(np, 0.0);
   // q^{n+1/2} = q^n + dt/2 J(q^n) v^n (src/quasiNeutrallyBuoyantFunctions.cu:2618-2624)
   qnb_half_positions(c, c->v, 0.5, c->q, c->q, qh);
   qnb_forcing(c, qh, f);
@@ -1176,9 +1255,26 @@ static void step_qnb(orc_ctx* c) {
       for (int i = 0; i < N; ++i) W[a][i] = base[a][i] - A[a][i];
     }
     solve(c, W, vt, false, false);
+    std::vector<double> dvp[3];
+    if (mass) {
+      // the predictor's excess-mass corrections, with no lagged term yet
+      // (src/firstStepQuasiNeutrallyBuoyant.cu:84,229-424)
+      std::vector<double> zero[3], unused[3];
+      for (int a = 0; a < 3; ++a) zero[a].assign(np, 0.0);
+      qnb_mass_correction(c, qh, vt, zero, dvp, unused);
+    }
     for (int a = 0; a < 3; ++a) {
       vh[a].resize(N);
-      for (int i = 0; i < N; ++i) vh[a][i] = 0.5 * (c->v[a][i] + vt[a][i]);
+      for (int i = 0; i < N; ++i) vh[a][i] = 0.5 * (c->v[a][i] + vt[a][i] + (mass ? dvp[a][i] : 0.0));
+    }
+    if (mass) {
+      // lagged term for the corrector: J(q') 1/2 dt nu L dv_p at q' = q^n + dt/2 J(q^{n+1/2}) v^{n+1/2}
+      // (findNeighborParticlesQuasiNeutrallyBuoyantTEST4_3, laplacianDeltaV_2,
+      // interpolateLaplacianDeltaV_2: src/firstStepQuasiNeutrallyBuoyant.cu:473-560)
+      std::vector<double> qp2[3], ldv[3];
+      qnb_half_positions(c, vh, 0.5, c->q, qh, qp2);
+      lap_term(c, dvp, ldv);
+      interp_all(c, qp2, ldv, c->dulag);
     }
     std::vector<double> Ah[3];
     adv_term(c, vh, Ah);  // calculateAdvectionFluid at v^{n+1/2} (src/calculateAdvectionFluid.cu:21-114)
@@ -1196,19 +1292,37 @@ static void step_qnb(orc_ctx* c) {
     }
   }
   solve(c, W, vt, false, false);
-  // particle velocity u = J(q^{n+1/2}) v~ (updateParticleVelocityme0)
-  for (int i = 0; i < np; ++i) {
-    double qp[3] = {qh[0][i], qh[1][i], qh[2][i]}, up[3];
-    interp_one(c, qp, vt, up);
-    for (int a = 0; a < 3; ++a) c->u[a][i] = up[a];
+  std::vector<double> dv[3];
+  if (mass) {
+    std::vector<double> un[3];
+    qnb_mass_correction(c, qh, vt, c->dulag, dv, un);
+    for (int a = 0; a < 3; ++a) c->u[a] = un[a];
+  } else {
+    // particle velocity u = J(q^{n+1/2}) v~ (updateParticleVelocityme0)
+    for (int i = 0; i < np; ++i) {
+      double qp[3] = {qh[0][i], qh[1][i], qh[2][i]}, up[3];
+      interp_one(c, qp, vt, up);
+      for (int a = 0; a < 3; ++a) c->u[a][i] = up[a];
+    }
   }
-  // v^{n+1/2}; q^{n+1} = q^n + dt J(q^{n+1/2}) v^{n+1/2}
+  // v^{n+1} = v~ + dv; v^{n+1/2} = (v^n + v^{n+1}) / 2; q^{n+1} = q^n + dt J(q^{n+1/2}) v^{n+1/2}
+  // (calculateVelocityAtHalfTimeStep, updateFluidQuasiNeutrallyBuoyantSemiImplicit)
+  if (mass)
+    for (int a = 0; a < 3; ++a)
+      for (int i = 0; i < N; ++i) vt[a][i] += dv[a][i];
   for (int a = 0; a < 3; ++a) {
     vh[a].resize(N);
     for (int i = 0; i < N; ++i) vh[a][i] = 0.5 * (c->v[a][i] + vt[a][i]);
   }
   std::vector<double> qn[3];
   qnb_half_positions(c, vh, 1.0, c->q, qh, qn);
+  if (mass) {
+    // next step's lagged term J(q^{n+1/2}) 1/2 dt nu L dv (laplacianDeltaV, interpolateLaplacianDeltaV,
+    // src/quasiNeutrallyBuoyantFunctions.cu:5294-5795)
+    std::vector<double> ldv[3];
+    lap_term(c, dv, ldv);
+    interp_all(c, qh, ldv, c->dulag);
+  }
   for (int a = 0; a < 3; ++a) { c->q[a] = qn[a]; c->v[a] = vt[a]; }
 }
 

@@ -28,6 +28,13 @@
  *   QNB step              mass=0 reduces to the incompressible step with no particles;
  *                         particle velocities = J v; divergence; at kT = 0 the bootstrap and
  *                         Adams-Bashforth steps over the pinned blocks
+ *   QNB excess mass       m_e -> 0 reproduces the mass-0 step; total momentum
+ *                         rho dV sum v + m_e sum u is conserved; at kT = 0 the bootstrap and
+ *                         Adams-Bashforth steps with the excess-mass corrections over the
+ *                         pinned blocks (spread, interpolation, projection, Crank-Nicolson
+ *                         solve, Laplacian and advection terms)
+ *   orc_project           idempotent, divergence-free output, kills discrete gradients,
+ *                         keeps divergence-free fields and the mean
  */
 #ifndef FHD_ORACLE_H
 #define FHD_ORACLE_H
@@ -84,6 +91,8 @@ int orc_divergence(orc_ctx* c, const double* v, double* div);     /* div: N */
 int orc_solve_incompressible(orc_ctx* c, const double* W, double* v);
 /* One Stokes solve: v = P W / (eta |g|^2), mode 0 zero (kernelUpdateVstokesLimit). */
 int orc_solve_stokes(orc_ctx* c, const double* W, double* v);
+/* projection onto divergence-free fields, mode 0 kept (projectionDivergenceFree) */
+int orc_project(orc_ctx* c, const double* W, double* v);
 /* Stencil terms: 0.5 dt (eta/rho) L v; advection (dt/4) D[(v+v')(v+v')]; noise divergence. */
 int orc_laplacian_term(orc_ctx* c, const double* v, double* out);
 int orc_advection_term(orc_ctx* c, const double* v, double* out);

@@ -26,7 +26,7 @@ def lib():
         for name in ("orc_set_velocity", "orc_get_velocity", "orc_set_particles", "orc_get_particles",
                      "orc_get_particle_velocities", "orc_get_forces", "orc_divergence"):
             getattr(L, name).argtypes = [C.c_void_p, dp]
-        for name in ("orc_solve_incompressible", "orc_solve_stokes", "orc_wca", "orc_bonded"):
+        for name in ("orc_solve_incompressible", "orc_solve_stokes", "orc_project", "orc_wca", "orc_bonded"):
             getattr(L, name).argtypes = [C.c_void_p, dp, dp]
         L.orc_interp.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_spread.argtypes = [C.c_void_p, dp, dp, dp]
@@ -200,6 +200,12 @@ class Oracle:
         W = np.ascontiguousarray(W, dtype=np.float64)
         v = np.zeros_like(W)
         lib().orc_solve_incompressible(self.h, _p(W), _p(v))
+        return v
+
+    def project(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        v = np.zeros_like(W)
+        lib().orc_project(self.h, _p(W), _p(v))
         return v
 
     def solve_stokes(self, W):
