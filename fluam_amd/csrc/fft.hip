@@ -185,6 +185,7 @@ FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, cons
   // Fourier-space operator in the staggered (shifted) basis; 1/N normalisation folded in.
   const float gy = a.gy[ky];
   const float2 ey = a.ey[ky];
+  const Key key = a.knoise ? resolve_key(a.key) : a.key;
   for (int e = threadIdx.x; e < mz * T; e += blockDim.x) {
     int z = e / T, t = e - z * T;
     int kx = kx0 + t;
@@ -193,9 +194,19 @@ FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, cons
     float2 ex = a.ex[kx];
     float gz = gzs[z];
     float2 ez = ezs[z];
-    float2 w0 = cmul(buf[e], cconj(ex));
-    float2 w1 = cmul(buf[mz * T + e], cconj(ey));
-    float2 w2 = cmul(buf[2 * mz * T + e], cconj(ez));
+    float2 b0 = buf[e], b1 = buf[mz * T + e], b2 = buf[2 * mz * T + e];
+    if (a.knoise) {
+      // the stochastic stress drawn in Fourier space (Stokes limit)
+      float2 nw[3];
+      kspace_stress_div(key, kx, ky, z, a.g.mx, my, mz, a.knoise, a.kn_f1, a.kn_f4, a.kn_sqrt_half_n, gx, gy, gz,
+                        ex, ey, ez, nw);
+      b0 = cadd(b0, nw[0]);
+      b1 = cadd(b1, nw[1]);
+      b2 = cadd(b2, nw[2]);
+    }
+    float2 w0 = cmul(b0, cconj(ex));
+    float2 w1 = cmul(b1, cconj(ey));
+    float2 w2 = cmul(b2, cconj(ez));
     float L = -(gx * gx + gy * gy + gz * gz);
     float2 o0, o1, o2;
     if (kx == 0 && ky == 0 && z == 0) {

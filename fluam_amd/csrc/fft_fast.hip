@@ -209,6 +209,7 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
   if (a.zmode == 0) {
     const float gy = a.gy[ky];
     const float2 ey = a.ey[ky];
+    const Key key = a.knoise ? resolve_key(a.key) : a.key;
     for (int e = threadIdx.x; e < MZ * TC; e += NT) {
       const int t = e / MZ, z = e - t * MZ;
       const int kx = t < T ? tile * T + t : ntile * T + tile;
@@ -217,9 +218,18 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
       const float gz = gzs[z];
       const float2 ez = ezs[z];
       const int i0 = (0 * TM + t) * LS + z, i1 = (1 * TM + t) * LS + z, i2 = (2 * TM + t) * LS + z;
-      const float2 w0 = cmul(buf[i0], cconj(ex));
-      const float2 w1 = cmul(buf[i1], cconj(ey));
-      const float2 w2 = cmul(buf[i2], cconj(ez));
+      float2 b0 = buf[i0], b1 = buf[i1], b2 = buf[i2];
+      if (a.knoise) {
+        float2 nw[3];
+        kspace_stress_div(key, kx, ky, z, a.g.mx, my, MZ, a.knoise, a.kn_f1, a.kn_f4, a.kn_sqrt_half_n, gx, gy,
+                          gz, ex, ey, ez, nw);
+        b0 = cadd(b0, nw[0]);
+        b1 = cadd(b1, nw[1]);
+        b2 = cadd(b2, nw[2]);
+      }
+      const float2 w0 = cmul(b0, cconj(ex));
+      const float2 w1 = cmul(b1, cconj(ey));
+      const float2 w2 = cmul(b2, cconj(ez));
       const float L = -(gx * gx + gy * gy + gz * gz);
       float2 o0, o1, o2;
       if (kx == 0 && ky == 0 && z == 0) {

@@ -21,8 +21,8 @@ FHD_DEV int xcd_item(int b, int n) {
 enum { kErrNonFinite = 1, kErrClamped = 2 };
 
 // Noise streams (counter word 1 = stream << 4 | sub).
-enum { FHD_STREAM_FLUID = 1, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4, FHD_STREAM_PART2 = 5,
-       FHD_STREAM_INIT_POS = 8, FHD_STREAM_INIT_FLUID = 9 };
+enum { FHD_STREAM_FLUID = 1, FHD_STREAM_FLUID_K = 2, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4,
+       FHD_STREAM_PART2 = 5, FHD_STREAM_INIT_POS = 8, FHD_STREAM_INIT_FLUID = 9 };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products map to
@@ -89,6 +89,72 @@ FHD_DEV void draw_normals4(const Key& key, uint32_t index, uint32_t stream, uint
   float2 a = box_muller(w.x, w.y);
   float2 b = box_muller(w.z, w.w);
   n[0] = a.x; n[1] = a.y; n[2] = b.x; n[3] = b.y;
+}
+
+// ---------------------------------------------------------------------------------------
+// Stochastic stress of the Stokes limit drawn in Fourier space.  Sigma is white: the DFT of a
+// real field of N independent N(0,1) values per component has, at each mode k with -k != k,
+// real and imaginary parts N(0, N/2) independent of every other mode except -k, where it is
+// the conjugate; at self-conjugate modes it is real N(0, N).  The half spectrum kx in [0, mx/2]
+// holds one mode of every pair except on the planes kx = 0 and kx = mx/2, where (ky, kz) and
+// (-ky, -kz) are both stored: there the member with the smaller ky + my kz draws the normals
+// and the other takes the conjugate.  Draws: stream FLUID_K, counter index kx + P (ky + my kz)
+// of the drawing member, subs 3 s .. 3 s + 2 for stress set s (12 normals: the real and
+// imaginary parts of XX, XY, XZ, YY, YZ, ZZ).  The divergence of the staggered stress
+// (diagonal at cell centres, off-diagonal at edges, src/kernelConstructW.cu:148-198) in the
+// stored basis is
+//   w_x = i [f1 g_x e_x S_xx + f4 g_y e_y* S_xy + f4 g_z e_z* S_xz]   (and cyclically),
+// g_a = (2/h_a) sin(pi k_a/m_a), e_a = e^{i pi k_a/m_a}.  nset 1: Sigma_1; 2: Sigma_1 + Sigma_2.
+FHD_DEV void kspace_stress_div(const Key& key, int kx, int ky, int kz, int mx, int my, int mz, int nset,
+                               float f1, float f4, float sqrt_half_n, float gx, float gy, float gz,
+                               float2 ex, float2 ey, float2 ez, float2 w[3]) {
+  int cy = ky, cz = kz;
+  bool conj = false, selfc = false;
+  if (kx == 0 || 2 * kx == mx) {
+    const int py = ky == 0 ? 0 : my - ky, pz = kz == 0 ? 0 : mz - kz;
+    const int L = ky + my * kz, Lp = py + my * pz;
+    if (Lp == L) {
+      selfc = true;
+    } else if (Lp < L) {
+      cy = py;
+      cz = pz;
+      conj = true;
+    }
+  }
+  const uint32_t index = (uint32_t)(kx + (mx / 2 + 1) * (cy + my * cz));
+  float z[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) z[k] = 0.0f;
+  for (int sset = 0; sset < nset; ++sset)
+#pragma unroll
+    for (int sub = 0; sub < 3; ++sub) {
+      float n[4];
+      draw_normals4(key, index, FHD_STREAM_FLUID_K, (uint32_t)(3 * sset + sub), n);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) z[4 * sub + k] += n[k];
+    }
+  // S_c = sqrt(N/2) (z_re + i z_im); self-conjugate: sqrt(N) z_re = sqrt(N/2) sqrt(2) z_re
+  float2 S[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const float re = z[2 * c], im = z[2 * c + 1];
+    S[c] = selfc ? make_float2(1.41421356237309515f * sqrt_half_n * re, 0.0f)
+                 : make_float2(sqrt_half_n * re, (conj ? -sqrt_half_n : sqrt_half_n) * im);
+  }
+  // i * (a e) * S for a real a and phase e: products of complex numbers
+  auto term = [](float a, float2 e, float2 Sv) {
+    const float2 p = make_float2(a * (e.x * Sv.x - e.y * Sv.y), a * (e.x * Sv.y + e.y * Sv.x));
+    return make_float2(-p.y, p.x);  // times i
+  };
+  const float2 exc = make_float2(ex.x, -ex.y), eyc = make_float2(ey.x, -ey.y), ezc = make_float2(ez.x, -ez.y);
+  // S: 0 XX, 1 XY, 2 XZ, 3 YY, 4 YZ, 5 ZZ
+  float2 a, b, c;
+  a = term(f1 * gx, ex, S[0]); b = term(f4 * gy, eyc, S[1]); c = term(f4 * gz, ezc, S[2]);
+  w[0] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
+  a = term(f1 * gy, ey, S[3]); b = term(f4 * gx, exc, S[1]); c = term(f4 * gz, ezc, S[4]);
+  w[1] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
+  a = term(f1 * gz, ez, S[5]); b = term(f4 * gx, exc, S[2]); c = term(f4 * gy, eyc, S[4]);
+  w[2] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
 }
 
 // ---------------------------------------------------------------------------------------

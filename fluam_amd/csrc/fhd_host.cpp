@@ -335,6 +335,10 @@ void build_tables(fhd_ctx* c) {
   a.btile = 8;
   a.zmode = 0;
   a.ncomp = 3;
+  a.knoise = 0;
+  a.kn_f1 = a.kn_f4 = 0.0f;
+  a.kn_sqrt_half_n = (float)sqrt(0.5 * (double)c->g.N);
+  memset(&a.key, 0, sizeof(a.key));
   const char* gen = getenv("FHD_FFT_GENERIC");  // 1: use the runtime-size FFT kernels everywhere
   a.generic = (gen && gen[0] == '1') ? 1 : 0;
   a.stokes = 0;
@@ -678,30 +682,19 @@ void step_incompressible(fhd_ctx* c) {
 
 // Stokes limit, midpoint: src/runSchemeStokesLimit.cu:163-276,
 // src/boundaryParticlesFunctionStokesLimit.cu:21-249.
-// W = S F (+ RFD drift) + noise D.Sigma.  The noise part needs nothing but the step counter, so
-// it is built into W on the side stream s2 while the particle kernels (sorting, WCA, source
-// records) run on the main stream; the spreading pass then adds S F into W node by node
-// (the same single float addition as a combined pass, so the same bits).  W is free for the
-// next noise pass once the xy-forward FFT pass has read it.  When profiling, everything runs
-// on the main stream so per-kernel times stay separable.
-bool noise_on_side(const fhd_ctx* c) { return !c->profiling && c->noise_overlap; }
-void stokes_noise_begin(fhd_ctx* c, int noise, float f1, float f4) {
-  const bool side = noise_on_side(c);
-  if (side) {
-    HIPCHK(hipEventRecord(c->ev_fork, c->s));
-    HIPCHK(hipStreamWaitEvent(c->s2, c->ev_fork, 0));
-  }
-  FluidWArgs w = base_w(c);
-  w.trace_free = 0;
-  w.noise = noise;
-  w.f1 = f1;
-  w.f4 = f4;
-  launch_fluid_w(w, side ? c->s2 : c->s);
-  mark(c, "fluid_w");
-  if (side) HIPCHK(hipEventRecord(c->ev_noise, c->s2));
-}
-void stokes_noise_join(fhd_ctx* c) {
-  if (noise_on_side(c)) HIPCHK(hipStreamWaitEvent(c->s, c->ev_noise, 0));
+// W = S F (+ RFD drift) + noise D.Sigma.  The stochastic stress is drawn in Fourier space by the
+// z pass (kspace_stress_div: the transform of a white stress field, with the staggered
+// divergence applied as its Fourier symbol), so W in real space holds only the particle terms.
+void fft_solve_stokes(fhd_ctx* c, int knoise, float f1, float f4) {
+  FftArgs a = c->fa;
+  a.stokes = 1;
+  a.cn_coef = 0.0f;
+  a.knoise = (f1 != 0.0f || f4 != 0.0f) ? knoise : 0;
+  a.kn_f1 = f1;
+  a.kn_f4 = f4;
+  a.key = make_key(c);
+  launch_fft_z_solve(a, c->spec, c->s);
+  mark(c, "fft_z_solve");
 }
 
 void step_stokes(fhd_ctx* c) {
@@ -711,8 +704,7 @@ void step_stokes(fhd_ctx* c) {
   const float f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity / (dV * dt));
   const float fact2 = (float)sqrt(p.temperature * p.extra_mobility * dt);
   const bool xm = p.extra_mobility != 0.0;
-  // solve 1: W = S F^n + sqrt(2) D.Sigma_1
-  stokes_noise_begin(c, 1, (float)(sqrt(2.0) * f1), (float)(sqrt(2.0) * f4));
+  // solve 1: W = S F^n, and sqrt(2) D.Sigma_1 in Fourier space
   if (c->np > 0) {
     sort_set(c, 0);
     wca(c, 0, c->F[0]);
@@ -723,19 +715,13 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[0].val = c->F[0];
     sa.grp[0].coef = 1.0f;
     sa.out = c->W;
-    sa.out_add = c->W;
-    stokes_noise_join(c);
+    sa.out_add = nullptr;
     run_spread(c, sa);
   } else {
-    stokes_noise_join(c);
+    launch_zero_f(c->W, 3 * (size_t)c->g.N, c->s);
   }
   fft_forward(c, c->W);
-  // solve 2's noise: W = sqrt(1/2) D.(Sigma_1 + Sigma_2).  With particles it starts after the
-  // first interpolation, beside the latency-bound sort / WCA / record kernels rather than the
-  // FFT passes (measured 5 us per step faster); without particles it starts at once.
-  const float g1 = (float)(sqrt(0.5) * f1), g4 = (float)(sqrt(0.5) * f4);
-  if (c->np == 0) stokes_noise_begin(c, 2, g1, g4);
-  fft_solve(c, 1, 0.0f);
+  fft_solve_stokes(c, 1, (float)(sqrt(2.0) * f1), (float)(sqrt(2.0) * f4));
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
   if (c->np > 0) {
     InterpArgs ia = interp_base(c, 0);
@@ -754,7 +740,6 @@ void step_stokes(fhd_ctx* c) {
     }
     launch_interp(ia, c->s);
     mark(c, "interp");
-    stokes_noise_begin(c, 2, g1, g4);
     sort_set(c, 1);
     wca(c, 1, c->F[1]);
     SpreadArgs sa = spread_base(c);
@@ -769,14 +754,14 @@ void step_stokes(fhd_ctx* c) {
     sa.grp[1].disp = (float)(0.5 * p.rfd_delta);
     sa.grp[1].rfd_quirk = p.rfd_mode == FHD_RFD_REFERENCE;
     sa.out = c->W;
-    sa.out_add = c->W;
-    stokes_noise_join(c);
+    sa.out_add = nullptr;
     run_spread(c, sa);
   } else {
-    stokes_noise_join(c);
+    launch_zero_f(c->W, 3 * (size_t)c->g.N, c->s);
   }
+  // solve 2: W = S F^{n+1/2} + RFD drift, and sqrt(1/2) D.(Sigma_1 + Sigma_2) in Fourier space
   fft_forward(c, c->W);
-  fft_solve(c, 1, 0.0f);
+  fft_solve_stokes(c, 2, (float)(sqrt(0.5) * f1), (float)(sqrt(0.5) * f4));
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);
   if (c->np > 0) {
     InterpArgs ia = interp_base(c, 1);

@@ -20,6 +20,11 @@ struct FftArgs {
   int btile;                      // kx columns per pass-B workgroup (size-specialised: 2, 4, 8 or 16)
   int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
   int ncomp;                      // velocity components transformed (3; 1 for a single field, transform-only modes)
+  int knoise;                     // Stokes: stochastic stress drawn in Fourier space and added before the
+                                  // solve (zmode 0): 0 none, 1 Sigma_1, 2 Sigma_1 + Sigma_2
+  float kn_f1, kn_f4;             // its diagonal / off-diagonal prefactors
+  float kn_sqrt_half_n;           // sqrt(N / 2)
+  Key key;
   int generic;                    // 1: force the generic (runtime-size) kernels
   int stokes;                     // 1: v = P W / (eta |g|^2), mode 0 -> 0
   float eta;

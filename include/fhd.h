@@ -135,10 +135,9 @@ int fhd_sync(fhd_ctx* c);
 /* Replay the steady-state step from captured HIP graphs (on != 0) instead of launching each
  * kernel.  No fluam counterpart (fluam launches every kernel each step); results are identical. */
 int fhd_set_graph(fhd_ctx* c, int on);
-/* Stokes limit: build the noise part of W on a second stream while the particle kernels run
- * (on != 0, the default), or on the context's own stream.  No fluam counterpart (fluam runs one
- * stream); results are identical either way.  Callers that keep many contexts busy at once (a
- * serving loop) already fill the GPU and gain from turning it off. */
+/* Kept for callers of an earlier version of the library: the Stokes-limit step used to build
+ * its noise field on a second stream.  The stress is now drawn in Fourier space inside the
+ * solve, so the setting has no effect; it is accepted and returns 1.  No fluam counterpart. */
 int fhd_set_noise_overlap(fhd_ctx* c, int on);
 
 /* Equal-time structure factor of the velocity, computed on the GPU from the current field

@@ -84,7 +84,7 @@ static void uniforms4(uint64_t seed, int64_t step, int stream, int sub, uint32_t
   for (int k = 0; k < 4; ++k) out[k] = (double)(w[k] >> 8) * (1.0 / 16777216.0);
 }
 
-enum { STREAM_FLUID1 = 1, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
+enum { STREAM_FLUID1 = 1, STREAM_FLUID_K = 2, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
        STREAM_INIT_POS = 8, STREAM_INIT_FLUID = 9 };
 
 // Stress noise of one cell and step: 12 normals n_k, k = 0..11, where n_k is word k%4 of the
@@ -1061,15 +1061,108 @@ static void rfd_spread(orc_ctx* c, const std::vector<double>* q, const std::vect
   }
 }
 
+// ---- The Stokes step's stochastic stress, drawn in Fourier space ----
+// Sigma is white: the DFT of N independent N(0,1) values per component has, at a mode k with
+// -k != k, real and imaginary parts N(0, N/2), independent of all modes but -k, where it is the
+// conjugate; self-conjugate modes are real N(0, N).  The modes with kx in [0, mx/2] draw their
+// values (on the planes kx = 0 and kx = mx/2 the member of each (ky, kz), (-ky, -kz) pair with
+// the smaller ky + my kz draws, the other is its conjugate): stream STREAM_FLUID_K, counter index
+// kx + P (ky + my kz), subs 3 s .. 3 s + 2 for stress set s, the 12 normals being the real and
+// imaginary parts of XX, XY, XZ, YY, YZ, ZZ; modes with kx > mx/2 are conjugates of modes
+// -k.  S[comp] gets the full spectrum of Sigma_1 (nset 1) or Sigma_1 + Sigma_2 (nset 2).
+static void kspace_sigma(const orc_ctx* c, int64_t step, int nset, std::vector<cplx>* S) {
+  const int mx = c->mx, my = c->my, mz = c->mz, N = c->N, P = mx / 2 + 1;
+  const double shn = sqrt(0.5 * (double)N);
+  for (int k = 0; k < 6; ++k) S[k].assign(N, cplx(0.0, 0.0));
+#pragma omp parallel for schedule(static)
+  for (int kz = 0; kz < mz; ++kz)
+    for (int ky = 0; ky < my; ++ky)
+      for (int kx = 0; kx < mx; ++kx) {
+        int hx = kx, hy = ky, hz = kz;
+        bool conj = false;
+        if (2 * kx > mx) {
+          hx = mx - kx;
+          hy = (my - ky) % my;
+          hz = (mz - kz) % mz;
+          conj = true;
+        }
+        int cy = hy, cz = hz;
+        bool selfc = false;
+        if (hx == 0 || 2 * hx == mx) {
+          const int py = (my - hy) % my, pz = (mz - hz) % mz;
+          const int L = hy + my * hz, Lp = py + my * pz;
+          if (Lp == L) {
+            selfc = true;
+          } else if (Lp < L) {
+            cy = py;
+            cz = pz;
+            conj = !conj;
+          }
+        }
+        const uint32_t index = (uint32_t)(hx + P * (cy + my * cz));
+        double z[12] = {0};
+        for (int s = 0; s < nset; ++s)
+          for (int sub = 0; sub < 3; ++sub) {
+            double n[4];
+            orc_normals4(c->p.seed, step, STREAM_FLUID_K, 3 * s + sub, index, n);
+            for (int k = 0; k < 4; ++k) z[4 * sub + k] += n[k];
+          }
+        const size_t i = (size_t)kx + (size_t)mx * (ky + (size_t)my * kz);
+        for (int k = 0; k < 6; ++k) {
+          cplx val = selfc ? cplx(sqrt(2.0) * shn * z[2 * k], 0.0) : cplx(shn * z[2 * k], shn * z[2 * k + 1]);
+          S[k][i] = conj ? std::conj(val) : val;
+        }
+      }
+}
+
+// The divergence of the staggered stress in Fourier space (diagonal at cell centres,
+// off-diagonal at edges, src/kernelConstructW.cu:148-198):
+//   w_x = i [f1 g_x e_x S_xx + f4 g_y e_y* S_xy + f4 g_z e_z* S_xz]  (and cyclically),
+// g_a = (2/h_a) sin(pi k_a/m_a), e_a = e^{i pi k_a/m_a}; returned in real space,
+// w = IDFT(w_hat) / N (real: w_hat is Hermitian).
+static void kspace_stress_div(orc_ctx* c, int64_t step, int nset, double f1, double f4, std::vector<double>* w) {
+  const int mx = c->mx, my = c->my, mz = c->mz, N = c->N;
+  std::vector<cplx> S[6], Wk[3];
+  kspace_sigma(c, step, nset, S);
+  const double h[3] = {c->dx, c->dy, c->dz};
+  const int m[3] = {mx, my, mz};
+  std::vector<double> g[3];
+  std::vector<cplx> e[3];
+  for (int a = 0; a < 3; ++a) {
+    g[a].resize(m[a]);
+    e[a].resize(m[a]);
+    for (int k = 0; k < m[a]; ++k) {
+      g[a][k] = 2.0 / h[a] * sin(M_PI * k / m[a]);
+      e[a][k] = std::polar(1.0, M_PI * k / m[a]);
+    }
+  }
+  for (int a = 0; a < 3; ++a) Wk[a].assign(N, cplx(0.0, 0.0));
+  const cplx I(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int kz = 0; kz < mz; ++kz)
+    for (int ky = 0; ky < my; ++ky)
+      for (int kx = 0; kx < mx; ++kx) {
+        const size_t i = (size_t)kx + (size_t)mx * (ky + (size_t)my * kz);
+        const cplx ex = e[0][kx], ey = e[1][ky], ez = e[2][kz];
+        const double gx = g[0][kx], gy = g[1][ky], gz = g[2][kz];
+        // S: 0 XX, 1 XY, 2 XZ, 3 YY, 4 YZ, 5 ZZ
+        Wk[0][i] = I * (f1 * gx * ex * S[0][i] + f4 * gy * std::conj(ey) * S[1][i] + f4 * gz * std::conj(ez) * S[2][i]);
+        Wk[1][i] = I * (f1 * gy * ey * S[3][i] + f4 * gx * std::conj(ex) * S[1][i] + f4 * gz * std::conj(ez) * S[4][i]);
+        Wk[2][i] = I * (f1 * gz * ez * S[5][i] + f4 * gx * std::conj(ex) * S[2][i] + f4 * gy * std::conj(ey) * S[4][i]);
+      }
+  for (int a = 0; a < 3; ++a) {
+    c->fft.run(Wk[a].data(), +1);
+    w[a].resize(N);
+    for (int i = 0; i < N; ++i) w[a][i] = Wk[a][i].real() / N;
+  }
+}
+
 static void step_stokes(orc_ctx* c) {
   const int N = c->N, np = c->p.np;
   const double dt = c->p.dt, dV = c->dV;
   const double fact1 = sqrt(4.0 * c->p.temperature * c->p.shearviscosity / (dV * dt));
   const double fact4 = sqrt(2.0 * c->p.temperature * c->p.shearviscosity / (dV * dt));
   const double fact2 = sqrt(c->p.temperature * c->p.extra_mobility * dt);
-  std::vector<double> Z1, Z2;
-  fluid_noise(c->p.seed, c->step, 0, N, Z1);
-  fluid_noise(c->p.seed, c->step, 1, N, Z2);
   std::vector<double> Wr[3], P1[3], P2[3];
   for (int a = 0; a < 3; ++a) { Wr[a].resize(np); P1[a].resize(np); P2[a].resize(np); }
   for (int i = 0; i < np; ++i) {
@@ -1088,9 +1181,13 @@ static void step_stokes(orc_ctx* c) {
     double Fp[3] = {c->F[0][i], c->F[1][i], c->F[2][i]};
     spread_one(c, qp, Fp, 1.0 / dV, f);
   }
-  // 2. W = S F + sqrt(2) D.Sigma1 ; solve
-  for (int a = 0; a < 3; ++a) W[a] = f[a];
-  noise_div(c, Z1, sqrt(2.0) * fact1, sqrt(2.0) * fact4, false, W);
+  // 2. W = S F + sqrt(2) D.Sigma1 (the stress drawn in Fourier space) ; solve
+  std::vector<double> nz[3];
+  kspace_stress_div(c, c->step, 1, sqrt(2.0) * fact1, sqrt(2.0) * fact4, nz);
+  for (int a = 0; a < 3; ++a) {
+    W[a] = f[a];
+    for (int i = 0; i < N; ++i) W[a][i] += nz[a][i];
+  }
   solve(c, W, v, true, false);
   // 3. q^{n+1/2} = q^n + dt/2 J(q^n) v (+ extra mobility)
   std::vector<double> qh[3], Fh[3];
@@ -1114,9 +1211,11 @@ static void step_stokes(orc_ctx* c) {
   }
   rfd_spread(c, c->q, Wr, f);
   // 5. W = f + sqrt(1/2) D.(Sigma1 + Sigma2); solve
-  for (int a = 0; a < 3; ++a) W[a] = f[a];
-  noise_div(c, Z1, sqrt(0.5) * fact1, sqrt(0.5) * fact4, false, W);
-  noise_div(c, Z2, sqrt(0.5) * fact1, sqrt(0.5) * fact4, false, W);
+  kspace_stress_div(c, c->step, 2, sqrt(0.5) * fact1, sqrt(0.5) * fact4, nz);
+  for (int a = 0; a < 3; ++a) {
+    W[a] = f[a];
+    for (int i = 0; i < N; ++i) W[a][i] += nz[a][i];
+  }
   solve(c, W, v, true, false);
   // 6. q^{n+1} = q^n + dt J(q^{n+1/2}) v (+ extra mobility)
   for (int i = 0; i < np; ++i) {
@@ -1372,6 +1471,26 @@ extern "C" int orc_noise_divergence(orc_ctx* c, const double* Z6, double fact1, 
   for (int a = 0; a < 3; ++a) O[a].assign(c->N, 0.0);
   noise_div(c, Z, fact1, fact4, trace_free != 0, O);
   pack3(c, O, out);
+  return 1;
+}
+// The Stokes step's stress drawn in Fourier space: sigma6 = IDFT(Sigma_hat) / N (real part; the
+// largest imaginary part goes to *max_imag) and w3 = its divergence, IDFT(w_hat) / N.
+extern "C" int orc_kspace_stress(orc_ctx* c, int64_t step, int nset, double f1, double f4, double* sigma6,
+                                 double* w3, double* max_imag) {
+  std::vector<cplx> S[6];
+  kspace_sigma(c, step, nset, S);
+  double mi = 0.0;
+  for (int k = 0; k < 6; ++k) {
+    c->fft.run(S[k].data(), +1);
+    for (int i = 0; i < c->N; ++i) {
+      sigma6[(size_t)k * c->N + i] = S[k][i].real() / c->N;
+      mi = std::max(mi, fabs(S[k][i].imag()) / c->N);
+    }
+  }
+  *max_imag = mi;
+  std::vector<double> w[3];
+  kspace_stress_div(c, step, nset, f1, f4, w);
+  pack3(c, w, w3);
   return 1;
 }
 extern "C" int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6) {

@@ -99,6 +99,11 @@ int orc_advection_term(orc_ctx* c, const double* v, double* out);
 int orc_noise_divergence(orc_ctx* c, const double* Z6, double fact1, double fact4, int trace_free,
                          double* out);
 int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6); /* which: 0 Sigma_1, 1 Sigma_2 */
+/* Stokes-limit stress drawn in Fourier space (nset 1: Sigma_1, 2: Sigma_1 + Sigma_2): the real
+ * fields sigma6 (6N), their divergence w3 (3N) with prefactors f1, f4, and the largest imaginary
+ * part of the inverse transform of Sigma_hat */
+int orc_kspace_stress(orc_ctx* c, int64_t step, int nset, double f1, double f4, double* sigma6, double* w3,
+                      double* max_imag);
 /* Initial positions used when the caller provides none (seeded uniform / simple cubic). */
 int orc_initial_positions(const fhd_params* p, double* q);
 

@@ -244,6 +244,18 @@ def noise_divergence(o, Z6, fact1, fact4, trace_free):
     return out
 
 
+def kspace_stress(o, step, nset, f1, f4):
+    """The Stokes step's stress drawn in Fourier space: (sigma6, w3, max imaginary part)."""
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    L.orc_kspace_stress.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_double, dp, dp, dp]
+    sig = np.zeros(6 * o.N)
+    w = np.zeros(3 * o.N)
+    mi = C.c_double(0.0)
+    L.orc_kspace_stress(o.h, step, nset, f1, f4, _p(sig), _p(w), C.byref(mi))
+    return sig, w, mi.value
+
+
 def fluid_noise(seed, step, stream, N):
     Z = np.zeros(6 * N)
     _bind_terms().orc_fluid_noise(seed, step, stream, N, _p(Z))

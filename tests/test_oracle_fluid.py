@@ -325,8 +325,9 @@ def test_stokes_stochastic_step_composes_pinned_blocks(tmp_path):
     """The full Stokes-limit step at kT > 0 (src/runSchemeStokesLimit.cu:163-276): solve 1 takes
     W = S F(q) + sqrt(2) D.Sigma_1; solve 2 takes W = S F(q_half) + (kT/delta)[S(q + delta/2 w) w
     - S(q - delta/2 w) w] + sqrt(1/2) D.(Sigma_1 + Sigma_2), with the RFD direction w and the
-    noise slots drawn from the step's counters.  Rebuilt here from the separately pinned noise,
-    noise divergence, spread, Stokes solve and interpolation; it equals the oracle's step."""
+    stress fields drawn from the step's counters.  Rebuilt here from the separately pinned
+    stress fields, noise divergence, spread, Stokes solve and interpolation; it equals the
+    oracle's step."""
     bonds = tmp_path / "traps.dat"
     bonds.write_text("0\n3\n0 0.7 0.0 3.0 -2.0 1.5\n1 0.4 0.5 -1.0 1.0 0.0\n2 1.1 0.0 0.0 0.0 -2.5\n")
     kT, eta, dt = 1e-3, 1.0, 0.2
@@ -340,19 +341,20 @@ def test_stokes_stochastic_step_composes_pinned_blocks(tmp_path):
     dV = 1.0 * 0.75 * 0.75
     f1 = np.sqrt(4 * kT * eta / (dV * dt))
     f4 = np.sqrt(2 * kT * eta / (dV * dt))
-    Z1 = O.fluid_noise(p.seed, 0, 0, N)
-    Z2 = O.fluid_noise(p.seed, 0, 1, N)
+    # the stress fields, drawn in Fourier space (pinned as white N(0,1) fields whose divergence is
+    # the pinned real-space noise divergence, tests/test_oracle_kspace_noise.py)
+    S1, _, _ = O.kspace_stress(o, 0, 1, 1.0, 1.0)
+    S12, _, _ = O.kspace_stress(o, 0, 2, 1.0, 1.0)
     w = np.array([O.normals4(p.seed, 0, 3, 0, i)[:3] for i in range(3)]).T.ravel()  # RFD directions
     delta = p.rfd_delta
 
     def vel(q, W):
         return o.interp(q, o.solve_stokes(W))
 
-    W1 = o.spread(q0, o.bonded(q0)) + O.noise_divergence(o, Z1, np.sqrt(2) * f1, np.sqrt(2) * f4, False)
+    W1 = o.spread(q0, o.bonded(q0)) + O.noise_divergence(o, S1, np.sqrt(2) * f1, np.sqrt(2) * f4, False)
     qh = q0 + 0.5 * dt * vel(q0, W1)
     rfd = o.spread(q0 + 0.5 * delta * w, kT / delta * w) - o.spread(q0 - 0.5 * delta * w, kT / delta * w)
-    W2 = (o.spread(qh, o.bonded(qh)) + rfd + O.noise_divergence(o, Z1, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False)
-          + O.noise_divergence(o, Z2, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False))
+    W2 = o.spread(qh, o.bonded(qh)) + rfd + O.noise_divergence(o, S12, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False)
     q1 = q0 + dt * vel(qh, W2)
     o.step(1)
     got = o.particles()
