@@ -832,14 +832,29 @@ FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const Ax
   int xs[NPTS];
 #pragma unroll
   for (int kx = 0; kx < NPTS; ++kx) xs[kx] = fold(wx.base + kx, g.mx);
+  // a stencil row that does not wrap around the box and ends inside it is read as one 16-byte
+  // load (global loads need only 4-byte alignment); otherwise node by node
+  const bool vec = wx.base >= 0 && wx.base + 4 <= g.mx;
   float sy = 0.0f;
 #pragma unroll
   for (int ky = 0; ky < NPTS; ++ky) {
     const int y = fold(wy.base + ky, g.my);
     const float* row = f + (size_t)g.mx * (y + (size_t)g.my * z);
+    float r[4];
+    if (vec) {
+      typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+      const f4u v = *reinterpret_cast<const f4u*>(row + wx.base);
+      r[0] = v.x;
+      r[1] = v.y;
+      r[2] = v.z;
+      r[3] = v.w;
+    } else {
+#pragma unroll
+      for (int kx = 0; kx < NPTS; ++kx) r[kx] = row[xs[kx]];
+    }
     float sx = 0.0f;
 #pragma unroll
-    for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * row[xs[kx]];
+    for (int kx = 0; kx < NPTS; ++kx) sx += wx.w[kx] * r[kx];
     sy += wy.w[ky] * sx;
   }
   return sy;  // weighted by wz.w[kz] when the rows are combined (static indices, no scratch)
