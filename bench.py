@@ -375,7 +375,8 @@ def main():
                    "cells": [mx, my, mz], "particles": npart,
                    "initial_positions": "seeded uniform-random by random sequential addition (no pair closer "
                                         "than sigma; data.main initialPositions 3)",
-                   "launch": "HIP graph replay of the step" if args.graph else "eager launches"},
+                   "launch": "HIP graph replay of the step" if args.graph else "eager launches",
+                   "schedule": sim.schedule()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": hbm_traffic(dom),
                      "algorithmic_bytes_per_launch": kb, "avg_launch_ms": kms},

@@ -61,6 +61,7 @@ def lib():
               "fhd_get_particle_velocities", "fhd_get_forces", "fhd_max_divergence"):
         getattr(L, n).argtypes = [vp, dp]
     L.fhd_step.argtypes = [vp, C.c_int]
+    L.fhd_get_schedule.argtypes = [vp, C.POINTER(C.c_int)]
     L.fhd_sync.argtypes = [vp]
     L.fhd_set_graph.argtypes = [vp, C.c_int]
     L.fhd_set_noise_overlap.argtypes = [vp, C.c_int]
@@ -283,6 +284,14 @@ class Simulation:
         ms = C.c_double(0)
         self._check(lib().fhd_time_steps(self.h, int(n), C.byref(ms)))
         return ms.value
+
+    def schedule(self):
+        """The autotuned schedule: z-pass column tile, xy and z radix plans, W-kernel z chunks."""
+        out = (C.c_int * 5)()
+        if not lib().fhd_get_schedule(self.h, out):
+            raise FhdError(_err(self.h))
+        return {"z_column_tile": out[0], "xy_radix_plan": out[1], "z_radix_plan": out[2],
+                "w_z_chunk": out[3], "w_z_chunk_batched": out[4]}
 
     def profile(self, n, nkern=32):
         kms = np.zeros(2 * nkern)

@@ -100,4 +100,16 @@ constexpr int plan_radix(int n, int st) {
   }
 }
 
+// Explicit radix plans, 5 bits per stage from the first (0 ends the plan; plan 0 is the default
+// above).  The schedule autotuner picks one per transform length among the candidates below.
+constexpr uint32_t plan2(int r0, int r1) { return (uint32_t)r0 | ((uint32_t)r1 << 5); }
+constexpr uint32_t plan3(int r0, int r1, int r2) { return plan2(r0, r1) | ((uint32_t)r2 << 10); }
+constexpr int plan_at(uint32_t plan, int n, int st) { return plan ? (int)((plan >> (5 * st)) & 31u) : plan_radix(n, st); }
+// Candidate plans: index 0 is the default for every length; 64 and 128 have two more.
+constexpr uint32_t kPlan64[3] = {0u, plan2(4, 16), plan2(8, 8)};
+constexpr uint32_t kPlan128[3] = {0u, plan2(8, 16), plan3(4, 4, 8)};
+constexpr uint32_t plan_choice(int n, int i) {
+  return i == 0 ? 0u : n == 64 ? kPlan64[i] : n == 128 ? kPlan128[i] : 0u;
+}
+
 }  // namespace fhd

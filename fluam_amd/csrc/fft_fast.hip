@@ -54,24 +54,24 @@ FHD_DEV void fstage(float2* buf, int lines, int LS, const float2* tw, float s) {
   __syncthreads();
 }
 
-template <int N, int ST, int NS, int ES, int NT, int MAXL, int TWM>
+template <int N, int ST, int NS, int ES, int NT, int MAXL, int TWM, uint32_t PLAN>
 struct FStages {
   static FHD_DEV void run(float2* buf, int lines, int LS, const float2* tw, float s) {
-    constexpr int R = plan_radix(N, ST);
+    constexpr int R = NS >= N ? 1 : plan_at(PLAN, N, ST);
     if constexpr (R > 1) {
       fstage<N, R, NS, ES, NT, MAXL, TWM>(buf, lines, LS, tw, s);
-      FStages<N, ST + 1, NS * R, ES, NT, MAXL, TWM>::run(buf, lines, LS, tw, s);
+      FStages<N, ST + 1, NS * R, ES, NT, MAXL, TWM, PLAN>::run(buf, lines, LS, tw, s);
     }
   }
 };
 
-template <int N, int ES, int NT, int MAXL, int TWM>
+template <int N, int ES, int NT, int MAXL, int TWM, uint32_t PLAN = 0u>
 FHD_DEV void ffft(float2* buf, int lines, int LS, const float2* tw, float s) {
-  FStages<N, 0, 1, ES, NT, MAXL, TWM>::run(buf, lines, LS, tw, s);
+  FStages<N, 0, 1, ES, NT, MAXL, TWM, PLAN>::run(buf, lines, LS, tw, s);
 }
 
 // ---------------- pass A: real plane -> half spectrum (x) -> FFT along y ----------------
-template <int MX, int MY, int NT>
+template <int MX, int MY, int NT, int PX = 0, int PY = 0>
 FHD_BODY void body_fftA(const FftArgs& a, const float* __restrict__ in, float2* __restrict__ spec, const Blk& b) {
   constexpr int H = MX / 2, P = H + 1;
   __shared__ float2 buf[MY * P];
@@ -89,7 +89,7 @@ FHD_BODY void body_fftA(const FftArgs& a, const float* __restrict__ in, float2* 
     buf[y * P + 2 * x4 + 1] = make_float2(v.z, v.w);
   }
   __syncthreads();
-  ffft<H, 1, NT, MY, 2>(buf, MY, P, twx, -1.0f);
+  ffft<H, 1, NT, MY, 2, plan_choice(H, PX)>(buf, MY, P, twx, -1.0f);
   constexpr int NP = H / 2 + 1;
   for (int e = threadIdx.x; e < MY * NP; e += NT) {
     const int y = e / NP, k = e - y * NP;
@@ -109,17 +109,18 @@ FHD_BODY void body_fftA(const FftArgs& a, const float* __restrict__ in, float2* 
     }
   }
   __syncthreads();
-  ffft<MY, P, NT, P, 1>(buf, P, 1, twy, -1.0f);
+  ffft<MY, P, NT, P, 1, plan_choice(MY, PY)>(buf, P, 1, twy, -1.0f);
   float2* dst = spec + (size_t)c * MY * mz * P + (size_t)z * P;
   for (int e = threadIdx.x; e < MY * P; e += NT) {
     const int ky = e / P, kx = e - ky * P;
     dst[(size_t)ky * mz * P + kx] = buf[e];
   }
 }
-FHD_KERNEL_PAIR((template <int MX, int MY, int NT>), NT, k_fftA, FftPass, body_fftA<MX, MY, NT>(a.f, a.in, a.spec, b))
+FHD_KERNEL_PAIR((template <int MX, int MY, int NT, int PX, int PY>), NT, k_fftA, FftPass,
+                body_fftA<MX, MY, NT, PX, PY>(a.f, a.in, a.spec, b))
 
 // ---------------- pass C: FFT along y (inverse) -> C2R along x -> out = alpha r + beta aux ----
-template <int MX, int MY, int NT>
+template <int MX, int MY, int NT, int PX = 0, int PY = 0>
 FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float* __restrict__ out,
                         const float* __restrict__ aux, float alpha, float beta, const Blk& b) {
   constexpr int H = MX / 2, P = H + 1;
@@ -135,7 +136,7 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
     buf[e] = src[(size_t)ky * mz * P + kx];
   }
   __syncthreads();
-  ffft<MY, P, NT, P, 1>(buf, P, 1, twy, 1.0f);
+  ffft<MY, P, NT, P, 1, plan_choice(MY, PY)>(buf, P, 1, twy, 1.0f);
   constexpr int NP = H / 2 + 1;
   for (int e = threadIdx.x; e < MY * NP; e += NT) {
     const int y = e / NP, k = e - y * NP;
@@ -152,7 +153,7 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
     row[k] = Zk;
   }
   __syncthreads();
-  ffft<H, 1, NT, MY, 2>(buf, MY, P, twx, 1.0f);
+  ffft<H, 1, NT, MY, 2, plan_choice(H, PX)>(buf, MY, P, twx, 1.0f);
   const size_t off = ((size_t)c * mz + z) * (size_t)(MX * MY);
   float4* dst = reinterpret_cast<float4*>(out + off);
   const float4* ax = aux ? reinterpret_cast<const float4*>(aux + off) : nullptr;
@@ -171,13 +172,13 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
     dst[e] = r;
   }
 }
-FHD_KERNEL_PAIR((template <int MX, int MY, int NT>), NT, k_fftC, FftPass,
-                body_fftC<MX, MY, NT>(a.f, a.spec, a.out, a.aux, a.alpha, a.beta, b))
+FHD_KERNEL_PAIR((template <int MX, int MY, int NT, int PX, int PY>), NT, k_fftC, FftPass,
+                body_fftC<MX, MY, NT, PX, PY>(a.f, a.spec, a.out, a.aux, a.alpha, a.beta, b))
 
 // ---------------- pass B: FFT along z, operator, inverse FFT along z ----------------
 // Workgroup (tile i, ky): kx columns i*T .. i*T+T-1, plus column ntile*T + i when P is not a
 // multiple of T (the Nyquist column for P = mx/2 + 1).  LDS lines: (c, t) at (c*(T+1)+t)*(MZ+1).
-template <int MZ, int NT, int TM>
+template <int MZ, int NT, int TM, int PZ = 0>
 FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, const Blk& b) {
   // TM: room for T <= TM - 1 columns plus one extra column
   constexpr int LS = MZ + 1;
@@ -205,7 +206,7 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
   }
   __syncthreads();
   const int lines = a.ncomp * TM;
-  if (a.zmode != 2) ffft<MZ, 1, NT, 3 * TM, 1>(buf, lines, LS, twz, -1.0f);
+  if (a.zmode != 2) ffft<MZ, 1, NT, 3 * TM, 1, plan_choice(MZ, PZ)>(buf, lines, LS, twz, -1.0f);
   if (a.zmode == 0) {
     const float gy = a.gy[ky];
     const float2 ey = a.ey[ky];
@@ -252,20 +253,42 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
     }
     __syncthreads();
   }
-  if (a.zmode != 1) ffft<MZ, 1, NT, 3 * TM, 1>(buf, lines, LS, twz, 1.0f);
+  if (a.zmode != 1) ffft<MZ, 1, NT, 3 * TM, 1, plan_choice(MZ, PZ)>(buf, lines, LS, twz, 1.0f);
   for (int e = threadIdx.x; e < tot; e += NT) {
     const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
     const int kx = t < T ? tile * T + t : ntile * T + tile;
     base[c * cstride + (size_t)z * P + kx] = buf[(c * TM + t) * LS + z];
   }
 }
-FHD_KERNEL_PAIR((template <int MZ, int NT, int TM>), NT, k_fftB, FftPass, body_fftB<MZ, NT, TM>(a.f, a.spec, a.T, b))
+FHD_KERNEL_PAIR((template <int MZ, int NT, int TM, int PZ>), NT, k_fftB, FftPass,
+                body_fftB<MZ, NT, TM, PZ>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------
+template <int MX, int MY, int NT, int PX, int PY>
+static void launch_AC_plan(const FftPass& x, int which, int mz, hipStream_t s) {
+  if (which == 0)
+    launch(k_fftA<MX, MY, NT, PX, PY>, k_fftA_b<MX, MY, NT, PX, PY>, FAM_FFT_XY_FWD, dim3(mz, x.f.ncomp), NT, x, s);
+  else
+    launch(k_fftC<MX, MY, NT, PX, PY>, k_fftC_b<MX, MY, NT, PX, PY>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
+}
+// The radix plans of the x and y transforms (plan_choice indices, px + 3 py) come from the
+// schedule autotuner for 128 x 128 planes; other planes use the default plan.
 template <int MX, int MY, int NT>
 static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
-  if (which == 0) launch(k_fftA<MX, MY, NT>, k_fftA_b<MX, MY, NT>, FAM_FFT_XY_FWD, dim3(mz, x.f.ncomp), NT, x, s);
-  else launch(k_fftC<MX, MY, NT>, k_fftC_b<MX, MY, NT>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
+  if constexpr (MX == 128 && MY == 128) {
+    switch (x.f.plan_xy) {
+      case 1: launch_AC_plan<MX, MY, NT, 1, 0>(x, which, mz, s); return;
+      case 2: launch_AC_plan<MX, MY, NT, 2, 0>(x, which, mz, s); return;
+      case 3: launch_AC_plan<MX, MY, NT, 0, 1>(x, which, mz, s); return;
+      case 4: launch_AC_plan<MX, MY, NT, 1, 1>(x, which, mz, s); return;
+      case 5: launch_AC_plan<MX, MY, NT, 2, 1>(x, which, mz, s); return;
+      case 6: launch_AC_plan<MX, MY, NT, 0, 2>(x, which, mz, s); return;
+      case 7: launch_AC_plan<MX, MY, NT, 1, 2>(x, which, mz, s); return;
+      case 8: launch_AC_plan<MX, MY, NT, 2, 2>(x, which, mz, s); return;
+      default: break;
+    }
+  }
+  launch_AC_plan<MX, MY, NT, 0, 0>(x, which, mz, s);
 }
 // Lanes per plane workgroup.  A single context's pass has only 3 mz plane workgroups, and more
 // lanes per plane cover more of the plane load and store latency: 1024 from 64^2 up (128^3:
@@ -333,12 +356,21 @@ static int column_tile(int P, int tmax) {
   return 1;
 }
 
+template <int MZ, int NT, int PZ>
+static void launch_B_plan(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
+  if (bt == 2) launch(k_fftB<MZ, NT, 3, PZ>, k_fftB_b<MZ, NT, 3, PZ>, FAM_FFT_Z, grid, NT, x, s);
+  else if (bt == 4) launch(k_fftB<MZ, NT, 5, PZ>, k_fftB_b<MZ, NT, 5, PZ>, FAM_FFT_Z, grid, NT, x, s);
+  else if (bt == 8) launch(k_fftB<MZ, NT, 9, PZ>, k_fftB_b<MZ, NT, 9, PZ>, FAM_FFT_Z, grid, NT, x, s);
+  else launch(k_fftB<MZ, NT, 17, PZ>, k_fftB_b<MZ, NT, 17, PZ>, FAM_FFT_Z, grid, NT, x, s);
+}
+// The z transform's radix plan (plan_choice index) comes from the autotuner for 128-point columns.
 template <int MZ, int NT>
 static void launch_B_nt(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
-  if (bt == 2) launch(k_fftB<MZ, NT, 3>, k_fftB_b<MZ, NT, 3>, FAM_FFT_Z, grid, NT, x, s);
-  else if (bt == 4) launch(k_fftB<MZ, NT, 5>, k_fftB_b<MZ, NT, 5>, FAM_FFT_Z, grid, NT, x, s);
-  else if (bt == 8) launch(k_fftB<MZ, NT, 9>, k_fftB_b<MZ, NT, 9>, FAM_FFT_Z, grid, NT, x, s);
-  else launch(k_fftB<MZ, NT, 17>, k_fftB_b<MZ, NT, 17>, FAM_FFT_Z, grid, NT, x, s);
+  if constexpr (MZ == 128) {
+    if (x.f.plan_zs == 1) return launch_B_plan<MZ, NT, 1>(x, bt, grid, s);
+    if (x.f.plan_zs == 2) return launch_B_plan<MZ, NT, 2>(x, bt, grid, s);
+  }
+  launch_B_plan<MZ, NT, 0>(x, bt, grid, s);
 }
 
 // Column-tile width of the z pass: 2, 4, 8 or 16 (chosen per grid shape by the context's

@@ -333,6 +333,8 @@ void build_tables(fhd_ctx* c) {
   make_plan(mz, &a.plan_z, &a.nst_z);
   a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
   a.btile = 8;
+  a.plan_xy = 0;
+  a.plan_zs = 0;
   a.zmode = 0;
   a.ncomp = 3;
   a.knoise = 0;
@@ -431,7 +433,7 @@ FluidWArgs base_w(fhd_ctx* c) {
 // process: a replica's schedule depends on its shape only, never on what else runs beside it,
 // and neither choice changes any result bit.
 struct Sched {
-  int btile, zchunk, zchunk_b;
+  int btile, zchunk, zchunk_b, plan_xy, plan_zs;
 };
 std::mutex g_sched_mu;
 std::map<std::array<int, 4>, Sched> g_sched;
@@ -465,16 +467,35 @@ void apply_schedule(fhd_ctx* c) {
   const bool tune = !(env && env[0] == '0');
   std::lock_guard<std::mutex> lock(g_sched_mu);
   auto it = g_sched.find(key);
-  Sched sc = {8, 8, 8};
+  Sched sc = {8, 8, 8, 0, 0};
   if (it != g_sched.end()) {
     sc = it->second;
   } else if (tune) {
     // scratch runs on the context's own buffers before its state exists (contents irrelevant)
     float best = 1e30f;
-    for (int bt : {2, 4, 8, 16}) {  // 2: more column workgroups on small grids
-      c->fa.btile = bt;
-      const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
-      if (t < best) { best = t; sc.btile = bt; }
+    // z pass: column tile and, for 128-point columns, the radix plan (16 8, 8 16 or 4 4 8)
+    const int nzp = c->p.mz == 128 ? 3 : 1;
+    for (int pz = 0; pz < nzp; ++pz)
+      for (int bt : {2, 4, 8, 16}) {  // 2: more column workgroups on small grids
+        c->fa.btile = bt;
+        c->fa.plan_zs = pz;
+        const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
+        if (t < best) { best = t; sc.btile = bt; sc.plan_zs = pz; }
+      }
+    c->fa.plan_zs = sc.plan_zs;
+    // xy passes of 128 x 128 planes: the radix plans of the 64-point x and 128-point y
+    // transforms (three each), timed as a forward and an inverse pass
+    if (c->p.mx == 128 && c->p.my == 128) {
+      best = 1e30f;
+      for (int pxy = 0; pxy < 9; ++pxy) {
+        c->fa.plan_xy = pxy;
+        const float t = time_launches(c, [&] {
+          launch_fft_xy_fwd(c->fa, c->W, c->spec, c->s);
+          launch_fft_xy_inv(c->fa, c->spec, c->W, nullptr, 1.0f, 0.0f, c->s);
+        }, 2);
+        if (t < best) { best = t; sc.plan_xy = pxy; }
+      }
+      c->fa.plan_xy = sc.plan_xy;
     }
     // W kernel: 2 planes per workgroup fill the chip on small grids (64^3: 512 workgroups) when
     // the context launches alone; a batched dispatch already holds many contexts' workgroups,
@@ -501,8 +522,22 @@ void apply_schedule(fhd_ctx* c) {
     g_sched[key] = sc;
   }
   c->fa.btile = sc.btile;
+  c->fa.plan_xy = sc.plan_xy;
+  c->fa.plan_zs = sc.plan_zs;
   c->zchunk = sc.zchunk;
   c->zchunk_b = sc.zchunk_b;
+  // explicit radix plans (tests and experiments): FHD_PLAN_XY in 0-8 (128 x 128 planes),
+  // FHD_PLAN_Z in 0-2 (128-point columns)
+  if (const char* e = getenv("FHD_PLAN_XY")) {
+    const int v = atoi(e);
+    if (v < 0 || v > 8) throw std::runtime_error("FHD_PLAN_XY must be 0-8");
+    c->fa.plan_xy = v;
+  }
+  if (const char* e = getenv("FHD_PLAN_Z")) {
+    const int v = atoi(e);
+    if (v < 0 || v > 2) throw std::runtime_error("FHD_PLAN_Z must be 0-2");
+    c->fa.plan_zs = v;
+  }
   // explicit schedule (tests and experiments): FHD_BTILE in {2, 4, 8, 16}, FHD_ZCHUNK >= 1
   if (const char* e = getenv("FHD_BTILE")) {
     const int v = atoi(e);
@@ -2055,6 +2090,16 @@ int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_name
     c->marks.clear();
   });
   return nout;
+}
+
+int fhd_get_schedule(const fhd_ctx* c, int out[5]) {
+  if (!c || !out) return 0;
+  out[0] = c->fa.btile;
+  out[1] = c->fa.plan_xy;
+  out[2] = c->fa.plan_zs;
+  out[3] = c->zchunk;
+  out[4] = c->zchunk_b;
+  return 1;
 }
 
 int fhd_max_divergence(fhd_ctx* c, double* maxdiv) {

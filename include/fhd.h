@@ -162,6 +162,12 @@ int fhd_get_np(const fhd_ctx* c);
 int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms);
 int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern);
 
+/* The schedule the context's autotuner chose for its grid shape (no fluam counterpart): out[0] the
+ * z-pass column tile, out[1] the xy passes' radix plan (0-8 for 128 x 128 planes: x plan + 3 y
+ * plan, plans 16 4 | 4 16 | 8 8 and 16 8 | 8 16 | 4 4 8), out[2] the z pass's radix plan (0-2 for
+ * 128-point columns), out[3] and out[4] the W kernel's z chunk alone and in batched dispatches. */
+int fhd_get_schedule(const fhd_ctx* c, int out[5]);
+
 /* Divergence diagnostics of the current velocity: max |D v| over cells, in units of
  * 1/time (discrete staggered divergence, src/kernelUpdateVIncompressible.cu:21-72 invariant). */
 int fhd_max_divergence(fhd_ctx* c, double* maxdiv);

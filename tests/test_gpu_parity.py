@@ -51,6 +51,25 @@ def test_fft_r2c_matches_numpy(shape):
     assert np.abs(back - a * a.size).max() < 1e-5 * np.abs(a * a.size).max()
 
 
+@pytest.mark.parametrize("plan_xy,plan_z", [(k, k % 3) for k in range(9)])
+def test_fft_every_radix_plan_matches_numpy(plan_xy, plan_z):
+    """The radix plans the schedule autotuner chooses among for 128^3 (x: 16 4, 4 16, 8 8;
+    y and z: 16 8, 8 16, 4 4 8), each forced: the transforms match numpy to FP32 accuracy."""
+    shape = (128, 128, 128)
+    rng = np.random.default_rng(100 + plan_xy)
+    a = rng.standard_normal(shape)
+    os.environ["FHD_PLAN_XY"] = str(plan_xy)
+    os.environ["FHD_PLAN_Z"] = str(plan_z)
+    try:
+        got = F.fft_r2c(a)
+        ref = np.fft.rfftn(a)
+        back = F.fft_c2r(ref, shape[2])
+    finally:
+        del os.environ["FHD_PLAN_XY"], os.environ["FHD_PLAN_Z"]
+    assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+    assert np.abs(back - a * a.size).max() < 1e-5 * np.abs(a * a.size).max()
+
+
 # ---------------- incompressible ----------------
 def test_incompressible_config1_parity_100_steps():
     """32^3, dx=1, rho=eta=1, kT=1e-3, dt=0.05, v0=0, 100 steps: |v_gpu - v_oracle| / |v| < 1e-5."""
