@@ -257,7 +257,7 @@ def bench_served(args):
         return p, int(rng.integers(50, 501))
 
     jobs = serve.poisson_jobs(args.rate, args.jobs, 5, make)
-    res = serve.Server().run(jobs)
+    res = serve.Server(merged=args.serve_mode == "merged").run(jobs)
     plain_s = plain_back_to_back(F, [(j.params, j.steps) for j in jobs])
     total = sum(j.cell_updates for j in jobs)
     plain = total / plain_s
@@ -268,7 +268,9 @@ def bench_served(args):
         "data": "synthetic: seeded job mix and Poisson arrivals",
         "config": {"workload": f"{args.jobs} jobs, Poisson arrivals at {args.rate}/s, cubic edges from "
                                "{32,64,96,128}, Stokes (4-point) or QNB (3-point), 50-500 steps",
-                   "jobs": args.jobs, "rate_per_s": args.rate},
+                   "jobs": args.jobs, "rate_per_s": args.rate,
+                   "serve_mode": ("running jobs regrouped into one merged-stage batch at each join and leave"
+                                  if args.serve_mode == "merged" else "a stream and graph replay per job")},
         "p50_latency_s": res["p50_latency_s"], "p99_latency_s": res["p99_latency_s"],
         "mean_latency_s": res["mean_latency_s"], "wall_s": res["wall_s"],
         "back_to_back_plain_cell_updates_per_s": plain,
@@ -292,6 +294,8 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=8.0, help="served: job arrivals per second")
+    ap.add_argument("--serve-mode", choices=("streams", "merged"), default="merged",
+                    help="served: a stream per job, or the running jobs as one merged-stage batch")
     ap.add_argument("--batch-mode", type=int, default=2, choices=(0, 1, 2),
                     help="batched: 0 = per-replica graphs, 1 = one captured graph per batch step, "
                          "2 = merged stages (one batched dispatch per stage and kernel instance)")

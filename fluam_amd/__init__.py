@@ -16,7 +16,7 @@ from ._abi import (FhdParams, make_params, SCHEME_INCOMPRESSIBLE, SCHEME_QUASI_N
 __all__ = ["FhdParams", "make_params", "Simulation", "lib", "load_datamain", "get_params", "main",
            "scheme_incompressible", "scheme_quasi_neutrally_buoyant", "scheme_stokes_limit",
            "SCHEME_INCOMPRESSIBLE", "SCHEME_QUASI_NEUTRALLY_BUOYANT", "SCHEME_STOKES_LIMIT",
-           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "bytes_needed", "mem_info",
+           "RFD_DISPLACED", "RFD_REFERENCE", "LIB_PATH", "Batch", "Group", "bytes_needed", "mem_info",
            "read_fluid_file", "write_fluid_file"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -76,6 +76,8 @@ def lib():
     L.fhd_batch_create.argtypes = [C.POINTER(FhdParams), C.c_int]
     L.fhd_batch_create.restype = vp
     L.fhd_batch_destroy.argtypes = [vp]
+    L.fhd_batch_adopt.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    L.fhd_batch_adopt.restype = vp
     L.fhd_batch_destroy.restype = None
     L.fhd_batch_size.argtypes = [vp]
     L.fhd_batch_replica.argtypes = [vp, C.c_int]
@@ -236,8 +238,8 @@ class Simulation:
         return s, n.value
 
     def set_noise_overlap(self, on=True):
-        """Stokes limit: build the noise on a second stream while the particle kernels run (the
-        default) or on the context's own stream (identical results)."""
+        """Accepted for callers of earlier versions; no effect (the Stokes-limit noise is drawn
+        in Fourier space inside the solve)."""
         self._check(lib().fhd_set_noise_overlap(self.h, 1 if on else 0))
 
     def sync(self):
@@ -386,6 +388,24 @@ class Batch:
         if getattr(self, "h", None):
             for r in self.replicas:
                 r.h = None
+            lib().fhd_batch_destroy(self.h)
+            self.h = None
+
+
+class Group(Batch):
+    """A batch over existing Simulations (fhd_batch_adopt), for a serving loop that regroups its
+    running jobs: closing the group leaves the Simulations alive."""
+
+    def __init__(self, sims):
+        n = len(sims)
+        arr = (C.c_void_p * n)(*[s.h for s in sims])
+        self.h = lib().fhd_batch_adopt(arr, n)
+        if not self.h:
+            raise FhdError(_err())
+        self.replicas = list(sims)
+
+    def close(self):
+        if getattr(self, "h", None):
             lib().fhd_batch_destroy(self.h)
             self.h = None
 

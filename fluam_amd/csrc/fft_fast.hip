@@ -271,8 +271,9 @@ static void launch_AC_plan(const FftPass& x, int which, int mz, hipStream_t s) {
   else
     launch(k_fftC<MX, MY, NT, PX, PY>, k_fftC_b<MX, MY, NT, PX, PY>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
 }
-// The radix plans of the x and y transforms (plan_choice indices, px + 3 py) come from the
-// schedule autotuner for 128 x 128 planes; other planes use the default plan.
+// The radix plans of the x and y transforms (plan_choice indices, px + 3 py), or 9 for the
+// register-transform kernels of fft128.hip, come from the schedule autotuner for 128 x 128
+// planes; other planes use the default plan.
 template <int MX, int MY, int NT>
 static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
   if constexpr (MX == 128 && MY == 128) {
@@ -285,6 +286,7 @@ static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
       case 6: launch_AC_plan<MX, MY, NT, 0, 2>(x, which, mz, s); return;
       case 7: launch_AC_plan<MX, MY, NT, 1, 2>(x, which, mz, s); return;
       case 8: launch_AC_plan<MX, MY, NT, 2, 2>(x, which, mz, s); return;
+      case 9: launch_fft128_xy(x, which, mz, s); return;  // transforms in registers (fft128.hip)
       default: break;
     }
   }

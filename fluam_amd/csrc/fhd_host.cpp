@@ -484,10 +484,11 @@ void apply_schedule(fhd_ctx* c) {
       }
     c->fa.plan_zs = sc.plan_zs;
     // xy passes of 128 x 128 planes: the radix plans of the 64-point x and 128-point y
-    // transforms (three each), timed as a forward and an inverse pass
+    // transforms (three each) of the LDS-stage kernels, or the register-transform kernels
+    // (plan 9), timed as a forward and an inverse pass
     if (c->p.mx == 128 && c->p.my == 128) {
       best = 1e30f;
-      for (int pxy = 0; pxy < 9; ++pxy) {
+      for (int pxy = 0; pxy < 10; ++pxy) {  // 9: the register-transform kernels
         c->fa.plan_xy = pxy;
         const float t = time_launches(c, [&] {
           launch_fft_xy_fwd(c->fa, c->W, c->spec, c->s);
@@ -526,11 +527,11 @@ void apply_schedule(fhd_ctx* c) {
   c->fa.plan_zs = sc.plan_zs;
   c->zchunk = sc.zchunk;
   c->zchunk_b = sc.zchunk_b;
-  // explicit radix plans (tests and experiments): FHD_PLAN_XY in 0-8 (128 x 128 planes),
+  // explicit radix plans (tests and experiments): FHD_PLAN_XY in 0-9 (128 x 128 planes),
   // FHD_PLAN_Z in 0-2 (128-point columns)
   if (const char* e = getenv("FHD_PLAN_XY")) {
     const int v = atoi(e);
-    if (v < 0 || v > 8) throw std::runtime_error("FHD_PLAN_XY must be 0-8");
+    if (v < 0 || v > 9) throw std::runtime_error("FHD_PLAN_XY must be 0-9");
     c->fa.plan_xy = v;
   }
   if (const char* e = getenv("FHD_PLAN_Z")) {
@@ -2286,6 +2287,7 @@ void fhd_internal_set_error(const char* e) { g_last_error = e ? e : ""; }
 constexpr int kMaxBranches = 8;
 struct fhd_batch {
   std::vector<fhd_ctx*> reps;  // each replica owns its memory and stream
+  bool owns = true;            // false for a batch over contexts the caller keeps (fhd_batch_adopt)
   hipStream_t timer = nullptr;
   std::string err;
   // whole-batch graphs: one graph per phase holding every replica's step as an independent
@@ -2411,6 +2413,28 @@ fhd_batch* fhd_batch_create(const fhd_params* p, int n) {
   return b;
 }
 
+fhd_batch* fhd_batch_adopt(fhd_ctx* const* ctxs, int n) {
+  fhd_batch* b = new fhd_batch();
+  b->owns = false;
+  try {
+    if (n <= 0 || !ctxs) throw std::runtime_error("a batch needs at least one context");
+    for (int i = 0; i < n; ++i) {
+      fhd_ctx* c = ctxs[i];
+      if (!c) throw std::runtime_error("null context " + std::to_string(i));
+      for (fhd_ctx* o : b->reps)
+        if (o == c) throw std::runtime_error("a context appears twice in the batch");
+      c->graph_mode = true;
+      b->reps.push_back(c);
+    }
+    HIPCHK(hipStreamCreate(&b->timer));
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    delete b;
+    return nullptr;
+  }
+  return b;
+}
+
 void fhd_batch_destroy(fhd_batch* b) {
   if (!b) return;
   if (b->timer) (void)hipStreamSynchronize(b->timer);
@@ -2428,7 +2452,10 @@ void fhd_batch_destroy(fhd_batch* b) {
   for (auto& e : b->join)
     if (e) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
-  for (fhd_ctx* c : b->reps) fhd_destroy(c);
+  // (adopted contexts stay with the caller; their streams were ordered after the batch's steps
+  // by fhd_batch_step, and the timer stream has drained above)
+  if (b->owns)
+    for (fhd_ctx* c : b->reps) fhd_destroy(c);
   if (b->timer) (void)hipStreamDestroy(b->timer);
   delete b;
 }

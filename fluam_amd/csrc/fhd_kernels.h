@@ -263,6 +263,8 @@ bool fast_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const flo
 
 // Launch helpers (defined in the .hip files).
 void launch_fft_xy_fwd(const FftArgs& a, const float* in, float2* spec, hipStream_t s);
+// The 128 x 128 xy passes with the transforms in registers (which 0: forward, 1: inverse).
+void launch_fft128_xy(const FftPass& x, int which, int mz, hipStream_t s);
 void launch_fft_z_solve(const FftArgs& a, float2* spec, hipStream_t s);
 void launch_fft_xy_inv(const FftArgs& a, const float2* spec, float* out, const float* aux, float alpha,
                        float beta, hipStream_t s);

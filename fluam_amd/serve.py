@@ -16,8 +16,8 @@ from typing import List, Optional
 
 import numpy as np
 
-from . import FhdError, Simulation, bytes_needed, mem_info
-from ._abi import FhdParams
+from . import FhdError, Group, Simulation, bytes_needed, mem_info
+from ._abi import FhdParams, SCHEME_QUASI_NEUTRALLY_BUOYANT
 
 
 @dataclass
@@ -53,7 +53,7 @@ class Server:
     particles back before its memory is released."""
 
     def __init__(self, budget: Optional[int] = None, max_inflight: int = 4, keep_results: bool = False,
-                 graphs: bool = True):
+                 graphs: bool = True, merged: bool = False, chunk: int = 8):
         if budget is None:
             free, _ = mem_info()
             budget = int(0.8 * free)
@@ -61,8 +61,34 @@ class Server:
         self.max_inflight = max_inflight
         self.keep_results = keep_results
         self.graphs = graphs
+        self.merged = merged
+        self.chunk = chunk
+
+    def _admit(self, j: Job, t0: float):
+        j._sim = Simulation(j.params)
+        j._sim.set_graph(self.graphs)
+        j.admitted = time.perf_counter() - t0
+
+    def _retire(self, j: Job, t0: float):
+        j.finished = time.perf_counter() - t0
+        if self.keep_results and not j.error:
+            j.velocity = j._sim.velocity()
+            if j.params.np > 0:
+                j.particles = j._sim.particles()
+        j._sim.close()
+        j._sim = None
+
+    def _report(self, done: List[Job], wall: float) -> dict:
+        ok = [j for j in done if not j.error]
+        lat = np.array([j.latency for j in ok]) if ok else np.array([0.0])
+        total = sum(j.cell_updates for j in ok)
+        return {"jobs": len(ok), "evicted": len(done) - len(ok), "wall_s": wall, "cell_updates_per_s": total / wall,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                "mean_latency_s": float(lat.mean())}
 
     def run(self, jobs: List[Job]) -> dict:
+        if self.merged:
+            return self._run_merged(jobs)
         pending = sorted(jobs, key=lambda j: j.arrival)
         for j in pending:
             j._bytes = bytes_needed(j.params)
@@ -77,12 +103,7 @@ class Server:
             # admission: arrived jobs, in arrival order, while memory allows
             while pending and pending[0].arrival <= now and used + pending[0]._bytes <= self.budget:
                 j = pending.pop(0)
-                j._sim = Simulation(j.params)
-                j._sim.set_graph(self.graphs)
-                # the admitted jobs already fill the GPU: a second stream per job only adds
-                # cross-stream waits
-                j._sim.set_noise_overlap(False)
-                j.admitted = time.perf_counter() - t0
+                self._admit(j, t0)
                 used += j._bytes
                 active.append(j)
             # one round: every admitted job advances one step.  Run-ahead is bounded: once a
@@ -122,13 +143,7 @@ class Server:
                         j.error = str(e)
                         finished = True
                 if finished:
-                    j.finished = time.perf_counter() - t0
-                    if self.keep_results and not j.error:
-                        j.velocity = j._sim.velocity()
-                        if j.params.np > 0:
-                            j.particles = j._sim.particles()
-                    j._sim.close()
-                    j._sim = None
+                    self._retire(j, t0)
                     used -= j._bytes
                     done.append(j)
                     progressed = True
@@ -139,13 +154,82 @@ class Server:
                 time.sleep(0.0002)
             if not active and pending:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
-        wall = time.perf_counter() - t0
-        ok = [j for j in done if not j.error]
-        lat = np.array([j.latency for j in ok]) if ok else np.array([0.0])
-        total = sum(j.cell_updates for j in ok)
-        return {"jobs": len(ok), "evicted": len(done) - len(ok), "wall_s": wall, "cell_updates_per_s": total / wall,
-                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-                "mean_latency_s": float(lat.mean())}
+        return self._report(done, time.perf_counter() - t0)
+
+    def _run_merged(self, jobs: List[Job]) -> dict:
+        """The running jobs as one merged-stage batch (fhd_batch_adopt, mode 2: one dispatch per
+        stage and kernel instance for all of them), regrouped whenever a job joins or leaves.
+        The group advances in chunks of up to `chunk` steps, cut at the next job's end; a job
+        leaves once its last chunk has completed.  Inertial-coupling jobs take their two
+        bootstrap steps alone before they join (a merged program needs steady-state steps)."""
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        for j in pending:
+            j._bytes = bytes_needed(j.params)
+            if j._bytes > self.budget:
+                raise ValueError(f"job {j.name} needs {j._bytes} bytes, more than the budget {self.budget}")
+        active: List[Job] = []
+        done: List[Job] = []
+        used = 0
+        group = None
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            changed = False
+            while pending and pending[0].arrival <= now and used + pending[0]._bytes <= self.budget:
+                j = pending.pop(0)
+                self._admit(j, t0)
+                if j.params.scheme == SCHEME_QUASI_NEUTRALLY_BUOYANT and j.params.np > 0:
+                    k = min(2, j.steps)
+                    j._sim.step(k)
+                    j._issued = k
+                used += j._bytes
+                active.append(j)
+                changed = True
+            running = [j for j in active if j._issued < j.steps]
+            members = [j._sim for j in running]
+            if group is not None and (changed or group.replicas != members):
+                group.close()
+                group = None
+            if group is None and running:
+                group = Group(members)
+                group.set_mode(2)
+            if running:
+                k = min(self.chunk, min(j.steps - j._issued for j in running))
+                try:
+                    group.step(k)
+                    group.sync()
+                except FhdError:
+                    # a device error flagged by some job's steps: evict the flagged ones
+                    for j in running:
+                        try:
+                            j._sim.sync()
+                        except FhdError as e:
+                            j.error = str(e)
+                for j in running:
+                    j._issued += k
+            # retire finished and evicted jobs (their work has completed: the group was synced)
+            still = []
+            for j in active:
+                if j.error is not None or j._issued >= j.steps:
+                    if j.error is None:
+                        try:
+                            j._sim.sync()
+                        except FhdError as e:
+                            j.error = str(e)
+                    if group is not None and j._sim in group.replicas:
+                        group.close()
+                        group = None
+                    self._retire(j, t0)
+                    used -= j._bytes
+                    done.append(j)
+                else:
+                    still.append(j)
+            active = still
+            if not active and pending:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+        if group is not None:
+            group.close()
+        return self._report(done, time.perf_counter() - t0)
 
 
 def poisson_jobs(rate: float, n: int, seed: int, make) -> List[Job]:

@@ -164,7 +164,8 @@ int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_name
 
 /* The schedule the context's autotuner chose for its grid shape (no fluam counterpart): out[0] the
  * z-pass column tile, out[1] the xy passes' radix plan (0-8 for 128 x 128 planes: x plan + 3 y
- * plan, plans 16 4 | 4 16 | 8 8 and 16 8 | 8 16 | 4 4 8), out[2] the z pass's radix plan (0-2 for
+ * plan, plans 16 4 | 4 16 | 8 8 and 16 8 | 8 16 | 4 4 8; 9: the transforms in registers with one
+ * exchange each), out[2] the z pass's radix plan (0-2 for
  * 128-point columns), out[3] and out[4] the W kernel's z chunk alone and in batched dispatches. */
 int fhd_get_schedule(const fhd_ctx* c, int out[5]);
 
@@ -189,6 +190,10 @@ int fhd_query(fhd_ctx* c);
  * Results equal those of each replica run alone. */
 typedef struct fhd_batch fhd_batch;
 fhd_batch* fhd_batch_create(const fhd_params* p, int n);   /* NULL on error (fhd_last_error) */
+/* A batch over contexts the caller keeps (for a serving loop that regroups its running jobs as
+ * they join and leave): fhd_batch_destroy leaves them alive, and each can be stepped alone or
+ * adopted by another batch afterwards.  A context belongs to at most one batch at a time. */
+fhd_batch* fhd_batch_adopt(fhd_ctx* const* ctxs, int n);
 void fhd_batch_destroy(fhd_batch* b);
 int fhd_batch_size(const fhd_batch* b);
 fhd_ctx* fhd_batch_replica(fhd_batch* b, int i);           /* owned by the batch */

@@ -145,9 +145,11 @@ def test_config4_ensemble_256_replicas():
     b.close()
 
 
-def test_config5_served_jobs_at_full_size():
+@pytest.mark.parametrize("merged", [False, True])
+def test_config5_served_jobs_at_full_size(merged):
     """Configuration 5's sizes: jobs on 128^3 (65,536 particles, Stokes 4-point), 96^3 (QNB 3-point)
-    and 64^3 grids served together give each the bits of its solo run."""
+    and 64^3 grids served together (per-job streams, or regrouped merged-stage batches) give each
+    the bits of its solo run."""
     def params(i, edge, npart, scheme):
         return make_params(scheme, cells=(edge, edge, edge), np=npart, kernel_points=4 if scheme == SCHEME_STOKES_LIMIT else 3,
                            temperature=1e-3, dt=0.05, seed=3000 + i, initial_positions=1, **WCA)
@@ -156,7 +158,7 @@ def test_config5_served_jobs_at_full_size():
     mix = [S.Job(f"j{i}", params(i, e, n, sc), steps, arrival=0.002 * i) for i, (e, n, sc, steps) in enumerate(specs)]
     for j in mix:
         _announce("config5_job", j.params)
-    res = S.Server(keep_results=True).run(mix)
+    res = S.Server(keep_results=True, merged=merged).run(mix)
     assert res["jobs"] == len(specs) and res["evicted"] == 0
     for i, (e, n, sc, steps) in enumerate(specs):
         alone = S.Job(f"a{i}", params(i, e, n, sc), steps)

@@ -126,6 +126,81 @@ def test_device_error_evicts_only_that_job(fake, monkeypatch):
     assert Failing.live == 0
 
 
+class FakeGroup:
+    """Stand-in merged-stage batch over FakeSims: a group step advances every member."""
+    built = 0
+
+    def __init__(self, sims):
+        self.replicas = list(sims)
+        self.open = True
+        FakeGroup.built += 1
+
+    def set_mode(self, m):
+        assert m == 2
+
+    def step(self, n=1):
+        assert self.open
+        for s in self.replicas:
+            s.step(n)
+
+    def sync(self):
+        err = None
+        for s in self.replicas:
+            try:
+                s.sync()
+            except F.FhdError as e:
+                err = e
+        if err:
+            raise err
+
+    def close(self):
+        self.open = False
+
+
+@pytest.fixture
+def fake_group(fake, monkeypatch):
+    FakeGroup.built = 0
+    monkeypatch.setattr(S, "Group", FakeGroup)
+    return FakeGroup
+
+
+def test_merged_jobs_join_and_leave(fake_group):
+    """Jobs of different lengths arriving at different times each take exactly their own step
+    count; the group is rebuilt as they join and leave (also when nothing new arrives)."""
+    jobs = [_job("a", 16, 50, 0.0), _job("b", 16, 9, 0.0), _job("c", 20, 23, 0.005), _job("d", 16, 4, 0.2)]
+    r = S.Server(budget=10 ** 9, keep_results=True, merged=True, chunk=8).run(jobs)
+    assert r["jobs"] == 4 and r["evicted"] == 0
+    for j in jobs:
+        assert j.velocity[0] == j.steps, j.name
+        assert j.finished is not None and j.finished >= j.admitted
+    assert fake_group.built >= 4
+    assert FakeSim.live == 0
+
+
+def test_merged_qnb_jobs_bootstrap_alone(fake_group):
+    p = make_params(SCHEME_QUASI_NEUTRALLY_BUOYANT, cells=(16, 16, 16), np=10)
+    jobs = [S.Job("q", p, 11, 0.0), _job("s", 16, 6, 0.0), S.Job("q1", p, 1, 0.0)]
+    S.Server(budget=10 ** 9, keep_results=True, merged=True).run(jobs)
+    assert [j.velocity[0] for j in jobs] == [11, 6, 1]
+
+
+def test_merged_device_error_evicts_only_that_job(fake_group, monkeypatch):
+    class Failing(FakeSim):
+        def sync(self):
+            super().sync()
+            if self.p.mx == 24 and self.steps >= 8:
+                raise F.FhdError("device error: non-finite particle displacement")
+
+    monkeypatch.setattr(S, "Simulation", Failing)
+    jobs = [_job("a", 16, 40, 0.0), _job("bad", 24, 40, 0.0), _job("c", 16, 30, 0.001)]
+    r = S.Server(budget=10 ** 9, keep_results=True, merged=True, chunk=4).run(jobs)
+    a, bad, c = jobs
+    assert bad.error and "non-finite" in bad.error and bad.velocity is None
+    assert a.velocity[0] == 40 and c.velocity[0] == 30
+    assert r["jobs"] == 2 and r["evicted"] == 1
+    assert Failing.live == 0
+
+
 def test_poisson_arrivals_are_increasing():
     jobs = S.poisson_jobs(50.0, 20, 3, lambda i, rng: (make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16)), 5))
     t = [j.arrival for j in jobs]
@@ -133,15 +208,18 @@ def test_poisson_arrivals_are_increasing():
 
 
 @pytest.mark.gpu
-def test_served_results_equal_alone():
-    """Jobs of different shapes and schemes served together give the same bits as each alone."""
+@pytest.mark.parametrize("merged", [False, True])
+def test_served_results_equal_alone(merged):
+    """Jobs of different shapes and schemes served together give the same bits as each alone,
+    with per-job streams and with the running jobs regrouped into one merged-stage batch."""
     def params(i, edge, np_, scheme):
         return make_params(scheme, cells=(edge, edge, edge), np=np_, kernel_points=3, temperature=1e-3,
                            dt=0.05, seed=500 + i, initial_positions=1, cutoff=2 ** (1 / 6), wca_sigma=1.0)
     specs = [(32, 200, SCHEME_STOKES_LIMIT, 30), (16, 0, SCHEME_STOKES_LIMIT, 50),
              (24, 100, SCHEME_QUASI_NEUTRALLY_BUOYANT, 40), (32, 0, SCHEME_QUASI_NEUTRALLY_BUOYANT, 20)]
     mix = [S.Job(f"j{i}", params(i, e, n, sc), steps, arrival=0.003 * i) for i, (e, n, sc, steps) in enumerate(specs)]
-    S.Server(keep_results=True).run(mix)
+    res = S.Server(keep_results=True, merged=merged, chunk=7).run(mix)
+    assert res["jobs"] == len(specs) and res["evicted"] == 0
     for i, (e, n, sc, steps) in enumerate(specs):
         alone = S.Job(f"a{i}", params(i, e, n, sc), steps)
         S.Server(keep_results=True).run([alone])
