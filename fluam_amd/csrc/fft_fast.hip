@@ -82,11 +82,23 @@ FHD_BODY void body_fftA(const FftArgs& a, const float* __restrict__ in, float2* 
   for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
   const float4* src = reinterpret_cast<const float4*>(in + ((size_t)c * mz + z) * (size_t)(MX * MY));
   constexpr int Q = MX / 4;
-  for (int e = threadIdx.x; e < MY * Q; e += NT) {
-    const int y = e / Q, x4 = e - y * Q;
-    const float4 v = src[e];
-    buf[y * P + 2 * x4] = make_float2(v.x, v.y);
-    buf[y * P + 2 * x4 + 1] = make_float2(v.z, v.w);
+  // every load of this lane in flight at once, then into LDS (a load-store loop would wait out
+  // one HBM round trip per element)
+  constexpr int NL = (MY * Q + NT - 1) / NT;
+  float4 v[NL];
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int e = threadIdx.x + k * NT;
+    if (e < MY * Q) v[k] = src[e];
+  }
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int e = threadIdx.x + k * NT;
+    if (e < MY * Q) {
+      const int y = e / Q, x4 = e - y * Q;
+      buf[y * P + 2 * x4] = make_float2(v[k].x, v[k].y);
+      buf[y * P + 2 * x4 + 1] = make_float2(v[k].z, v[k].w);
+    }
   }
   __syncthreads();
   ffft<H, 1, NT, MY, 2, plan_choice(H, PX)>(buf, MY, P, twx, -1.0f);
@@ -131,9 +143,22 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
   for (int i = threadIdx.x; i < MX; i += NT) twx[i] = a.twx[i];
   for (int i = threadIdx.x; i < MY; i += NT) twy[i] = a.twy[i];
   const float2* src = spec + (size_t)c * MY * mz * P + (size_t)z * P;
-  for (int e = threadIdx.x; e < MY * P; e += NT) {
-    const int ky = e / P, kx = e - ky * P;
-    buf[e] = src[(size_t)ky * mz * P + kx];
+  constexpr int NL = (MY * P + NT - 1) / NT;
+  {
+    float2 v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < MY * P) {
+        const int ky = e / P, kx = e - ky * P;
+        v[k] = src[(size_t)ky * mz * P + kx];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < MY * P) buf[e] = v[k];
+    }
   }
   __syncthreads();
   ffft<MY, P, NT, P, 1, plan_choice(MY, PY)>(buf, P, 1, twy, 1.0f);
@@ -158,18 +183,30 @@ FHD_BODY void body_fftC(const FftArgs& a, const float2* __restrict__ spec, float
   float4* dst = reinterpret_cast<float4*>(out + off);
   const float4* ax = aux ? reinterpret_cast<const float4*>(aux + off) : nullptr;
   constexpr int Q = MX / 4;
-  for (int e = threadIdx.x; e < MY * Q; e += NT) {
-    const int y = e / Q, x4 = e - y * Q;
-    const float2 p0 = buf[y * P + 2 * x4], p1 = buf[y * P + 2 * x4 + 1];
-    float4 r = make_float4(alpha * p0.x, alpha * p0.y, alpha * p1.x, alpha * p1.y);
-    if (ax) {
-      const float4 q = ax[e];
-      r.x += beta * q.x;
-      r.y += beta * q.y;
-      r.z += beta * q.z;
-      r.w += beta * q.w;
+  constexpr int NS = (MY * Q + NT - 1) / NT;
+  float4 q[NS];
+  if (ax) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < MY * Q) q[k] = ax[e];
     }
-    dst[e] = r;
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int e = threadIdx.x + k * NT;
+    if (e < MY * Q) {
+      const int y = e / Q, x4 = e - y * Q;
+      const float2 p0 = buf[y * P + 2 * x4], p1 = buf[y * P + 2 * x4 + 1];
+      float4 r = make_float4(alpha * p0.x, alpha * p0.y, alpha * p1.x, alpha * p1.y);
+      if (ax) {
+        r.x += beta * q[k].x;
+        r.y += beta * q[k].y;
+        r.z += beta * q[k].z;
+        r.w += beta * q[k].w;
+      }
+      dst[e] = r;
+    }
   }
 }
 FHD_KERNEL_PAIR((template <int MX, int MY, int NT, int PX, int PY>), NT, k_fftC, FftPass,
@@ -198,11 +235,28 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
   }
   const size_t cstride = (size_t)my * MZ * P;
   float2* base = spec + (size_t)ky * MZ * P;
-  const int tot = a.ncomp * MZ * TC;
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
-    const int kx = t < T ? tile * T + t : ntile * T + tile;
-    buf[(c * TM + t) * LS + z] = base[c * cstride + (size_t)z * P + kx];
+  // elements (c, z, t) on a TM-wide grid (t fastest, constant divisors); slots t >= TC idle.
+  // Every load of this lane is in flight before the first LDS write.
+  constexpr int EL = 3 * MZ * TM, NL = (EL + NT - 1) / NT;
+  const int ncomp = a.ncomp;
+  const int cs32 = (int)cstride;  // ncomp * cstride elements per ky row: far below 2^31 at every size
+  {
+    float2 v[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int e = threadIdx.x + k * NT;
+      const int t = e % TM, r = e / TM, z = r % MZ, c = r / MZ;
+      if (e < EL && c < ncomp && t < TC) {
+        const int kx = t < T ? tile * T + t : ntile * T + tile;
+        v[k] = base[c * cs32 + z * P + kx];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k) {
+      const int e = threadIdx.x + k * NT;
+      const int t = e % TM, r = e / TM, z = r % MZ, c = r / MZ;
+      if (e < EL && c < ncomp && t < TC) buf[(c * TM + t) * LS + z] = v[k];
+    }
   }
   __syncthreads();
   const int lines = a.ncomp * TM;
@@ -254,13 +308,21 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
     __syncthreads();
   }
   if (a.zmode != 1) ffft<MZ, 1, NT, 3 * TM, 1, plan_choice(MZ, PZ)>(buf, lines, LS, twz, 1.0f);
-  for (int e = threadIdx.x; e < tot; e += NT) {
-    const int c = e / (MZ * TC), r = e - c * MZ * TC, z = r / TC, t = r - z * TC;
-    const int kx = t < T ? tile * T + t : ntile * T + tile;
-    base[c * cstride + (size_t)z * P + kx] = buf[(c * TM + t) * LS + z];
+  // the lane index, opaque to the compiler: the store indices are recomputed here rather than
+  // kept live in registers from the loads through both transforms
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+#pragma unroll
+  for (int k = 0; k < NL; ++k) {
+    const int e = tid + k * NT;
+    const int t = e % TM, r = e / TM, z = r % MZ, c = r / MZ;
+    if (e < EL && c < ncomp && t < TC) {
+      const int kx = t < T ? tile * T + t : ntile * T + tile;
+      base[c * cs32 + z * P + kx] = buf[(c * TM + t) * LS + z];
+    }
   }
 }
-FHD_KERNEL_PAIR((template <int MZ, int NT, int TM, int PZ>), NT, k_fftB, FftPass,
+FHD_KERNEL_PAIR_OCC((template <int MZ, int NT, int TM, int PZ>), NT, 8, k_fftB, FftPass,
                 body_fftB<MZ, NT, TM, PZ>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------

@@ -63,6 +63,18 @@ FHD_BODY const A& batch_entry(const int4* __restrict__ blk, const int4* __restri
     __VA_ARGS__;                                                                                   \
   }
 
+// The same pair with a minimum of WPE waves per SIMD (a register budget of 512 / WPE VGPRs).
+#define FHD_KERNEL_PAIR_OCC(TPL, NT, WPE, name, A, ...)                                           \
+  FHD_UNPAREN(TPL) __global__ __launch_bounds__(NT, WPE) void name(A a) {                          \
+    const Blk b = blk_here();                                                                      \
+    __VA_ARGS__;                                                                                   \
+  }                                                                                                \
+  FHD_UNPAREN(TPL) __global__ __launch_bounds__(NT, WPE) void name##_b(FHD_BATCH_PARAMS) {        \
+    Blk b;                                                                                         \
+    const A& a = batch_entry<A>(fhd_blk, fhd_dims, fhd_args, fhd_stride, b);                       \
+    __VA_ARGS__;                                                                                   \
+  }
+
 // Stage families: launches of one family from different contexts merge into one dispatch
 // (one per distinct kernel instance).
 enum Family {
