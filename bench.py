@@ -31,8 +31,9 @@ def kernel_bytes(name, mx, my, mz, npart):
     pass.  Per cell: spread writes 12 B; the W + xy-forward pass reads 12 and writes 12 (its noise
     is generated in registers); the z pass reads and writes the half spectrum, 24; the xy-inverse
     pass 24; the interpolation reads 12.  Per particle: sort 32, force 24, spread 24,
-    interpolation and update 24.  Passes the formula does not charge (the separate noise pass,
-    the spreading records) get 0 B; their cost is time the fused pipeline would not spend."""
+    interpolation and update 24.  Passes the formula does not charge (the spreading records;
+    the W pass of the other schemes) get 0 B; their cost is time a fused pipeline would not
+    spend.  The Stokes step's noise is drawn in registers inside the z pass."""
     N = mx * my * mz
     per = {
         "spread": 12 * N + 24 * npart,
@@ -294,7 +295,7 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--jobs", type=int, default=24)
     ap.add_argument("--rate", type=float, default=8.0, help="served: job arrivals per second")
-    ap.add_argument("--serve-mode", choices=("streams", "merged"), default="merged",
+    ap.add_argument("--serve-mode", choices=("streams", "merged"), default="streams",
                     help="served: a stream per job, or the running jobs as one merged-stage batch")
     ap.add_argument("--batch-mode", type=int, default=2, choices=(0, 1, 2),
                     help="batched: 0 = per-replica graphs, 1 = one captured graph per batch step, "

@@ -105,11 +105,19 @@ constexpr int plan_radix(int n, int st) {
 constexpr uint32_t plan2(int r0, int r1) { return (uint32_t)r0 | ((uint32_t)r1 << 5); }
 constexpr uint32_t plan3(int r0, int r1, int r2) { return plan2(r0, r1) | ((uint32_t)r2 << 10); }
 constexpr int plan_at(uint32_t plan, int n, int st) { return plan ? (int)((plan >> (5 * st)) & 31u) : plan_radix(n, st); }
-// Candidate plans: index 0 is the default for every length; 64 and 128 have two more.
+// Candidate plans: index 0 is the default for every length; 48, 64, 96 and 128 have two more
+// (for 48 and 96, other placements of the radix-3 stage).
+constexpr uint32_t kPlan48[3] = {0u, plan2(3, 16), plan3(4, 4, 3)};
 constexpr uint32_t kPlan64[3] = {0u, plan2(4, 16), plan2(8, 8)};
+constexpr uint32_t kPlan96[3] = {0u, plan3(8, 4, 3), plan3(3, 8, 4)};
 constexpr uint32_t kPlan128[3] = {0u, plan2(8, 16), plan3(4, 4, 8)};
 constexpr uint32_t plan_choice(int n, int i) {
-  return i == 0 ? 0u : n == 64 ? kPlan64[i] : n == 128 ? kPlan128[i] : 0u;
+  return i == 0     ? 0u
+         : n == 48  ? kPlan48[i]
+         : n == 64  ? kPlan64[i]
+         : n == 96  ? kPlan96[i]
+         : n == 128 ? kPlan128[i]
+                    : 0u;
 }
 
 }  // namespace fhd

@@ -322,7 +322,11 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
     }
   }
 }
-FHD_KERNEL_PAIR_OCC((template <int MZ, int NT, int TM, int PZ>), NT, 8, k_fftB, FftPass,
+// 8 waves per SIMD (a 64-VGPR budget) where the LDS of that many workgroups fits in a CU
+constexpr int fftb_waves(int MZ, int NT, int TM) {
+  return (32 / (NT / 64)) * (3 * TM * (MZ + 1) * 8 + MZ * 20) <= 160 * 1024 ? 8 : 1;
+}
+FHD_KERNEL_PAIR_OCC((template <int MZ, int NT, int TM, int PZ>), NT, fftb_waves(MZ, NT, TM), k_fftB, FftPass,
                 body_fftB<MZ, NT, TM, PZ>(a.f, a.spec, a.T, b))
 
 // ---------------- dispatch ----------------
@@ -333,12 +337,12 @@ static void launch_AC_plan(const FftPass& x, int which, int mz, hipStream_t s) {
   else
     launch(k_fftC<MX, MY, NT, PX, PY>, k_fftC_b<MX, MY, NT, PX, PY>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
 }
-// The radix plans of the x and y transforms (plan_choice indices, px + 3 py), or 9 for the
-// register-transform kernels of fft128.hip, come from the schedule autotuner for 128 x 128
-// planes; other planes use the default plan.
+// The radix plans of the x and y transforms (plan_choice indices, px + 3 py) come from the
+// schedule autotuner for 96 x 96 and 128 x 128 planes (128 x 128 also has 9: the
+// register-transform kernels of fft128.hip); other planes use the default plan.
 template <int MX, int MY, int NT>
 static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
-  if constexpr (MX == 128 && MY == 128) {
+  if constexpr ((MX == 128 && MY == 128) || (MX == 96 && MY == 96)) {
     switch (x.f.plan_xy) {
       case 1: launch_AC_plan<MX, MY, NT, 1, 0>(x, which, mz, s); return;
       case 2: launch_AC_plan<MX, MY, NT, 2, 0>(x, which, mz, s); return;
@@ -348,7 +352,12 @@ static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
       case 6: launch_AC_plan<MX, MY, NT, 0, 2>(x, which, mz, s); return;
       case 7: launch_AC_plan<MX, MY, NT, 1, 2>(x, which, mz, s); return;
       case 8: launch_AC_plan<MX, MY, NT, 2, 2>(x, which, mz, s); return;
-      case 9: launch_fft128_xy(x, which, mz, s); return;  // transforms in registers (fft128.hip)
+      case 9:
+        if constexpr (MX == 128) {
+          launch_fft128_xy(x, which, mz, s);  // transforms in registers (fft128.hip)
+          return;
+        }
+        break;
       default: break;
     }
   }
@@ -427,10 +436,11 @@ static void launch_B_plan(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
   else if (bt == 8) launch(k_fftB<MZ, NT, 9, PZ>, k_fftB_b<MZ, NT, 9, PZ>, FAM_FFT_Z, grid, NT, x, s);
   else launch(k_fftB<MZ, NT, 17, PZ>, k_fftB_b<MZ, NT, 17, PZ>, FAM_FFT_Z, grid, NT, x, s);
 }
-// The z transform's radix plan (plan_choice index) comes from the autotuner for 128-point columns.
+// The z transform's radix plan (plan_choice index) comes from the autotuner for 48-, 96- and
+// 128-point columns.
 template <int MZ, int NT>
 static void launch_B_nt(const FftPass& x, int bt, dim3 grid, hipStream_t s) {
-  if constexpr (MZ == 128) {
+  if constexpr (MZ == 48 || MZ == 96 || MZ == 128) {
     if (x.f.plan_zs == 1) return launch_B_plan<MZ, NT, 1>(x, bt, grid, s);
     if (x.f.plan_zs == 2) return launch_B_plan<MZ, NT, 2>(x, bt, grid, s);
   }

@@ -473,22 +473,34 @@ void apply_schedule(fhd_ctx* c) {
   } else if (tune) {
     // scratch runs on the context's own buffers before its state exists (contents irrelevant)
     float best = 1e30f;
-    // z pass: column tile and, for 128-point columns, the radix plan (16 8, 8 16 or 4 4 8)
-    const int nzp = c->p.mz == 128 ? 3 : 1;
+    const bool stokes = c->p.scheme == FHD_SCHEME_STOKES_LIMIT;
+    // z pass: column tile and, for 48-, 96- and 128-point columns, the radix plan (128: 16 8,
+    // 8 16 or 4 4 8; 96: 16 2 3, 8 4 3 or 3 8 4; 48: 16 3, 3 16 or 4 4 3), timed in the form
+    // the scheme launches (the Stokes pass draws its stochastic stress)
+    FftArgs za = c->fa;
+    if (stokes) {
+      za.stokes = 1;
+      za.knoise = 2;
+      za.kn_f1 = za.kn_f4 = 1.0f;
+      za.key = make_key(c);
+    }
+    const int mzp = c->p.mz;
+    const int nzp = (mzp == 48 || mzp == 96 || mzp == 128) ? 3 : 1;
     for (int pz = 0; pz < nzp; ++pz)
       for (int bt : {2, 4, 8, 16}) {  // 2: more column workgroups on small grids
-        c->fa.btile = bt;
-        c->fa.plan_zs = pz;
-        const float t = time_launches(c, [&] { launch_fft_z_solve(c->fa, c->spec, c->s); }, 4);
+        za.btile = bt;
+        za.plan_zs = pz;
+        const float t = time_launches(c, [&] { launch_fft_z_solve(za, c->spec, c->s); }, 4);
         if (t < best) { best = t; sc.btile = bt; sc.plan_zs = pz; }
       }
     c->fa.plan_zs = sc.plan_zs;
-    // xy passes of 128 x 128 planes: the radix plans of the 64-point x and 128-point y
-    // transforms (three each) of the LDS-stage kernels, or the register-transform kernels
+    // xy passes of 96 x 96 and 128 x 128 planes: the radix plans of the x and y transforms
+    // (three each) of the LDS-stage kernels, and for 128 x 128 the register-transform kernels
     // (plan 9), timed as a forward and an inverse pass
-    if (c->p.mx == 128 && c->p.my == 128) {
+    const bool sq128 = c->p.mx == 128 && c->p.my == 128, sq96 = c->p.mx == 96 && c->p.my == 96;
+    if (sq128 || sq96) {
       best = 1e30f;
-      for (int pxy = 0; pxy < 10; ++pxy) {  // 9: the register-transform kernels
+      for (int pxy = 0; pxy < (sq128 ? 10 : 9); ++pxy) {  // 9: the register-transform kernels
         c->fa.plan_xy = pxy;
         const float t = time_launches(c, [&] {
           launch_fft_xy_fwd(c->fa, c->W, c->spec, c->s);
@@ -498,24 +510,23 @@ void apply_schedule(fhd_ctx* c) {
       }
       c->fa.plan_xy = sc.plan_xy;
     }
-    // W kernel: 2 planes per workgroup fill the chip on small grids (64^3: 512 workgroups) when
-    // the context launches alone; a batched dispatch already holds many contexts' workgroups,
-    // where the redundant halo planes of short chunks cost more, so it keeps the best of 4-16
+    // W kernel (incompressible and QNB; the Stokes step has none): 2 planes per workgroup fill
+    // the chip on small grids (64^3: 512 workgroups) when the context launches alone; a
+    // batched dispatch already holds many contexts' workgroups, where the redundant halo planes
+    // of short chunks cost more, so it keeps the best of 4-16
     best = 1e30f;
     float best_b = 1e30f;
     for (int zc : {2, 4, 8, 16}) {
+      if (stokes) break;
       c->zchunk = zc;
-      // the form the scheme launches: noise only (Stokes), or the full stencil with advection
       FluidWArgs w = base_w(c);
       w.noise = 1;
       w.f1 = w.f4 = 1.0f;
-      if (c->p.scheme != FHD_SCHEME_STOKES_LIMIT) {
-        w.v = c->v;
-        w.vadv = c->v;
-        w.lap_coef = 1.0f;
-        w.adv_coef = 1.0f;
-        w.trace_free = 1;
-      }
+      w.v = c->v;
+      w.vadv = c->v;
+      w.lap_coef = 1.0f;
+      w.adv_coef = 1.0f;
+      w.trace_free = 1;
       const float t = time_launches(c, [&] { launch_fluid_w(w, c->s); }, 4);
       if (t < best) { best = t; sc.zchunk = zc; }
       if (zc >= 4 && t < best_b) { best_b = t; sc.zchunk_b = zc; }
@@ -527,8 +538,8 @@ void apply_schedule(fhd_ctx* c) {
   c->fa.plan_zs = sc.plan_zs;
   c->zchunk = sc.zchunk;
   c->zchunk_b = sc.zchunk_b;
-  // explicit radix plans (tests and experiments): FHD_PLAN_XY in 0-9 (128 x 128 planes),
-  // FHD_PLAN_Z in 0-2 (128-point columns)
+  // explicit radix plans (tests and experiments): FHD_PLAN_XY in 0-9 (128 x 128 planes; 0-8
+  // for 96 x 96), FHD_PLAN_Z in 0-2 (48-, 96- and 128-point columns)
   if (const char* e = getenv("FHD_PLAN_XY")) {
     const int v = atoi(e);
     if (v < 0 || v > 9) throw std::runtime_error("FHD_PLAN_XY must be 0-9");
@@ -553,7 +564,8 @@ void apply_schedule(fhd_ctx* c) {
 }
 
 // ---------------- particles ----------------
-// Counting sort of position set k by tile, then each tile ordered by particle index.
+// Counting sort of position set k by tile (gated off on the device while every particle is
+// still inside its listed tile's skin).
 void sort_set(fhd_ctx* c, int k) {
   SortArgs a;
   a.g = c->g;

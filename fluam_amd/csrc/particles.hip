@@ -1,8 +1,10 @@
 // particles.hip -- immersed-boundary particle kernels for gfx950.
 //
-//  * binning: particles are bucketed by an 8x8x8-cell tile of the fluid grid (counting sort),
-//    then each bucket is ordered by particle index, so every later pass visits particles in a
-//    fixed order whatever order the atomics placed them in.
+//  * binning: particles are bucketed by an 8x8x8-cell tile of the fluid grid (counting sort,
+//    two launches).  The order inside a bucket is the order the atomics handed out, so every
+//    later sum over a tile's particles is formed in fixed point, where it does not depend on
+//    that order.  The sort is kept between passes: a particle stays listed in its tile until
+//    it leaves the tile's skin, and the sort launches do nothing until a flag says one has.
 //  * WCA forces, one workgroup per tile, from the particles of the neighbouring tiles staged
 //    in LDS (fluam's cell-list pair loop, src/stokesLimitFunctions.cu:160-653, with the
 //    analytic potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).

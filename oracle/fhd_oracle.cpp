@@ -1,6 +1,6 @@
 // fhd_oracle.cpp -- FP64 CPU restatement of fluam's incompressible (schemeIncompressible),
-// inertial-coupling (schemeQuasiNeutrallyBuoyant, excess mass 0) and Stokes-limit
-// (schemeStokesLimit, midpoint) time steps.
+// inertial-coupling (schemeQuasiNeutrallyBuoyant, with and without excess mass) and
+// Stokes-limit (schemeStokesLimit, midpoint) time steps.
 //
 // TEST INFRASTRUCTURE: the checker for libfhd.so.  Loaded only by tests/, the smoke check
 // and the benchmark's CPU-baseline leg.  Never linked into the product.
@@ -8,7 +8,8 @@
 // Each function cites the fluam source it restates.  Deliberate differences from fluam
 // (all shared with the GPU library, so parity compares like with like):
 //   * noise: counter-based Philox4x32-10 + Box-Muller instead of cuRAND XORWOW
-//     (src/randomNumbers.cu:25-40); same slot meaning (6 stress components per cell).
+//     (src/randomNumbers.cu:25-40); same slot meaning (6 stress components per cell).  The
+//     Stokes step draws the Fourier transform of a white stress field instead (same law).
 //   * kernels: closed-form Peskin kernels in FP64 (fluam evaluates sqrtf in its 3-point
 //     delta, src/deltaGPU.cu:57-68, and uses float textures for the QNB drift and the
 //     WCA force, src/deltaGPU.cu:74-105, src/initForcesNonBonded.cu:35-66).

@@ -15,6 +15,16 @@
  *   orc_philox            Random123 known-answer vectors
  *   orc_fft3d             numpy.fft golden fixtures (tests/golden) + brute-force DFT
  *   orc_delta3/orc_delta4 closed-form values and Peskin moment conditions
+ *   orc_ddelta3/4         central finite differences of the pinned kernels; moments
+ *   orc_spread_drift      the QNB thermal drift equals the finite-difference gradient of the
+ *                         pinned spread (3- and 4-point)
+ *   orc_spread_quirk      built directly from its definition (nodes at q +- d/2, weights at q)
+ *   orc_laplacian_term,   dense staggered L and D on a 4^3 grid; fluctuation-dissipation
+ *   orc_noise_divergence  D D^T = -L
+ *   orc_advection_term    momentum conservation, skew properties, continuum limit
+ *   orc_kspace_stress     Hermitian (real after the inverse transform), white N(0,1) real
+ *                         fields, per-mode second moments, divergence symbol equal to the
+ *                         pinned real-space noise divergence
  *   orc_wca               analytic WCA (zero at 2^(1/6) sigma), Newton's third law
  *   orc_bonded            closed-form traps and springs, F = -grad V by finite differences of
  *                         V = sum k/2 (r - r0)^2, Newton's third law
@@ -23,11 +33,12 @@
  *                         divergence, momentum conservation, equal-time S(k)
  *   Stokes solve          dense saddle-point solve on a 4^3 grid, zero mean
  *   Stokes step           the midpoint rule over the pinned trap force, spread, Stokes solve,
- *                         interpolation, noise slots, noise divergence and RFD spreads
- *                         (kT = 0 and kT > 0)
+ *                         interpolation, Fourier-space stress and RFD spreads (kT = 0 and
+ *                         kT > 0), with extra mobility and with the reference's RFD variant
  *   QNB step              mass=0 reduces to the incompressible step with no particles;
- *                         particle velocities = J v; divergence; at kT = 0 the bootstrap and
- *                         Adams-Bashforth steps over the pinned blocks
+ *                         particle velocities = J v; divergence; the bootstrap and
+ *                         Adams-Bashforth steps over the pinned blocks at kT = 0 and kT > 0
+ *                         (noise and thermal drift)
  *   QNB excess mass       m_e -> 0 reproduces the mass-0 step; total momentum
  *                         rho dV sum v + m_e sum u is conserved; at kT = 0 the bootstrap and
  *                         Adams-Bashforth steps with the excess-mass corrections over the

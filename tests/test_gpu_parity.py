@@ -51,13 +51,15 @@ def test_fft_r2c_matches_numpy(shape):
     assert np.abs(back - a * a.size).max() < 1e-5 * np.abs(a * a.size).max()
 
 
-@pytest.mark.parametrize("plan_xy,plan_z", [(k, k % 3) for k in range(10)])
-def test_fft_every_radix_plan_matches_numpy(plan_xy, plan_z):
-    """The radix plans the schedule autotuner chooses among for 128^3 (x: 16 4, 4 16, 8 8;
-    y and z: 16 8, 8 16, 4 4 8; xy plan 9: the register-transform kernels), each forced: the
-    transforms match numpy to FP32 accuracy."""
-    shape = (128, 128, 128)
-    rng = np.random.default_rng(100 + plan_xy)
+@pytest.mark.parametrize("shape,plan_xy,plan_z", [((128, 128, 128), k, k % 3) for k in range(10)]
+                         + [((96, 96, 96), k, (k + 1) % 3) for k in range(9)]
+                         + [((48, 32, 64), 0, k) for k in range(3)])
+def test_fft_every_radix_plan_matches_numpy(shape, plan_xy, plan_z):
+    """The radix plans the schedule autotuner chooses among, each forced: for 128^3 (x: 16 4,
+    4 16, 8 8; y and z: 16 8, 8 16, 4 4 8; xy plan 9: the register-transform kernels), for 96^3
+    (x: 16 3, 3 16, 4 4 3; y and z: 16 2 3, 8 4 3, 3 8 4) and 48-point columns (16 3, 3 16,
+    4 4 3).  The transforms match numpy to FP32 accuracy."""
+    rng = np.random.default_rng(100 + plan_xy + shape[0])
     a = rng.standard_normal(shape)
     os.environ["FHD_PLAN_XY"] = str(plan_xy)
     os.environ["FHD_PLAN_Z"] = str(plan_z)
