@@ -163,10 +163,11 @@ int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms);
 int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern);
 
 /* The schedule the context's autotuner chose for its grid shape (no fluam counterpart): out[0] the
- * z-pass column tile, out[1] the xy passes' radix plan (0-8 for 128 x 128 planes: x plan + 3 y
- * plan, plans 16 4 | 4 16 | 8 8 and 16 8 | 8 16 | 4 4 8; 9: the transforms in registers with one
- * exchange each), out[2] the z pass's radix plan (0-2 for
- * 128-point columns), out[3] and out[4] the W kernel's z chunk alone and in batched dispatches. */
+ * z-pass column tile, out[1] the xy passes' radix plan (x plan + 3 y plan, 0-8, for 128 x 128
+ * planes: x 16 4 | 4 16 | 8 8, y 16 8 | 8 16 | 4 4 8, and 9: the transforms in registers; for
+ * 96 x 96 planes: x 16 3 | 3 16 | 4 4 3, y 16 2 3 | 8 4 3 | 3 8 4; 0 elsewhere), out[2] the z
+ * pass's radix plan (0-2 for 48-, 96- and 128-point columns, the same candidates as y), out[3]
+ * and out[4] the W kernel's z chunk alone and in batched dispatches. */
 int fhd_get_schedule(const fhd_ctx* c, int out[5]);
 
 /* Divergence diagnostics of the current velocity: max |D v| over cells, in units of
