@@ -176,6 +176,12 @@ struct fhd_ctx {
   float4 *dulag = nullptr, *du = nullptr, *dp = nullptr, *jvt = nullptr, *ms = nullptr;
   float skin[3] = {-1, -1, -1};  // cells a listed particle may stray outside its tile (-1: any)
   float skin_len = 0.0f;
+  // WCA neighbour lists (by id): kNbCap entries per particle, column-major; their reference
+  // positions; the stale flag; the list skin (length) beyond the cutoff
+  int *nb_idx = nullptr, *nb_cnt = nullptr, *nb_flag = nullptr;
+  int4* nb_ref_cell = nullptr;
+  float4* nb_ref_off = nullptr;
+  float nb_skin = 0.0f;
   int* start[2] = {nullptr, nullptr};
   int* sid[2] = {nullptr, nullptr};
   int4* scell[2] = {nullptr, nullptr};
@@ -355,7 +361,8 @@ void free_ctx(fhd_ctx* c) {
                   c->F[0], c->F[1], c->u, c->count, c->slot, c->tmp, c->start[0], c->start[1], c->sid[0],
                   c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
                   c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo, c->sk_acc,
-                  c->slot_of, c->sflag, c->M1, c->dulag, c->du, c->dp, c->jvt, c->ms};
+                  c->slot_of, c->sflag, c->M1, c->dulag, c->du, c->dp, c->jvt, c->ms, c->nb_idx, c->nb_cnt,
+                  c->nb_flag, c->nb_ref_cell, c->nb_ref_off};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -612,6 +619,16 @@ void bonded(fhd_ctx* c, int k, float4* F) {
   launch_bonded(a, c->s);
   mark(c, "bonded");
 }
+NbList nb_list(fhd_ctx* c) {
+  NbList n;
+  n.idx = c->nb_idx;
+  n.cnt = c->nb_cnt;
+  n.ref_cell = c->nb_ref_cell;
+  n.ref_off = c->nb_ref_off;
+  n.flag = c->nb_flag;
+  return n;
+}
+
 void wca(fhd_ctx* c, int k, float4* F) {
   if (!c->p.nonbonded || c->p.cutoff <= 0) {
     launch_zero_f4(F, c->np, c->s);
@@ -622,13 +639,19 @@ void wca(fhd_ctx* c, int k, float4* F) {
   a.g = c->g;
   a.t = c->t;
   a.set = sset(c, k);
+  a.cell = c->cell[k];
+  a.off = c->off[k];
   a.rc2 = (float)(c->p.cutoff * c->p.cutoff);
+  a.rl2 = (float)((c->p.cutoff + c->nb_skin) * (c->p.cutoff + c->nb_skin));
   a.sig2 = (float)(c->p.wca_sigma * c->p.wca_sigma);
   a.eps48 = (float)(48.0 * c->p.wca_epsilon);
   a.skin_len = c->skin_len;
+  a.nb = nb_list(c);
   a.F = F;
   a.np = c->np;
-  launch_wca(a, c->s);
+  launch_wca(a, c->s);  // the list build, which returns at once while the lists are valid
+  mark(c, "wca_build");
+  launch_wca_list(a, c->s);
   mark(c, "wca");
   bonded(c, k, F);
 }
@@ -665,6 +688,9 @@ InterpArgs interp_base(fhd_ctx* c, int eval_set) {
   a.eval = sset(c, eval_set);
   a.key = make_key(c);
   a.err = c->errd;
+  a.nb = nb_list(c);
+  const double move = 0.45 * c->nb_skin;
+  a.nb_move2 = (float)(move * move);
   a.np = c->np;
   return a;
 }
@@ -1776,21 +1802,45 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
       c->sflag = dalloc<int>(2);
       const int flags0[2] = {1, 0};
       HIPCHK(hipMemcpy(c->sflag, flags0, sizeof(flags0), hipMemcpyHostToDevice));
-      // skin of the kept sort, per axis with three or more tiles: half a cell (the WCA pass
+      // skin of the WCA neighbour lists: a quarter of the cutoff, so that a list built at
+      // radius rc + skin stays complete for hundreds of steps at the particles' ~1e-3 cells
+      // per step; the list radius stays within a tile (axes with three or more tiles) and
+      // below half the box
+      const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz};
+      const double h[3] = {c->dx, c->dy, c->dz};
+      const bool wca_on = p.nonbonded && p.cutoff > 0;
+      if (wca_on) {
+        double sk = 0.25 * p.cutoff;
+        const int m[3] = {p.mx, p.my, p.mz};
+        for (int a = 0; a < 3; ++a) {
+          if (NT[a] >= 3) sk = std::min(sk, T[a] * h[a] - p.cutoff);
+          sk = std::min(sk, 0.49 * m[a] * h[a] - p.cutoff);
+        }
+        c->nb_skin = (float)std::max(sk, 0.0);
+        c->nb_idx = dalloc<int>((size_t)kNbCap * c->np);
+        c->nb_cnt = dalloc<int>(c->np);
+        c->nb_ref_cell = dalloc<int4>(c->np);
+        c->nb_ref_off = dalloc<float4>(c->np);
+        c->nb_flag = dalloc<int>(1);
+        HIPCHK(hipMemset(c->nb_idx, 0, sizeof(int) * kNbCap * c->np));  // every entry a valid id
+        HIPCHK(hipMemset(c->nb_cnt, 0, sizeof(int) * c->np));
+        HIPCHK(hipMemset(c->nb_ref_cell, 0, sizeof(int4) * c->np));
+        HIPCHK(hipMemset(c->nb_ref_off, 0, sizeof(float4) * c->np));
+        const int one = 1;
+        HIPCHK(hipMemcpy(c->nb_flag, &one, sizeof(int), hipMemcpyHostToDevice));
+      }
+      // skin of the kept sort, per axis with three or more tiles: half a cell (the list build
       // culls from the tile grown by it, so it is kept small; particles move ~1e-3 cells per
       // step, so a sort is rarely needed).  The spreading pass finds a source in its tile's
       // neighbours while its stencil (two cells each way, one more for an RFD displacement)
-      // stays within them, s <= T - 3; the WCA pass finds every pair while no tile beyond the
-      // neighbours can hold a partner, 2 s h + rc <= T h.
+      // stays within them, s <= T - 3; the list build finds every pair within its radius while
+      // no tile beyond the neighbours can hold a partner, 2 s h + rc + list skin <= T h.
       {
-        const int T[3] = {t.tx, t.ty, t.tz}, NT[3] = {t.ntx, t.nty, t.ntz};
-        const double h[3] = {c->dx, c->dy, c->dz};
-        const bool wca_on = p.nonbonded && p.cutoff > 0;
         c->skin_len = 0.0f;
         for (int a = 0; a < 3; ++a) {
           if (NT[a] <= 2) { c->skin[a] = -1.0f; continue; }
           double sk = std::min(0.5, (double)(T[a] - 3));
-          if (wca_on) sk = std::min(sk, (T[a] * h[a] - p.cutoff) / (2.0 * h[a]));
+          if (wca_on) sk = std::min(sk, (T[a] * h[a] - p.cutoff - c->nb_skin) / (2.0 * h[a]));
           c->skin[a] = (float)std::max(sk, 0.0);
           c->skin_len = std::max(c->skin_len, (float)(c->skin[a] * h[a]));
         }
@@ -1921,6 +1971,7 @@ int fhd_set_particles(fhd_ctx* c, const double* q) {
       HIPCHK(hipMemcpy(c->off[0], ho.data(), np * sizeof(float4), hipMemcpyHostToDevice));
       const int one = 1;  // the sorted copies are stale: sort again before the next use
       HIPCHK(hipMemcpy(c->sflag, &one, sizeof(int), hipMemcpyHostToDevice));
+      if (c->nb_flag) HIPCHK(hipMemcpy(c->nb_flag, &one, sizeof(int), hipMemcpyHostToDevice));  // and rebuild the lists
     }
   });
   return 1;

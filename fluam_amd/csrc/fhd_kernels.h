@@ -150,6 +150,18 @@ struct SpreadArgs {
 // stays listed in its tile while it lies within `skin` cells of the tile along every axis that
 // has three or more tiles; beyond that the pass raises *skin_flag and the set is sorted again
 // before its next use.
+// Verlet neighbour lists of the WCA pass: every pair closer than rc + list skin when the list
+// was built, kept until some particle has moved more than 0.45 list skins from where it was
+// then (interpolation passes raise `flag`; the next force pass rebuilds).
+constexpr int kNbCap = 32;  // neighbours stored per particle
+struct NbList {
+  int* idx;           // neighbour ids, column-major: entry j of particle i at idx[j * np + i]
+  int* cnt;           // neighbours of each particle (> kNbCap: the list overflowed)
+  int4* ref_cell;     // positions at the build (wrapped cell, offset), by id
+  float4* ref_off;
+  int* flag;          // 1: rebuild the lists before the next force pass
+};
+
 struct SkinArgs {
   int4* s_cell;          // sorted copy to update (wrapped cell; w = listed tile, kept)
   float4* s_off;
@@ -179,6 +191,8 @@ struct InterpArgs {
   float aN; int nnoise;                // + aN (Z_1 [+ Z_2]) particle noise
   Key key;
   int* err;
+  NbList nb;             // neighbour lists to invalidate when a particle moves (flag null: none)
+  float nb_move2;        // squared displacement from the list's reference that invalidates it
   int np;
 };
 
@@ -232,8 +246,12 @@ struct WcaArgs {
   GridDev g;
   Tiles t;
   SortedSet set;
+  const int4* cell;   // the same positions by id (the list pass)
+  const float4* off;
   float rc2, sig2, eps48;
+  float rl2;       // list radius squared (rc + list skin)^2
   float skin_len;  // how far (length) a listed particle may lie outside its tile
+  NbList nb;
   float4* F;   // by id
   int np;
 };
@@ -277,6 +295,7 @@ void launch_sk_accum(const float2* spec, double* acc, size_t n, hipStream_t s); 
 
 void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per-tile ordering
 void launch_wca(const WcaArgs& a, hipStream_t s);
+void launch_wca_list(const WcaArgs& a, hipStream_t s);
 void launch_bonded(const BondArgs& a, hipStream_t s);  // F += bond forces (after the WCA pass)
 void launch_src_prep(const SpreadArgs& a, hipStream_t s);  // per-source records (run first)
 void launch_spread(const SpreadArgs& a, hipStream_t s);    // tile accumulation

@@ -167,145 +167,166 @@ FHD_DEV long long fixed_term(float x) {
 constexpr int kWcaThreads = 256;
 constexpr int kWcaSlices = 16;  // most lanes sharing one particle's candidates
 constexpr int kWcaCap = 512;    // culled candidates staged in LDS at once
+constexpr int kWcaBuildGroups = 1024;  // most workgroups of a list build
 
-// WCA, one workgroup per tile: F_i = sum_j 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij for
-// r < rc (fluam's cell-list pair loop, src/stokesLimitFunctions.cu:160-653, with the analytic
-// potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).  The
-// particles of the neighbouring tiles that lie within rc of the tile are compacted into LDS in
-// sorted order (a deterministic ballot compaction), once per tile when they fit kWcaCap
+// WCA pair term on particle i from particle j: F_ij = 48 eps [(s/r)^12 - (s/r)^6 / 2] / r^2 r_ij
+// for 0 < r < rc (fluam's pair loop, src/stokesLimitFunctions.cu:160-653, with the analytic
+// potential instead of the float texture table of src/initForcesNonBonded.cu:21-66).  Cells
+// wrapped, offsets from the cell centre.  Returns false when the pair does not interact.
+FHD_DEV bool wca_term(const WcaArgs& a, int4 ci, float4 di, int4 cj, float4 dj, float& tx, float& ty, float& tz) {
+  const GridDev& g = a.g;
+  const float rx = (float)min_image(ci.x - cj.x, g.mx) * g.dx + (di.x - dj.x);
+  const float ry = (float)min_image(ci.y - cj.y, g.my) * g.dy + (di.y - dj.y);
+  const float rz = (float)min_image(ci.z - cj.z, g.mz) * g.dz + (di.z - dj.z);
+  float r2 = rx * rx + ry * ry + rz * rz;
+  if (!(r2 < a.rc2 && r2 > 0.0f)) return false;
+  // separations below sigma/100 are evaluated at sigma/100, so the force stays finite
+  // (fluam's tabulated force is bounded near r = 0 too, src/initForcesNonBonded.cu:59)
+  r2 = fmaxf(r2, 1.0e-4f * a.sig2);
+  const float s2 = a.sig2 / r2;
+  const float s6 = s2 * s2 * s2;
+  const float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
+  tx = f * rx;
+  ty = f * ry;
+  tz = f * rz;
+  return true;
+}
+
+// Neighbour-list build, one workgroup per tile, run only when the lists are flagged stale.  The
+// particles of the neighbouring tiles that lie within the list radius of the tile (grown by the
+// sort's skin) are compacted into LDS (ballot compaction), once per tile when they fit kWcaCap
 // entries, else in passes of kWcaCap; S lanes share each of the tile's particles (S = 16, or
-// the largest power of two with S x particles <= 256 in crowded tiles), lane l taking
-// candidates l, l + S, ... of each pass, and the partial forces are combined by a fixed
-// xor-shuffle tree, so every sum is formed in the same order on every run.
+// the largest power of two with S x particles <= 256 in crowded tiles) and append every
+// candidate closer than the list radius to its list.  The order inside a list is that of the
+// LDS atomics; the force pass sums in fixed point, where the order does not matter.
 FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
   __shared__ int4 ccell[kWcaCap];   // w: sorted slot
   __shared__ float4 coff[kWcaCap];
   __shared__ int nb_start[32], nb_pref[33];
   __shared__ int wsum[kWcaThreads / 64];
+  __shared__ int ncnt[kWcaThreads];
+  if (*a.nb.flag == 0) return;  // lists still valid (uniform over the grid)
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int tile = xcd_item(b.x, t.n);
-  const int s0 = a.set.start[tile], nown = a.set.start[tile + 1] - s0;
-  if (nown == 0) return;  // uniform: no barrier is skipped by part of the group
-  const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
-  const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
-  int lox, hix, loy, hiy, loz, hiz;
-  nb_range(t.ntx, lox, hix);
-  nb_range(t.nty, loy, hiy);
-  nb_range(t.ntz, loz, hiz);
-  const int nx = hix - lox + 1, ny = hiy - loy + 1, nz = hiz - loz + 1;
-  const int nnb = nx * ny * nz;  // <= 27
-  if (tid < 64) {
-    int cnt = 0;
-    if (tid < nnb) {
-      const int dx = lox + tid % nx, dy = loy + (tid / nx) % ny, dz = loz + tid / (nx * ny);
-      const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
-      const int j0 = a.set.start[nt];
-      nb_start[tid] = j0;
-      cnt = a.set.start[nt + 1] - j0;
-    }
-    const int incl = wave_incl_scan(cnt);
-    if (tid < 32) nb_pref[tid + 1] = incl;
-    if (tid == 0) nb_pref[0] = 0;
-  }
-  __syncthreads();
-  const int nall = nb_pref[nnb];
-  // conservative cull radius (the pair test below is exact), measured from the tile grown by
-  // the skin: a listed particle may lie up to skin_len outside its tile along each axis
-  const float rcc = sqrtf(a.rc2) * 1.001f + 1e-6f;
-  const float skin = a.skin_len;
-  const float Tx = t.tx * g.dx, Ty = t.ty * g.dy, Tz = t.tz * g.dz;
-  // The distance to the tile is measured from the candidate's image nearest the tile centre,
-  // folded on whole cells, so it can be off by up to one cell near the half-box point.  Along
-  // an axis where that point lies farther than rc + one cell outside the tile, the cull is
-  // exact; along an axis with one tile every candidate is inside.  Otherwise keep everything.
-  auto axis_ok = [&](int nt_a, int m, int T, float d) { return nt_a == 1 || 0.5f * (m - T) * d > rcc + skin + d; };
-  const bool cull = axis_ok(t.ntx, g.mx, t.tx, g.dx) && axis_ok(t.nty, g.my, t.ty, g.dy) &&
-                    axis_ok(t.ntz, g.mz, t.tz, g.dz);
-  // Compacts the near candidates with rank in [p0, p0 + kWcaCap) into LDS; returns how many
-  // near candidates there are in all.
-  auto stage = [&](int p0) {
-    int seen = 0;
-    for (int c0 = 0; c0 < nall; c0 += kWcaThreads) {
-      const int e = c0 + tid;
-      bool near = false;
-      int4 cj = make_int4(0, 0, 0, 0);
-      float4 dj = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (e < nall) {
-        // largest lo with nb_pref[lo] <= e (entries past nnb hold nall)
-        int lo = 0;
-#pragma unroll
-        for (int step = 16; step >= 1; step >>= 1)
-          if (nb_pref[lo + step] <= e) lo += step;
-        const int j = nb_start[lo] + (e - nb_pref[lo]);
-        cj = a.set.cell[j];
-        dj = a.set.off[j];
-        cj.w = j;
-        near = true;
-        if (cull) {
-          // position relative to the tile centre (nearest image), then the distance outside
-          // the tile along each axis (zero along an axis with a single tile)
-          const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
-          const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
-          const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
-          const float ex = t.ntx == 1 ? 0.0f : fmaxf(0.0f, fabsf(px) - 0.5f * Tx - skin);
-          const float ey = t.nty == 1 ? 0.0f : fmaxf(0.0f, fabsf(py) - 0.5f * Ty - skin);
-          const float ez = t.ntz == 1 ? 0.0f : fmaxf(0.0f, fabsf(pz) - 0.5f * Tz - skin);
-          near = ex * ex + ey * ey + ez * ez <= rcc * rcc;
-        }
+  // a grid of at most kWcaBuildGroups workgroups (a stride loop over the tiles), so the launch
+  // that finds the lists valid is short
+  for (int tb = b.x; tb < t.n; tb += b.nx) {
+    __syncthreads();  // the previous tile's LDS consumed
+    const int tile = xcd_item(tb, t.n);
+    const int s0 = a.set.start[tile], nown = a.set.start[tile + 1] - s0;
+    if (nown == 0) continue;  // uniform: no barrier is skipped by part of the group
+    const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
+    const int ox = tx * t.tx, oy = ty * t.ty, oz = tz * t.tz;
+    int lox, hix, loy, hiy, loz, hiz;
+    nb_range(t.ntx, lox, hix);
+    nb_range(t.nty, loy, hiy);
+    nb_range(t.ntz, loz, hiz);
+    const int nx = hix - lox + 1, ny = hiy - loy + 1, nz = hiz - loz + 1;
+    const int nnb = nx * ny * nz;  // <= 27
+    if (tid < 64) {
+      int cnt = 0;
+      if (tid < nnb) {
+        const int dx = lox + tid % nx, dy = loy + (tid / nx) % ny, dz = loz + tid / (nx * ny);
+        const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
+        const int j0 = a.set.start[nt];
+        nb_start[tid] = j0;
+        cnt = a.set.start[nt + 1] - j0;
       }
-      const unsigned long long bal = __ballot(near);
-      const int before_w = __popcll(bal & ((1ull << lane) - 1ull));
-      __syncthreads();  // wsum of the previous chunk consumed
-      if (lane == 0) wsum[wv] = __popcll(bal);
-      __syncthreads();
-      int base = seen, nk = 0;
-#pragma unroll
-      for (int w = 0; w < kWcaThreads / 64; ++w) {
-        base += w < wv ? wsum[w] : 0;
-        nk += wsum[w];
-      }
-      const int rank = base + before_w - p0;
-      if (near && rank >= 0 && rank < kWcaCap) {
-        ccell[rank] = cj;
-        coff[rank] = dj;
-      }
-      seen += nk;
+      const int incl = wave_incl_scan(cnt);
+      if (tid < 32) nb_pref[tid + 1] = incl;
+      if (tid == 0) nb_pref[0] = 0;
     }
     __syncthreads();
-    return seen;
-  };
-  const int nnear = stage(0);
-  const int npass = (nnear + kWcaCap - 1) / kWcaCap;
-  // lanes per own particle: up to kWcaSlices, fewer in crowded tiles so that one group of
-  // kWcaThreads / S particles covers the tile and the candidates are staged once
-  int S = kWcaSlices;
-  while (S > 1 && S * nown > kWcaThreads) S >>= 1;
-  for (int i0 = 0; i0 < nown; i0 += kWcaThreads / S) {
-    const int il = tid / S, sl = tid % S;
-    const int si = s0 + i0 + il;
-    const bool own = i0 + il < nown;
-    int4 ci = make_int4(0, 0, 0, 0);
-    float4 di = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (own) {
-      ci = a.set.cell[si];
-      di = a.set.off[si];
-    }
-    // The force is summed as 64-bit integers at a scale set by the largest force component of
-    // any of the particle's pair terms: each term keeps its FP32 value and integer sums are
-    // exact, so the force does not depend on the order of the candidates (the order inside a
-    // tile is that of the sort's atomics).  A first sweep finds the largest component and keeps
-    // up to four terms per lane in registers (pairs within rc are few); a second sweep
-    // recomputes the terms only when some lane had more, or the candidates came in passes.
-    float m = 0.0f, scale = 0.0f;
-    long long sx = 0, sy = 0, sz = 0;
-    float t0x = 0, t0y = 0, t0z = 0, t1x = 0, t1y = 0, t1z = 0, t2x = 0, t2y = 0, t2z = 0, t3x = 0, t3y = 0,
-          t3z = 0;
-    int nterm = 0;
-    for (int sweep = 0; sweep < 2; ++sweep) {
+    const int nall = nb_pref[nnb];
+    // conservative cull radius (the pair test below is exact), measured from the tile grown by
+    // the skin: a listed particle may lie up to skin_len outside its tile along each axis
+    const float rcc = sqrtf(a.rl2) * 1.001f + 1e-6f;
+    const float skin = a.skin_len;
+    const float Tx = t.tx * g.dx, Ty = t.ty * g.dy, Tz = t.tz * g.dz;
+    // The distance to the tile is measured from the candidate's image nearest the tile centre,
+    // folded on whole cells, so it can be off by up to one cell near the half-box point.  Along
+    // an axis where that point lies farther than the radius + one cell outside the tile, the
+    // cull is exact; along an axis with one tile every candidate is inside.  Otherwise keep
+    // everything.
+    auto axis_ok = [&](int nt_a, int m, int T, float d) { return nt_a == 1 || 0.5f * (m - T) * d > rcc + skin + d; };
+    const bool cull = axis_ok(t.ntx, g.mx, t.tx, g.dx) && axis_ok(t.nty, g.my, t.ty, g.dy) &&
+                      axis_ok(t.ntz, g.mz, t.tz, g.dz);
+    // Compacts the near candidates with rank in [p0, p0 + kWcaCap) into LDS; returns how many
+    // near candidates there are in all.
+    auto stage = [&](int p0) {
+      int seen = 0;
+      for (int c0 = 0; c0 < nall; c0 += kWcaThreads) {
+        const int e = c0 + tid;
+        bool near = false;
+        int4 cj = make_int4(0, 0, 0, 0);
+        float4 dj = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (e < nall) {
+          // largest lo with nb_pref[lo] <= e (entries past nnb hold nall)
+          int lo = 0;
+  #pragma unroll
+          for (int step = 16; step >= 1; step >>= 1)
+            if (nb_pref[lo + step] <= e) lo += step;
+          const int j = nb_start[lo] + (e - nb_pref[lo]);
+          cj = a.set.cell[j];
+          dj = a.set.off[j];
+          cj.w = j;
+          near = true;
+          if (cull) {
+            // position relative to the tile centre (nearest image), then the distance outside
+            // the tile along each axis (zero along an axis with a single tile)
+            const float px = ((float)min_image(cj.x - (ox + t.tx / 2), g.mx) + 0.5f) * g.dx + dj.x;
+            const float py = ((float)min_image(cj.y - (oy + t.ty / 2), g.my) + 0.5f) * g.dy + dj.y;
+            const float pz = ((float)min_image(cj.z - (oz + t.tz / 2), g.mz) + 0.5f) * g.dz + dj.z;
+            const float ex = t.ntx == 1 ? 0.0f : fmaxf(0.0f, fabsf(px) - 0.5f * Tx - skin);
+            const float ey = t.nty == 1 ? 0.0f : fmaxf(0.0f, fabsf(py) - 0.5f * Ty - skin);
+            const float ez = t.ntz == 1 ? 0.0f : fmaxf(0.0f, fabsf(pz) - 0.5f * Tz - skin);
+            near = ex * ex + ey * ey + ez * ez <= rcc * rcc;
+          }
+        }
+        const unsigned long long bal = __ballot(near);
+        const int before_w = __popcll(bal & ((1ull << lane) - 1ull));
+        __syncthreads();  // wsum of the previous chunk consumed
+        if (lane == 0) wsum[wv] = __popcll(bal);
+        __syncthreads();
+        int base = seen, nk = 0;
+  #pragma unroll
+        for (int w = 0; w < kWcaThreads / 64; ++w) {
+          base += w < wv ? wsum[w] : 0;
+          nk += wsum[w];
+        }
+        const int rank = base + before_w - p0;
+        if (near && rank >= 0 && rank < kWcaCap) {
+          ccell[rank] = cj;
+          coff[rank] = dj;
+        }
+        seen += nk;
+      }
+      __syncthreads();
+      return seen;
+    };
+    const int nnear = stage(0);
+    const int npass = (nnear + kWcaCap - 1) / kWcaCap;
+    // lanes per own particle: up to kWcaSlices, fewer in crowded tiles so that one group of
+    // kWcaThreads / S particles covers the tile and the candidates are staged once
+    int S = kWcaSlices;
+    while (S > 1 && S * nown > kWcaThreads) S >>= 1;
+    for (int i0 = 0; i0 < nown; i0 += kWcaThreads / S) {
+      const int il = tid / S, sl = tid % S;
+      const int si = s0 + i0 + il;
+      const bool own = i0 + il < nown;
+      int4 ci = make_int4(0, 0, 0, 0);
+      float4 di = make_float4(0.f, 0.f, 0.f, 0.f);
+      int id = 0;
+      if (own) {
+        ci = a.set.cell[si];
+        di = a.set.off[si];
+        id = a.set.id[si];
+      }
+      ncnt[tid] = 0;  // slot il of the group: entry il * S (lane sl == 0 of each particle)
+      __syncthreads();
       for (int p = 0; p < npass; ++p) {
-        if (npass > 1) {
+        if (npass > 1 && p > 0) {
           __syncthreads();  // the previous pass's candidates consumed
           stage(p * kWcaCap);
         }
@@ -315,60 +336,141 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
             const int4 cc = ccell[k];
             if (cc.w == si) continue;
             const float4 dd = coff[k];
-            float rx = (float)min_image(ci.x - cc.x, g.mx) * g.dx + (di.x - dd.x);
-            float ry = (float)min_image(ci.y - cc.y, g.my) * g.dy + (di.y - dd.y);
-            float rz = (float)min_image(ci.z - cc.z, g.mz) * g.dz + (di.z - dd.z);
-            float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < a.rc2 && r2 > 0.0f) {
-              // separations below sigma/100 are evaluated at sigma/100, so the force stays finite
-              // (fluam's tabulated force is bounded near r = 0 too, src/initForcesNonBonded.cu:59)
-              r2 = fmaxf(r2, 1.0e-4f * a.sig2);
-              const float s2 = a.sig2 / r2;
-              const float s6 = s2 * s2 * s2;
-              const float f = a.eps48 * (s6 * s6 - 0.5f * s6) / r2;
-              const float tx = f * rx, ty = f * ry, tz = f * rz;
-              if (sweep == 0) {
-                m = fmaxf(m, fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz))));
-                if (nterm == 0) { t0x = tx; t0y = ty; t0z = tz; }
-                else if (nterm == 1) { t1x = tx; t1y = ty; t1z = tz; }
-                else if (nterm == 2) { t2x = tx; t2y = ty; t2z = tz; }
-                else if (nterm == 3) { t3x = tx; t3y = ty; t3z = tz; }
-                ++nterm;
-              } else {
-                sx += fixed_term(tx * scale);
-                sy += fixed_term(ty * scale);
-                sz += fixed_term(tz * scale);
-              }
+            const float rx = (float)min_image(ci.x - cc.x, g.mx) * g.dx + (di.x - dd.x);
+            const float ry = (float)min_image(ci.y - cc.y, g.my) * g.dy + (di.y - dd.y);
+            const float rz = (float)min_image(ci.z - cc.z, g.mz) * g.dz + (di.z - dd.z);
+            if (rx * rx + ry * ry + rz * rz < a.rl2) {
+              const int pos = atomicAdd(&ncnt[il * S], 1);
+              if (pos < kNbCap) a.nb.idx[(size_t)pos * a.np + id] = a.set.id[cc.w];
             }
           }
         }
       }
+      __syncthreads();
+      if (own && sl == 0) {
+        a.nb.cnt[id] = ncnt[il * S];
+        a.nb.ref_cell[id] = ci;
+        a.nb.ref_off[id] = di;
+      }
+      if (npass > 1 && i0 + kWcaThreads / S < nown) {
+        __syncthreads();
+        stage(0);  // the next group of own particles starts from the first pass again
+      }
+    }
+  }
+}
+FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
+
+// The WCA force from the neighbour lists, one lane per particle (by id).  The force is summed
+// as 64-bit integers at a scale set by the largest force component of any of the particle's
+// pair terms: each term keeps its FP32 value and integer sums are exact, so the force does not
+// depend on the order of the list.  A particle whose list overflowed takes every particle of
+// the 27 tiles around its tile (the sorted set) instead.  Also clears the stale-list flag that
+// the build before it consumed.
+FHD_BODY void body_wca_list(const WcaArgs& a, const Blk& b) {
+  const int i = b.x * blockDim.x + threadIdx.x;
+  if (i == 0) *a.nb.flag = 0;
+  if (i >= a.np) return;
+  const GridDev& g = a.g;
+  const Tiles& t = a.t;
+  const int4 c0 = a.cell[i];
+  const int4 ci = make_int4(wrapi(c0.x, g.mx), wrapi(c0.y, g.my), wrapi(c0.z, g.mz), 0);
+  const float4 di = a.off[i];
+  const int n = a.nb.cnt[i];
+  // the first kNbHead entries are loaded with the count and their positions gathered together
+  // (entries past the count hold valid ids from earlier builds, or 0, and are not used)
+  constexpr int kNbHead = 4;
+  int jd[kNbHead];
+#pragma unroll
+  for (int j = 0; j < kNbHead; ++j) jd[j] = a.nb.idx[(size_t)j * a.np + i];
+  int4 cr[kNbHead];
+  float4 dr[kNbHead];
+#pragma unroll
+  for (int j = 0; j < kNbHead; ++j) {
+    cr[j] = a.cell[jd[j]];
+    dr[j] = a.off[jd[j]];
+  }
+  float m = 0.0f, scale = 0.0f;
+  long long sx = 0, sy = 0, sz = 0;
+  if (n <= kNbCap) {
+    float hx[kNbHead], hy[kNbHead], hz[kNbHead];
+    bool hon[kNbHead];
+#pragma unroll
+    for (int j = 0; j < kNbHead; ++j) {
+      const int4 cj = make_int4(wrapi(cr[j].x, g.mx), wrapi(cr[j].y, g.my), wrapi(cr[j].z, g.mz), 0);
+      hon[j] = j < n && wca_term(a, ci, di, cj, dr[j], hx[j], hy[j], hz[j]);
+      if (hon[j]) m = fmaxf(m, fmaxf(fabsf(hx[j]), fmaxf(fabsf(hy[j]), fabsf(hz[j]))));
+    }
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      for (int j = kNbHead; j < n; ++j) {
+        const int jj = a.nb.idx[(size_t)j * a.np + i];
+        const int4 cq = a.cell[jj];
+        const int4 cj = make_int4(wrapi(cq.x, g.mx), wrapi(cq.y, g.my), wrapi(cq.z, g.mz), 0);
+        float tx, ty, tz;
+        if (!wca_term(a, ci, di, cj, a.off[jj], tx, ty, tz)) continue;
+        if (sweep == 0) {
+          m = fmaxf(m, fmaxf(fabsf(tx), fmaxf(fabsf(ty), fabsf(tz))));
+        } else {
+          sx += fixed_term(tx * scale);
+          sy += fixed_term(ty * scale);
+          sz += fixed_term(tz * scale);
+        }
+      }
       if (sweep == 0) {
-        for (int o = 1; o < S; o <<= 1) m = fmaxf(m, __shfl_xor(m, o));
         if (!(m <= 3.0e38f)) m = 3.0e38f;  // an overflowed term saturates
         int e = 0;
         if (m > 0.0f) frexpf(m, &e);  // m < 2^e
         scale = ldexpf(1.0f, 40 - e);
         m = ldexpf(1.0f, e - 40);  // from here on: the inverse scale
-        const bool again = __syncthreads_or(nterm > 4) || npass > 1;  // uniform over the group
-        if (!again) {
-          if (nterm > 0) { sx += fixed_term(t0x * scale); sy += fixed_term(t0y * scale); sz += fixed_term(t0z * scale); }
-          if (nterm > 1) { sx += fixed_term(t1x * scale); sy += fixed_term(t1y * scale); sz += fixed_term(t1z * scale); }
-          if (nterm > 2) { sx += fixed_term(t2x * scale); sy += fixed_term(t2y * scale); sz += fixed_term(t2z * scale); }
-          if (nterm > 3) { sx += fixed_term(t3x * scale); sy += fixed_term(t3y * scale); sz += fixed_term(t3z * scale); }
-          break;
-        }
+#pragma unroll
+        for (int j = 0; j < kNbHead; ++j)
+          if (hon[j]) {
+            sx += fixed_term(hx[j] * scale);
+            sy += fixed_term(hy[j] * scale);
+            sz += fixed_term(hz[j] * scale);
+          }
+        if (n <= kNbHead) break;
       }
     }
-    for (int o = 1; o < S; o <<= 1) {
-      sx += __shfl_xor(sx, o);
-      sy += __shfl_xor(sy, o);
-      sz += __shfl_xor(sz, o);
+  } else {
+    // every particle listed in the tiles around this particle's tile (a particle within rc
+    // lies at most one tile away from it: tiles are 4 or more cells and the skin half a cell)
+    const int tile = tile_of(g, t, c0);
+    const int tx = tile % t.ntx, ty = (tile / t.ntx) % t.nty, tz = tile / (t.ntx * t.nty);
+    int lox, hix, loy, hiy, loz, hiz;
+    nb_range(t.ntx, lox, hix);
+    nb_range(t.nty, loy, hiy);
+    nb_range(t.ntz, loz, hiz);
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      for (int dz = loz; dz <= hiz; ++dz)
+        for (int dy = loy; dy <= hiy; ++dy)
+          for (int dx = lox; dx <= hix; ++dx) {
+            const int nt = wrapi(tx + dx, t.ntx) + t.ntx * (wrapi(ty + dy, t.nty) + t.nty * wrapi(tz + dz, t.ntz));
+            for (int j = a.set.start[nt]; j < a.set.start[nt + 1]; ++j) {
+              if (a.set.id[j] == i) continue;
+              float fx, fy, fz;
+              if (!wca_term(a, ci, di, a.set.cell[j], a.set.off[j], fx, fy, fz)) continue;
+              if (sweep == 0) {
+                m = fmaxf(m, fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz))));
+              } else {
+                sx += fixed_term(fx * scale);
+                sy += fixed_term(fy * scale);
+                sz += fixed_term(fz * scale);
+              }
+            }
+          }
+      if (sweep == 0) {
+        if (!(m <= 3.0e38f)) m = 3.0e38f;
+        int e = 0;
+        if (m > 0.0f) frexpf(m, &e);
+        scale = ldexpf(1.0f, 40 - e);
+        m = ldexpf(1.0f, e - 40);
+      }
     }
-    if (own && sl == 0) a.F[a.set.id[si]] = make_float4((float)sx * m, (float)sy * m, (float)sz * m, 0.0f);
   }
+  a.F[i] = make_float4((float)sx * m, (float)sy * m, (float)sz * m, 0.0f);
 }
-FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
+FHD_KERNEL_PAIR((), 256, k_wca_list, WcaArgs, body_wca_list(a, b))
 
 // Harmonic bonds, one lane per particle (src/forceBondedGPU.cu:21-121): for each bond partner
 // (another particle, then fixed points, each in file order) F_i += -k (1 - r0/r) (r_i - r_j),
@@ -875,9 +977,32 @@ FHD_DEV float combine_rows(float row, const AxisWeights& wz) {
 // xor-shuffle tree, then lane 0 applies the position update.
 // Position update of sorted particle s from its interpolated velocities u1 = J f1, u2 = J f2:
 // out = base + a1 u1 + a2 u2 + extra mobility + particle noise (by particle index).
-FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const float u2[3]) {
+// Per-particle values the position update needs, loaded when the pass starts so that their
+// latency hides under the gathers: the id, the base position and the neighbour-list reference.
+struct InterpPre {
+  int id;
+  int4 bc, rc;
+  float4 bo, ro;
+};
+FHD_DEV InterpPre interp_pre(const InterpArgs& a, int s) {
+  InterpPre p;
+  p.id = a.eval.id[s];
+  p.bc = p.rc = make_int4(0, 0, 0, 0);
+  p.bo = p.ro = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (a.out_cell) {
+    p.bc = a.base_cell[p.id];
+    p.bo = a.base_off[p.id];
+    if (a.nb.flag) {
+      p.rc = a.nb.ref_cell[p.id];
+      p.ro = a.nb.ref_off[p.id];
+    }
+  }
+  return p;
+}
+
+FHD_DEV void interp_finish(const InterpArgs& a, int s, const InterpPre& pre, const float u1[3], const float u2[3]) {
   const GridDev& g = a.g;
-  const int id = a.eval.id[s];
+  const int id = pre.id;
   float disp[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) disp[k] = a.a1 * u1[k] + a.a2 * u2[k];
@@ -907,8 +1032,8 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
     a.u_out[id] = make_float4(two ? u2[0] : u1[0], two ? u2[1] : u1[1], two ? u2[2] : u1[2], 0.0f);
   }
   if (a.out_cell) {
-    int4 bc = a.base_cell[id];
-    float4 bo = a.base_off[id];
+    int4 bc = pre.bc;
+    const float4 bo = pre.bo;
     bool ok = isfinite(disp[0]) && isfinite(disp[1]) && isfinite(disp[2]);
     if (!ok) {
       atomicOr(a.err, kErrNonFinite);
@@ -931,6 +1056,16 @@ FHD_DEV void interp_finish(const InterpArgs& a, int s, const float u1[3], const 
     renorm_axis(bc.z, oz, g.dz, g.idz);
     a.out_cell[id] = bc;
     a.out_off[id] = make_float4(ox, oy, oz, 0.0f);
+    if (a.nb.flag) {
+      // the WCA neighbour lists hold every pair that was within rc + skin at the build; they
+      // stay complete while no particle has moved more than half a skin from where it was then
+      const int4 rc = pre.rc;
+      const float4 ro = pre.ro;
+      const float mx = (float)min_image(wrapi(bc.x, g.mx) - rc.x, g.mx) * g.dx + (ox - ro.x);
+      const float my = (float)min_image(wrapi(bc.y, g.my) - rc.y, g.my) * g.dy + (oy - ro.y);
+      const float mz = (float)min_image(wrapi(bc.z, g.mz) - rc.z, g.mz) * g.dz + (oz - ro.z);
+      if (!(mx * mx + my * my + mz * mz <= a.nb_move2)) *a.nb.flag = 1;
+    }
     if (a.sk.s_cell) {
       // the sorted copy of the new position, in the slot of the set it replaces; the particle
       // stays listed in that slot's tile unless it left the tile's skin
@@ -968,9 +1103,11 @@ FHD_BODY void body_interp(const InterpArgs& a, const Blk& b) {
   float u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
   SrcWeights w;
   const bool act = s < a.np && q < NPTS;
+  InterpPre pre;
   if (s < a.np) {
     const int4 c = a.eval.cell[s];
     const float4 d = a.eval.off[s];
+    if (q == 0) pre = interp_pre(a, s);
     source_weights<NPTS>(c, make_float3(d.x * g.idx, d.y * g.idy, d.z * g.idz), w, false);
   }
 #pragma unroll
@@ -993,7 +1130,7 @@ FHD_BODY void body_interp(const InterpArgs& a, const Blk& b) {
     if (gid == 0 && a.sk.clear_flag) *a.sk.clear_flag = 0;
   }
   if (s >= a.np || q != 0) return;
-  interp_finish(a, s, u1, u2);
+  interp_finish(a, s, pre, u1, u2);
 }
 FHD_KERNEL_PAIR((template <int NPTS>), 256, k_interp, InterpArgs, body_interp<NPTS>(a, b))
 
@@ -1096,7 +1233,10 @@ void launch_sort(const SortArgs& a, hipStream_t s) {
   }
 }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
-  if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n), kWcaThreads, a, s);
+  if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n < kWcaBuildGroups ? a.t.n : kWcaBuildGroups), kWcaThreads, a, s);
+}
+void launch_wca_list(const WcaArgs& a, hipStream_t s) {
+  if (a.np > 0) launch(k_wca_list, k_wca_list_b, FAM_WCA_LIST, dim3(nblk(a.np, 256)), 256, a, s);
 }
 void launch_bonded(const BondArgs& a, hipStream_t s) {
   if (a.np > 0) launch(k_bonded, k_bonded_b, FAM_BONDED, dim3((a.np + kBondThreads - 1) / kBondThreads), kBondThreads, a, s);

@@ -211,6 +211,59 @@ def test_stokes_wca_forces_match():
     assert rel(fg, fo) < 1e-4
 
 
+def test_wca_list_overflow_falls_back_to_tiles():
+    """48 particles packed within 0.9 sigma of one point (each has all the others within the
+    list radius, more than the 32 a list holds) among 150 spread-out ones: the packed particles
+    take the tile scan, and every force matches the oracle."""
+    rng = np.random.default_rng(11)
+    # dt small enough that the steep forces move nothing far: the step only measures them
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=198, kernel_points=4, temperature=1e-3,
+                    dt=1e-9, seed=29, initial_positions=0, **WCA)
+    _announce("wca_list_overflow", p)
+    u = []
+    while len(u) < 48:  # no two packed particles closer than 0.3 sigma
+        c = rng.uniform(-0.9, 0.9, 3)
+        if np.linalg.norm(c) <= 0.9 and all(np.linalg.norm(c - v) >= 0.3 for v in u):
+            u.append(c)
+    u = np.array(u)
+    lattice = np.array([[x, y, z] for x in range(-7, 8, 2) for y in range(-7, 8, 2) for z in range(-7, 8, 2)],
+                       dtype=float)
+    far = lattice[np.linalg.norm(lattice - 2.5, axis=1) > 3.0][:150] + 0.1 * rng.standard_normal((150, 3))
+    pts = np.concatenate([u + 2.5, far])
+    q = pts.T.ravel()
+    g, o = F.Simulation(p), O.Oracle(p)
+    g.set_particles(q)
+    o.set_particles(q)
+    g.step(1)
+    o.step(1)
+    fg, fo = g.forces(), o.forces()
+    assert np.isfinite(fg).all() and np.abs(fo).max() > 0
+    assert rel(fg, fo) < 1e-4
+
+
+def test_wca_lists_rebuilt_as_particles_move():
+    """kT = 0.05 moves particles ~0.02 per step, so the neighbour lists (cutoff + a quarter
+    cutoff) go stale every few steps and are rebuilt; 40 steps of 400 particles at 0.1 per cell
+    (pairs form and break) match the oracle's trajectories."""
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=400, kernel_points=4, temperature=0.05,
+                    dt=0.05, seed=31, initial_positions=3, **WCA)
+    _announce("wca_list_rebuild", p)
+    _particle_parity(p, 40)
+
+
+def test_wca_lists_graph_replay_equals_eager():
+    p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=400, kernel_points=4, temperature=0.05,
+                    dt=0.05, seed=32, initial_positions=3, **WCA)
+    _announce("wca_list_graph", p)
+    a, b = F.Simulation(p), F.Simulation(p)
+    b.set_graph(True)
+    for _ in range(4):
+        a.step(5)
+        b.step(5)
+        assert np.array_equal(a.particles(), b.particles())
+        assert np.array_equal(a.forces(), b.forces())
+
+
 def test_stokes_reference_rfd_mode_runs():
     p = make_params(SCHEME_STOKES_LIMIT, cells=(16, 16, 16), np=64, kernel_points=3, nonbonded=0,
                     temperature=1e-3, dt=0.05, seed=24, initial_positions=0, rfd_mode=F.RFD_REFERENCE, **WCA)
