@@ -196,9 +196,9 @@ FHD_BODY void body_fft_z_solve(const FftArgs& a, float2* __restrict__ spec, cons
     float2 ez = ezs[z];
     float2 b0 = buf[e], b1 = buf[mz * T + e], b2 = buf[2 * mz * T + e];
     if (a.knoise) {
-      // the stochastic stress drawn in Fourier space (Stokes limit)
+      // the kept part of the stochastic stress divergence, drawn in Fourier space (Stokes limit)
       float2 nw[3];
-      kspace_stress_div(key, kx, ky, z, a.g.mx, my, mz, a.knoise, a.kn_f1, a.kn_f4, a.kn_sqrt_half_n, gx, gy, gz,
+      kspace_noise(key, kx, ky, z, a.g.mx, my, mz, a.knoise, a.kn_f4, a.kn_sqrt_half_n, gx, gy, gz,
                         ex, ey, ez, nw);
       b0 = cadd(b0, nw[0]);
       b1 = cadd(b1, nw[1]);

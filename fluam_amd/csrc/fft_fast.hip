@@ -276,7 +276,7 @@ FHD_BODY void body_fftB(const FftArgs& a, float2* __restrict__ spec, int T, cons
       float2 b0 = buf[i0], b1 = buf[i1], b2 = buf[i2];
       if (a.knoise) {
         float2 nw[3];
-        kspace_stress_div(key, kx, ky, z, a.g.mx, my, MZ, a.knoise, a.kn_f1, a.kn_f4, a.kn_sqrt_half_n, gx, gy,
+        kspace_noise(key, kx, ky, z, a.g.mx, my, MZ, a.knoise, a.kn_f4, a.kn_sqrt_half_n, gx, gy,
                           gz, ex, ey, ez, nw);
         b0 = cadd(b0, nw[0]);
         b1 = cadd(b1, nw[1]);

@@ -22,7 +22,7 @@ enum { kErrNonFinite = 1, kErrClamped = 2 };
 
 // Noise streams (counter word 1 = stream << 4 | sub).
 enum { FHD_STREAM_FLUID = 1, FHD_STREAM_FLUID_K = 2, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4,
-       FHD_STREAM_PART2 = 5, FHD_STREAM_INIT_POS = 8, FHD_STREAM_INIT_FLUID = 9 };
+       FHD_STREAM_PART2 = 5, FHD_STREAM_FLUID_P = 6, FHD_STREAM_INIT_POS = 8, FHD_STREAM_INIT_FLUID = 9 };
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products map to
@@ -91,23 +91,23 @@ FHD_DEV void draw_normals4(const Key& key, uint32_t index, uint32_t stream, uint
   n[0] = a.x; n[1] = a.y; n[2] = b.x; n[3] = b.y;
 }
 
-// ---------------------------------------------------------------------------------------
-// Stochastic stress of the Stokes limit drawn in Fourier space.  Sigma is white: the DFT of a
-// real field of N independent N(0,1) values per component has, at each mode k with -k != k,
-// real and imaginary parts N(0, N/2) independent of every other mode except -k, where it is
-// the conjugate; at self-conjugate modes it is real N(0, N).  The half spectrum kx in [0, mx/2]
-// holds one mode of every pair except on the planes kx = 0 and kx = mx/2, where (ky, kz) and
-// (-ky, -kz) are both stored: there the member with the smaller ky + my kz draws the normals
-// and the other takes the conjugate.  Draws: stream FLUID_K, counter index kx + P (ky + my kz)
-// of the drawing member, subs 3 s .. 3 s + 2 for stress set s (12 normals: the real and
-// imaginary parts of XX, XY, XZ, YY, YZ, ZZ).  The divergence of the staggered stress
-// (diagonal at cell centres, off-diagonal at edges, src/kernelConstructW.cu:148-198) in the
-// stored basis is
-//   w_x = i [f1 g_x e_x S_xx + f4 g_y e_y* S_xy + f4 g_z e_z* S_xz]   (and cyclically),
-// g_a = (2/h_a) sin(pi k_a/m_a), e_a = e^{i pi k_a/m_a}.  nset 1: Sigma_1; 2: Sigma_1 + Sigma_2.
-FHD_DEV void kspace_stress_div(const Key& key, int kx, int ky, int kz, int mx, int my, int mz, int nset,
-                               float f1, float f4, float sqrt_half_n, float gx, float gy, float gz,
-                               float2 ex, float2 ey, float2 ez, float2 w[3]) {
+// The part of the Stokes step's noise source D.Sigma that the solve keeps, drawn per mode in
+// Fourier space (the half spectrum, at mode (kx, ky, kz) with symbol g and shifts e).  The
+// solve projects with P_e = E P E* (P = I - g g^T / |g|^2); for a white stress with
+// f1^2 = 2 f4^2 (fluam's prefactors, src/createCellsStokesLimitGPU.cu:58-59) P_e D.Sigma has
+// covariance N f4^2 |g|^2 P_e per mode, so it is drawn as u = f4 |g| E (Z1 t1 + Z2 t2) with an
+// orthonormal basis t1, t2 of the plane normal to g and complex normals E|Z|^2 = N: four normals
+// per mode and stress set instead of twelve.  Basis: with k_a != 0 on all three axes,
+// t1 = (g_y, -g_x, 0)/|.| and t2 = g x t1 / |g|; on two axes a < b (c the other), t1 = e_c and
+// t2 = (-g_b e_a + g_a e_b)/|g|; on one axis a, the other two unit vectors in axis order.  On the
+// planes kx = 0 and kx = mx/2 the member of each (ky, kz), (-ky, -kz) pair with the smaller
+// ky + my kz draws (counter index kx + P (ky + my kz), stream FHD_STREAM_FLUID_P, sub = stress
+// set) and the other is its conjugate; g is even in k and the canonical mode's shift is
+// -conj(e) on an axis with k != 0.  At a self-conjugate mode Z is real (sqrt(N) n) along a
+// basis vector on the k = 0 axes and imaginary along one on the k = m/2 axes, so u is real.
+FHD_DEV void kspace_noise(const Key& key, int kx, int ky, int kz, int mx, int my, int mz, int nset, float f4,
+                          float sqrt_half_n, float gx, float gy, float gz, float2 ex, float2 ey, float2 ez,
+                          float2 w[3]) {
   int cy = ky, cz = kz;
   bool conj = false, selfc = false;
   if (kx == 0 || 2 * kx == mx) {
@@ -119,42 +119,66 @@ FHD_DEV void kspace_stress_div(const Key& key, int kx, int ky, int kz, int mx, i
       cy = py;
       cz = pz;
       conj = true;
+      if (ky != 0) ey = make_float2(-ey.x, ey.y);
+      if (kz != 0) ez = make_float2(-ez.x, ez.y);
     }
+  }
+  const bool ax = kx != 0, ay = ky != 0, az = kz != 0;
+  if (!ax && !ay && !az) {  // the mean: nothing to keep
+    w[0] = w[1] = w[2] = make_float2(0.0f, 0.0f);
+    return;
+  }
+  const float gn = sqrtf(gx * gx + gy * gy + gz * gz);
+  float t1[3] = {0.0f, 0.0f, 0.0f}, t2[3] = {0.0f, 0.0f, 0.0f};
+  bool im1 = false, im2 = false;  // at a self-conjugate mode: Z imaginary along t1 / t2
+  if (ax && ay && az) {
+    const float r = 1.0f / sqrtf(gx * gx + gy * gy);
+    t1[0] = gy * r;
+    t1[1] = -gx * r;
+    const float ig = 1.0f / gn;
+    t2[0] = (-gz * t1[1]) * ig;
+    t2[1] = (gz * t1[0]) * ig;
+    t2[2] = (gx * t1[1] - gy * t1[0]) * ig;
+    im1 = im2 = true;
+  } else if ((int)ax + (int)ay + (int)az == 2) {
+    const int c = !ax ? 0 : !ay ? 1 : 2;
+    const int a = c == 0 ? 1 : 0, b = c == 2 ? 1 : 2;
+    const float g3[3] = {gx, gy, gz};
+    const float ig = 1.0f / gn;
+    t1[c] = 1.0f;
+    t2[a] = -g3[b] * ig;
+    t2[b] = g3[a] * ig;
+    im2 = true;
+  } else {
+    const int a = ax ? 0 : ay ? 1 : 2;
+    t1[a == 0 ? 1 : 0] = 1.0f;
+    t2[a == 2 ? 1 : 2] = 1.0f;
   }
   const uint32_t index = (uint32_t)(kx + (mx / 2 + 1) * (cy + my * cz));
-  float z[12];
+  float n[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int s = 0; s < nset; ++s) {
+    float d[4];
+    draw_normals4(key, index, FHD_STREAM_FLUID_P, (uint32_t)s, d);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) z[k] = 0.0f;
-  for (int sset = 0; sset < nset; ++sset)
-#pragma unroll
-    for (int sub = 0; sub < 3; ++sub) {
-      float n[4];
-      draw_normals4(key, index, FHD_STREAM_FLUID_K, (uint32_t)(3 * sset + sub), n);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) z[4 * sub + k] += n[k];
-    }
-  // S_c = sqrt(N/2) (z_re + i z_im); self-conjugate: sqrt(N) z_re = sqrt(N/2) sqrt(2) z_re
-  float2 S[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    const float re = z[2 * c], im = z[2 * c + 1];
-    S[c] = selfc ? make_float2(1.41421356237309515f * sqrt_half_n * re, 0.0f)
-                 : make_float2(sqrt_half_n * re, (conj ? -sqrt_half_n : sqrt_half_n) * im);
+    for (int k = 0; k < 4; ++k) n[k] += d[k];
   }
-  // i * (a e) * S for a real a and phase e: products of complex numbers
-  auto term = [](float a, float2 e, float2 Sv) {
-    const float2 p = make_float2(a * (e.x * Sv.x - e.y * Sv.y), a * (e.x * Sv.y + e.y * Sv.x));
-    return make_float2(-p.y, p.x);  // times i
-  };
-  const float2 exc = make_float2(ex.x, -ex.y), eyc = make_float2(ey.x, -ey.y), ezc = make_float2(ez.x, -ez.y);
-  // S: 0 XX, 1 XY, 2 XZ, 3 YY, 4 YZ, 5 ZZ
-  float2 a, b, c;
-  a = term(f1 * gx, ex, S[0]); b = term(f4 * gy, eyc, S[1]); c = term(f4 * gz, ezc, S[2]);
-  w[0] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
-  a = term(f1 * gy, ey, S[3]); b = term(f4 * gx, exc, S[1]); c = term(f4 * gz, ezc, S[4]);
-  w[1] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
-  a = term(f1 * gz, ez, S[5]); b = term(f4 * gx, exc, S[2]); c = term(f4 * gy, eyc, S[4]);
-  w[2] = make_float2(a.x + b.x + c.x, a.y + b.y + c.y);
+  float2 Z1, Z2;
+  if (selfc) {
+    const float sn = 1.41421356237309515f * sqrt_half_n;
+    Z1 = im1 ? make_float2(0.0f, sn * n[0]) : make_float2(sn * n[0], 0.0f);
+    Z2 = im2 ? make_float2(0.0f, sn * n[2]) : make_float2(sn * n[2], 0.0f);
+  } else {
+    Z1 = make_float2(sqrt_half_n * n[0], sqrt_half_n * n[1]);
+    Z2 = make_float2(sqrt_half_n * n[2], sqrt_half_n * n[3]);
+  }
+  const float amp = f4 * gn;
+  const float2 e3[3] = {ex, ey, ez};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float2 v = make_float2(amp * (Z1.x * t1[a] + Z2.x * t2[a]), amp * (Z1.y * t1[a] + Z2.y * t2[a]));
+    const float2 u = make_float2(e3[a].x * v.x - e3[a].y * v.y, e3[a].x * v.y + e3[a].y * v.x);
+    w[a] = conj ? make_float2(u.x, -u.y) : u;
+  }
 }
 
 // ---------------------------------------------------------------------------------------

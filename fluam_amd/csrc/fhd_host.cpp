@@ -756,9 +756,10 @@ void step_incompressible(fhd_ctx* c) {
 
 // Stokes limit, midpoint: src/runSchemeStokesLimit.cu:163-276,
 // src/boundaryParticlesFunctionStokesLimit.cu:21-249.
-// W = S F (+ RFD drift) + noise D.Sigma.  The stochastic stress is drawn in Fourier space by the
-// z pass (kspace_stress_div: the transform of a white stress field, with the staggered
-// divergence applied as its Fourier symbol), so W in real space holds only the particle terms.
+// W = S F (+ RFD drift) + noise D.Sigma.  The noise is drawn in Fourier space by the z pass as
+// the part of D.Sigma that the solve keeps (kspace_noise: two complex normals per mode in the
+// plane normal to g, the law of the projected divergence of a white stress whose prefactors
+// satisfy f1^2 = 2 f4^2, as fluam's do), so W in real space holds only the particle terms.
 void fft_solve_stokes(fhd_ctx* c, int knoise, float f1, float f4) {
   FftArgs a = c->fa;
   a.stokes = 1;
@@ -778,7 +779,7 @@ void step_stokes(fhd_ctx* c) {
   const float f4 = (float)sqrt(2.0 * p.temperature * p.shearviscosity / (dV * dt));
   const float fact2 = (float)sqrt(p.temperature * p.extra_mobility * dt);
   const bool xm = p.extra_mobility != 0.0;
-  // solve 1: W = S F^n, and sqrt(2) D.Sigma_1 in Fourier space
+  // solve 1: W = S F^n, and the kept part of sqrt(2) D.Sigma_1, drawn in Fourier space
   if (c->np > 0) {
     sort_set(c, 0);
     wca(c, 0, c->F[0]);
@@ -833,7 +834,7 @@ void step_stokes(fhd_ctx* c) {
   } else {
     launch_zero_f(c->W, 3 * (size_t)c->g.N, c->s);
   }
-  // solve 2: W = S F^{n+1/2} + RFD drift, and sqrt(1/2) D.(Sigma_1 + Sigma_2) in Fourier space
+  // solve 2: W = S F^{n+1/2} + RFD drift, and the kept part of sqrt(1/2) D.(Sigma_1 + Sigma_2)
   fft_forward(c, c->W);
   fft_solve_stokes(c, 2, (float)(sqrt(0.5) * f1), (float)(sqrt(0.5) * f4));
   fft_inverse(c, c->v, nullptr, 1.0f, 0.0f);

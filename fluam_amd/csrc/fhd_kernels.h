@@ -22,9 +22,9 @@ struct FftArgs {
   int plan_zs;                    // radix plan of the z pass (128-point columns), 0 default
   int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
   int ncomp;                      // velocity components transformed (3; 1 for a single field, transform-only modes)
-  int knoise;                     // Stokes: stochastic stress drawn in Fourier space and added before the
-                                  // solve (zmode 0): 0 none, 1 Sigma_1, 2 Sigma_1 + Sigma_2
-  float kn_f1, kn_f4;             // its diagonal / off-diagonal prefactors
+  int knoise;                     // Stokes: the kept part of the stress divergence, drawn in Fourier space
+                                  // and added before the solve (zmode 0): 0 none, 1 Sigma_1, 2 Sigma_1 + Sigma_2
+  float kn_f1, kn_f4;             // the stress's diagonal / off-diagonal prefactors (f1^2 = 2 f4^2; the draw uses f4)
   float kn_sqrt_half_n;           // sqrt(N / 2)
   Key key;
   int generic;                    // 1: force the generic (runtime-size) kernels

@@ -86,7 +86,7 @@ static void uniforms4(uint64_t seed, int64_t step, int stream, int sub, uint32_t
 }
 
 enum { STREAM_FLUID1 = 1, STREAM_FLUID_K = 2, STREAM_RFD = 3, STREAM_PART1 = 4, STREAM_PART2 = 5,
-       STREAM_INIT_POS = 8, STREAM_INIT_FLUID = 9 };
+       STREAM_FLUID_P = 6, STREAM_INIT_POS = 8, STREAM_INIT_FLUID = 9 };
 
 // Stress noise of one cell and step: 12 normals n_k, k = 0..11, where n_k is word k%4 of the
 // Philox draw with sub = k/4 (stream STREAM_FLUID1).  Sigma_1 = n_0..n_5 and Sigma_2 = n_6..n_11
@@ -1158,6 +1158,123 @@ static void kspace_stress_div(orc_ctx* c, int64_t step, int nset, double f1, dou
   }
 }
 
+// The part of the stress divergence D.Sigma that the Stokes solve keeps, drawn directly.  The
+// solve applies P_e = E P E* (E = diag(e_a), P = I - g g^T / |g|^2 with the real symbol g) to
+// the source; for the stress of kspace_stress_div with f1^2 = 2 f4^2, D.Sigma has per-mode
+// covariance N f4^2 E (|g|^2 I + g g^T) E*, so P_e D.Sigma has covariance N f4^2 |g|^2 P_e: two
+// independent complex normals along an orthonormal basis t1, t2 of the plane normal to g,
+//   u = f4 |g| E (Z1 t1 + Z2 t2),  E|Z|^2 = N,
+// the same law as P_e D.Sigma with a third of the normals.  The basis (no branch on magnitudes):
+// with k_a != 0 on all three axes, t1 = (g_y, -g_x, 0)/|.| and t2 = g x t1 / |g|; on two axes
+// a < b (c the other), t1 = e_c and t2 = (-g_b e_a + g_a e_b)/|g|; on one axis a, the other two
+// unit vectors in axis order.  Modes are canonicalised as in kspace_sigma (a partner is the
+// conjugate of its canonical mode, which draws); at a self-conjugate mode a component along a
+// basis vector on the k_a = 0 axes is real (Z = sqrt(N) n) and one on the k_a = m_a/2 axes
+// imaginary (Z = i sqrt(N) n), so the field is real.  Counter: stream STREAM_FLUID_P, sub s for
+// stress set s, index kx + P (ky + my kz) of the canonical mode; generic modes take
+// Z1 = sqrt(N/2)(n0 + i n1), Z2 = sqrt(N/2)(n2 + i n3), self-conjugate ones n0 and n2.
+static void kspace_noise(orc_ctx* c, int64_t step, int nset, double f4, std::vector<double>* u,
+                         double* max_imag = nullptr) {
+  const int mx = c->mx, my = c->my, mz = c->mz, N = c->N, P = mx / 2 + 1;
+  const double h[3] = {c->dx, c->dy, c->dz};
+  const int m[3] = {mx, my, mz};
+  const double shn = sqrt(0.5 * (double)N), sn = sqrt((double)N);
+  std::vector<cplx> U[3];
+  for (int a = 0; a < 3; ++a) U[a].assign(N, cplx(0.0, 0.0));
+  const cplx I(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int kz = 0; kz < mz; ++kz)
+    for (int ky = 0; ky < my; ++ky)
+      for (int kx = 0; kx < mx; ++kx) {
+        int hx = kx, hy = ky, hz = kz;
+        bool conj = false;
+        if (2 * kx > mx) {
+          hx = mx - kx;
+          hy = (my - ky) % my;
+          hz = (mz - kz) % mz;
+          conj = true;
+        }
+        int cy = hy, cz = hz;
+        bool selfc = false;
+        if (hx == 0 || 2 * hx == mx) {
+          const int py = (my - hy) % my, pz = (mz - hz) % mz;
+          const int L = hy + my * hz, Lp = py + my * pz;
+          if (Lp == L) {
+            selfc = true;
+          } else if (Lp < L) {
+            cy = py;
+            cz = pz;
+            conj = !conj;
+          }
+        }
+        const size_t i = (size_t)kx + (size_t)mx * (ky + (size_t)my * kz);
+        const int kc[3] = {hx, cy, cz};
+        if (hx == 0 && cy == 0 && cz == 0) continue;  // the mean: nothing to keep
+        double g[3];
+        cplx e[3];
+        bool act[3];
+        for (int a = 0; a < 3; ++a) {
+          g[a] = 2.0 / h[a] * sin(M_PI * kc[a] / m[a]);
+          e[a] = std::polar(1.0, M_PI * kc[a] / m[a]);
+          act[a] = kc[a] != 0;
+        }
+        const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        double t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+        int s1 = 1, s2 = 1;  // +1: along the k_a = 0 axes, -1: along the active ones
+        const int nact = act[0] + act[1] + act[2];
+        if (nact == 3) {
+          const double r = sqrt(g[0] * g[0] + g[1] * g[1]);
+          t1[0] = g[1] / r;
+          t1[1] = -g[0] / r;
+          t2[0] = (g[1] * t1[2] - g[2] * t1[1]) / gn;
+          t2[1] = (g[2] * t1[0] - g[0] * t1[2]) / gn;
+          t2[2] = (g[0] * t1[1] - g[1] * t1[0]) / gn;
+          s1 = s2 = -1;
+        } else if (nact == 2) {
+          const int cc = !act[0] ? 0 : !act[1] ? 1 : 2;
+          const int aa = cc == 0 ? 1 : 0, bb = cc == 2 ? 1 : 2;
+          t1[cc] = 1.0;
+          t2[aa] = -g[bb] / gn;
+          t2[bb] = g[aa] / gn;
+          s2 = -1;
+        } else {
+          const int aa = act[0] ? 0 : act[1] ? 1 : 2;
+          const int b1 = aa == 0 ? 1 : 0, b2 = aa == 2 ? 1 : 2;
+          t1[b1] = 1.0;
+          t2[b2] = 1.0;
+        }
+        const uint32_t index = (uint32_t)(hx + P * (cy + my * cz));
+        double n[4] = {0, 0, 0, 0};
+        for (int st = 0; st < nset; ++st) {
+          double d[4];
+          orc_normals4(c->p.seed, step, STREAM_FLUID_P, st, index, d);
+          for (int k = 0; k < 4; ++k) n[k] += d[k];
+        }
+        cplx Z1, Z2;
+        if (selfc) {
+          Z1 = (s1 > 0 ? cplx(1.0, 0.0) : I) * (sn * n[0]);
+          Z2 = (s2 > 0 ? cplx(1.0, 0.0) : I) * (sn * n[2]);
+        } else {
+          Z1 = cplx(shn * n[0], shn * n[1]);
+          Z2 = cplx(shn * n[2], shn * n[3]);
+        }
+        for (int a = 0; a < 3; ++a) {
+          const cplx v = f4 * gn * e[a] * (Z1 * t1[a] + Z2 * t2[a]);
+          U[a][i] = conj ? std::conj(v) : v;
+        }
+      }
+  double mi = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    c->fft.run(U[a].data(), +1);
+    u[a].resize(N);
+    for (int i = 0; i < N; ++i) {
+      u[a][i] = U[a][i].real() / N;
+      mi = std::max(mi, fabs(U[a][i].imag()) / N);
+    }
+  }
+  if (max_imag) *max_imag = mi;
+}
+
 static void step_stokes(orc_ctx* c) {
   const int N = c->N, np = c->p.np;
   const double dt = c->p.dt, dV = c->dV;
@@ -1184,7 +1301,7 @@ static void step_stokes(orc_ctx* c) {
   }
   // 2. W = S F + sqrt(2) D.Sigma1 (the stress drawn in Fourier space) ; solve
   std::vector<double> nz[3];
-  kspace_stress_div(c, c->step, 1, sqrt(2.0) * fact1, sqrt(2.0) * fact4, nz);
+  kspace_noise(c, c->step, 1, sqrt(2.0) * fact4, nz);
   for (int a = 0; a < 3; ++a) {
     W[a] = f[a];
     for (int i = 0; i < N; ++i) W[a][i] += nz[a][i];
@@ -1212,7 +1329,7 @@ static void step_stokes(orc_ctx* c) {
   }
   rfd_spread(c, c->q, Wr, f);
   // 5. W = f + sqrt(1/2) D.(Sigma1 + Sigma2); solve
-  kspace_stress_div(c, c->step, 2, sqrt(0.5) * fact1, sqrt(0.5) * fact4, nz);
+  kspace_noise(c, c->step, 2, sqrt(0.5) * fact4, nz);
   for (int a = 0; a < 3; ++a) {
     W[a] = f[a];
     for (int i = 0; i < N; ++i) W[a][i] += nz[a][i];
@@ -1492,6 +1609,12 @@ extern "C" int orc_kspace_stress(orc_ctx* c, int64_t step, int nset, double f1, 
   std::vector<double> w[3];
   kspace_stress_div(c, step, nset, f1, f4, w);
   pack3(c, w, w3);
+  return 1;
+}
+extern "C" int orc_kspace_noise(orc_ctx* c, int64_t step, int nset, double f4, double* u3, double* max_imag) {
+  std::vector<double> u[3];
+  kspace_noise(c, step, nset, f4, u, max_imag);
+  pack3(c, u, u3);
   return 1;
 }
 extern "C" int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6) {

@@ -25,6 +25,9 @@
  *   orc_kspace_stress     Hermitian (real after the inverse transform), white N(0,1) real
  *                         fields, per-mode second moments, divergence symbol equal to the
  *                         pinned real-space noise divergence
+ *   orc_kspace_noise      real, discretely divergence-free, unchanged by the pinned projection;
+ *                         per-mode second moments equal those of the pinned projection of the
+ *                         pinned stress divergence (over many draws)
  *   orc_wca               analytic WCA (zero at 2^(1/6) sigma), Newton's third law
  *   orc_bonded            closed-form traps and springs, F = -grad V by finite differences of
  *                         V = sum k/2 (r - r0)^2, Newton's third law
@@ -115,6 +118,11 @@ int orc_fluid_noise(uint64_t seed, int64_t step, int which, int N, double* Z6); 
  * part of the inverse transform of Sigma_hat */
 int orc_kspace_stress(orc_ctx* c, int64_t step, int nset, double f1, double f4, double* sigma6, double* w3,
                       double* max_imag);
+/* The part of the Stokes noise source that the solve keeps, drawn directly in Fourier space:
+ * u = f4 |g| E (Z1 t1 + Z2 t2), the projected stress divergence's law (f1^2 = 2 f4^2); nset 1:
+ * set 1, nset 2: sets 1 + 2.  u3: 3 x N real-space values (IDFT / N); *max_imag: the largest
+ * imaginary part of the IDFT (zero for a Hermitian spectrum). */
+int orc_kspace_noise(orc_ctx* c, int64_t step, int nset, double f4, double* u3, double* max_imag);
 /* Initial positions used when the caller provides none (seeded uniform / simple cubic). */
 int orc_initial_positions(const fhd_params* p, double* q);
 

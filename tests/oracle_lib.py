@@ -256,6 +256,18 @@ def kspace_stress(o, step, nset, f1, f4):
     return sig, w, mi.value
 
 
+def kspace_noise(o, step, nset, f4):
+    """The Stokes step's noise source as the solve keeps it, drawn in Fourier space: (3 x N real
+    space values, largest imaginary part of the inverse transform)."""
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    L.orc_kspace_noise.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_double, dp, dp]
+    u = np.zeros(3 * o.N)
+    mi = C.c_double(0.0)
+    L.orc_kspace_noise(o.h, step, nset, f4, _p(u), C.byref(mi))
+    return u, mi.value
+
+
 def fluid_noise(seed, step, stream, N):
     Z = np.zeros(6 * N)
     _bind_terms().orc_fluid_noise(seed, step, stream, N, _p(Z))

@@ -341,20 +341,21 @@ def test_stokes_stochastic_step_composes_pinned_blocks(tmp_path):
     dV = 1.0 * 0.75 * 0.75
     f1 = np.sqrt(4 * kT * eta / (dV * dt))
     f4 = np.sqrt(2 * kT * eta / (dV * dt))
-    # the stress fields, drawn in Fourier space (pinned as white N(0,1) fields whose divergence is
-    # the pinned real-space noise divergence, tests/test_oracle_kspace_noise.py)
-    S1, _, _ = O.kspace_stress(o, 0, 1, 1.0, 1.0)
-    S12, _, _ = O.kspace_stress(o, 0, 2, 1.0, 1.0)
+    # the noise sources as the solve keeps them, drawn in Fourier space (pinned as real,
+    # divergence-free, projection-invariant fields with the law of the projected divergence of
+    # the pinned stress, tests/test_oracle_kspace_noise.py)
+    U1, _ = O.kspace_noise(o, 0, 1, np.sqrt(2) * f4)
+    U12, _ = O.kspace_noise(o, 0, 2, np.sqrt(0.5) * f4)
     w = np.array([O.normals4(p.seed, 0, 3, 0, i)[:3] for i in range(3)]).T.ravel()  # RFD directions
     delta = p.rfd_delta
 
     def vel(q, W):
         return o.interp(q, o.solve_stokes(W))
 
-    W1 = o.spread(q0, o.bonded(q0)) + O.noise_divergence(o, S1, np.sqrt(2) * f1, np.sqrt(2) * f4, False)
+    W1 = o.spread(q0, o.bonded(q0)) + U1
     qh = q0 + 0.5 * dt * vel(q0, W1)
     rfd = o.spread(q0 + 0.5 * delta * w, kT / delta * w) - o.spread(q0 - 0.5 * delta * w, kT / delta * w)
-    W2 = o.spread(qh, o.bonded(qh)) + rfd + O.noise_divergence(o, S12, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False)
+    W2 = o.spread(qh, o.bonded(qh)) + rfd + U12
     q1 = q0 + dt * vel(qh, W2)
     o.step(1)
     got = o.particles()

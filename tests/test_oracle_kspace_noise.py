@@ -82,3 +82,70 @@ def test_stress_divergence_symbol_equals_the_real_space_divergence(nset, f1, f4)
     ref = O.noise_divergence(o, sig, f1, f4, False)
     assert np.abs(w - ref).max() < 1e-12 * np.abs(ref).max()
     assert np.abs(ref).max() > 0.1
+
+
+# ---- the projected noise source, drawn directly (orc_kspace_noise) ----
+# The Stokes solve keeps only P_e D.Sigma of the noise source (P_e: the solve's projection in
+# the frame of the staggering shifts e_a).  With the step's prefactors (f1^2 = 2 f4^2) that part
+# has per-mode covariance N f4^2 |g|^2 E P E*, which two complex normals per mode reproduce.
+
+@pytest.mark.parametrize("cells,box", [((6, 4, 4), None), ((8, 6, 4), (8.0, 9.0, 3.0)), ((5, 6, 3), None),
+                                       ((7, 5, 4), (7.0, 4.0, 6.0))])
+def test_projected_noise_is_real_divergence_free_and_projected(cells, box):
+    o = _oracle(cells, box=box)
+    for nset in (1, 2):
+        u, mi = O.kspace_noise(o, 4, nset, 1.3)
+        top = np.abs(u).max()
+        assert top > 0.1
+        assert mi < 1e-13 * top                             # Hermitian spectrum
+        assert np.abs(o.divergence(u)).max() < 1e-12 * top  # discretely divergence-free
+        assert np.abs(o.project(u) - u).max() < 1e-12 * top  # the solve's projection keeps it
+
+
+def _mode_cov(fields, cells):
+    mx, my, mz = cells
+    n = mx * my * mz
+    X = np.fft.fftn(fields.reshape(3, mz, my, mx), axes=(1, 2, 3)).reshape(3, n)
+    return X[:, None] * X[None].conj()
+
+
+def test_projected_noise_has_the_law_of_the_projected_stress_divergence():
+    """Over 800 draws on a box with unequal cell sizes: the per-mode 3x3 covariance of the new
+    draw and of the pinned projection of the pinned stress divergence (f1 = sqrt(2) f4) both
+    equal N f4^2 |g|^2 E P E*, and each other, to sampling accuracy."""
+    cells, box = (6, 4, 4), (6.0, 5.0, 3.0)
+    o = _oracle(cells, box=box, seed=23)
+    mx, my, mz = cells
+    N = o.N
+    h = (box[0] / mx, box[1] / my, box[2] / mz)
+    kz, ky, kx = np.meshgrid(np.arange(mz), np.arange(my), np.arange(mx), indexing="ij")
+    k = (kx, ky, kz)
+    m = (mx, my, mz)
+    g = np.stack([2 / h[a] * np.sin(np.pi * k[a] / m[a]) for a in range(3)]).reshape(3, N)
+    e = np.exp(1j * np.pi * np.stack([k[a] / m[a] for a in range(3)])).reshape(3, N)
+    g2 = (g ** 2).sum(0)
+    safe = np.where(g2 > 0, g2, 1.0)
+    f4 = 0.8
+    C = np.zeros((3, 3, N), complex)
+    for a in range(3):
+        for b in range(3):
+            C[a, b] = N * f4 ** 2 * (g2 * (a == b) - g[a] * g[b] * (g2 > 0)) * e[a] * np.conj(e[b])
+    nd = 800
+    A = np.zeros((3, 3, N), complex)
+    B = np.zeros((3, 3, N), complex)
+    for st in range(nd):
+        u, _ = O.kspace_noise(o, st, 1, f4)
+        _, w, _ = O.kspace_stress(o, st, 1, np.sqrt(2) * f4, f4)
+        A += _mode_cov(u, cells)
+        B += _mode_cov(o.project(w), cells)
+    A /= nd
+    B /= nd
+    tr = np.real(C[0, 0] + C[1, 1] + C[2, 2])
+    live = tr > 0
+    assert not np.any(np.abs(A[:, :, ~live]) > 1e-20) and live.sum() == N - 1  # only the mean is empty
+    tol = 4.0 / np.sqrt(nd)
+    for X in (A, B):
+        dev = np.abs(X[:, :, live] - C[:, :, live]).max(axis=(0, 1)) / tr[live]
+        assert dev.max() < tol, dev.max()
+    dev = np.abs(A[:, :, live] - B[:, :, live]).max(axis=(0, 1)) / tr[live]
+    assert dev.max() < 1.5 * tol

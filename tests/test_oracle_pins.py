@@ -123,8 +123,8 @@ def _stokes_rebuild(o, p, q0, mu_e, quirk):
     dV = (p.lx / p.mx) * (p.ly / p.my) * (p.lz / p.mz)
     f1 = np.sqrt(4 * kT * eta / (dV * dt))
     f4 = np.sqrt(2 * kT * eta / (dV * dt))
-    S1, _, _ = O.kspace_stress(o, 0, 1, 1.0, 1.0)    # the stress fields (drawn in Fourier space)
-    S12, _, _ = O.kspace_stress(o, 0, 2, 1.0, 1.0)
+    U1, _ = O.kspace_noise(o, 0, 1, np.sqrt(2) * f4)  # the kept noise sources (drawn in Fourier space)
+    U12, _ = O.kspace_noise(o, 0, 2, np.sqrt(0.5) * f4)
 
     def draws(stream):
         return np.array([O.normals4(p.seed, 0, stream, 0, i)[:3] for i in range(npart)]).T.ravel()
@@ -137,7 +137,7 @@ def _stokes_rebuild(o, p, q0, mu_e, quirk):
         return o.interp(q, o.solve_stokes(W))
 
     F0 = o.bonded(q0)
-    W1 = o.spread(q0, F0) + O.noise_divergence(o, S1, np.sqrt(2) * f1, np.sqrt(2) * f4, False)
+    W1 = o.spread(q0, F0) + U1
     qh = q0 + 0.5 * dt * vel(q0, W1) + 0.5 * dt * mu_e * F0 + fact2 * zp1
     if quirk:
         rfd = (o.spread_quirk(q0, q0 + 0.5 * delta * w, w, kT / delta / dV)
@@ -145,7 +145,7 @@ def _stokes_rebuild(o, p, q0, mu_e, quirk):
     else:
         rfd = o.spread(q0 + 0.5 * delta * w, kT / delta * w) - o.spread(q0 - 0.5 * delta * w, kT / delta * w)
     Fh = o.bonded(qh)
-    W2 = o.spread(qh, Fh) + rfd + O.noise_divergence(o, S12, np.sqrt(0.5) * f1, np.sqrt(0.5) * f4, False)
+    W2 = o.spread(qh, Fh) + rfd + U12
     return q0 + dt * vel(qh, W2) + dt * mu_e * Fh + fact2 * (zp1 + zp2)
 
 
