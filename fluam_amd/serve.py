@@ -3,8 +3,8 @@
 Jobs (a parameter set and a step count) arrive over time.  The loop admits a job as soon as it
 has arrived and its device memory (fhd_bytes_needed) fits in the budget, keeps its fields,
 particles and noise counter resident in its own context (own stream, graph-replayed steps), and
-advances every admitted job by one step per round, so jobs join and leave at step boundaries
-and their kernels overlap on the GPU.  A job's result does not depend on what runs beside it:
+advances every admitted job by a few steps per round, so jobs join and leave at step boundaries
+and their kernels overlap on the GPU (or, merged, runs them as one merged-stage group).  A job's result does not depend on what runs beside it:
 every kernel is deterministic and the noise is keyed by the job's own seed and step.
 
 Reports per-job latency (arrival to completion observed by the loop), p50/p99 and the
@@ -48,12 +48,13 @@ class Job:
 
 class Server:
     """Continuous-batching loop.  budget: device bytes the jobs may hold together (default 80%
-    of the free device memory).  max_inflight: rounds a job may run ahead of its completed work,
-    which bounds the host's lead over the GPU.  keep_results: copy each job's final velocity and
-    particles back before its memory is released."""
+    of the free device memory).  round_steps: steps each running job issues per round (streams
+    mode), so that every stream holds several replays at once.  max_inflight: steps a job may
+    run ahead of its completed work, which bounds the host's lead over the GPU.  keep_results:
+    copy each job's final velocity and particles back before its memory is released."""
 
-    def __init__(self, budget: Optional[int] = None, max_inflight: int = 4, keep_results: bool = False,
-                 graphs: bool = True, merged: bool = False, chunk: int = 8):
+    def __init__(self, budget: Optional[int] = None, max_inflight: int = 8, keep_results: bool = False,
+                 graphs: bool = True, merged: bool = False, chunk: int = 8, round_steps: int = 4):
         if budget is None:
             free, _ = mem_info()
             budget = int(0.8 * free)
@@ -63,6 +64,7 @@ class Server:
         self.graphs = graphs
         self.merged = merged
         self.chunk = chunk
+        self.round_steps = max(1, round_steps)
 
     def _admit(self, j: Job, t0: float):
         j._sim = Simulation(j.params)
@@ -106,26 +108,27 @@ class Server:
                 self._admit(j, t0)
                 used += j._bytes
                 active.append(j)
-            # one round: every admitted job advances one step.  Run-ahead is bounded: once a
-            # job has issued max_inflight steps past its last marker, it waits (without blocking
-            # the others) until the GPU has reached that marker.
+            # one round: every admitted job advances up to round_steps steps.  Run-ahead is
+            # bounded: once a job has issued max_inflight steps past its last marker, it waits
+            # (without blocking the others) until the GPU has reached that marker.
             progressed = False
             for j in active:
                 if j._issued >= j.steps or j.error:
                     continue
+                k = min(self.round_steps, j.steps - j._issued)
                 try:
                     if j._issued - j._mark >= self.max_inflight:
                         if not j._sim.done():
                             continue
                         j._sim.record()
                         j._mark = j._issued
-                    j._sim.step(1)
+                    j._sim.step(k)
                 except FhdError as e:
                     # a device error flagged by this job's steps (fluam's errorKernel): the job is
                     # evicted at once; the others keep running
                     j.error = str(e)
                     continue
-                j._issued += 1
+                j._issued += k
                 progressed = True
                 if j._issued == j.steps:
                     j._sim.record()
