@@ -146,10 +146,6 @@ struct SpreadArgs {
   int nrec0, nrec_tot;  // records of group 0 and of both groups (set by launch_src_prep)
 };
 
-// The sorted copies of an updated position set are kept in place between sorts: a particle
-// stays listed in its tile while it lies within `skin` cells of the tile along every axis that
-// has three or more tiles; beyond that the pass raises *skin_flag and the set is sorted again
-// before its next use.
 // Verlet neighbour lists of the WCA pass: every pair closer than rc + list skin when the list
 // was built, kept until some particle has moved more than 0.45 list skins from where it was
 // then (interpolation passes raise `flag`; the next force pass rebuilds).
@@ -162,6 +158,10 @@ struct NbList {
   int* flag;          // 1: rebuild the lists before the next force pass
 };
 
+// The sorted copies of an updated position set are kept in place between sorts: a particle
+// stays listed in its tile while it lies within `skin` cells of the tile along every axis that
+// has three or more tiles; beyond that the pass raises *skin_flag and the set is sorted again
+// before its next use.
 struct SkinArgs {
   int4* s_cell;          // sorted copy to update (wrapped cell; w = listed tile, kept)
   float4* s_off;
