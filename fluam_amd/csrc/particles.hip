@@ -157,6 +157,18 @@ FHD_DEV int wave_incl_scan(int v) {
   return v;
 }
 
+// Largest value over the wave, in every lane (butterfly over lane-index bits).  An LDS
+// atomicMax with per-lane values would do the same, but the compiler expands it into a
+// 64-iteration scalar read-lane loop per wave, which saturates the CU's one scalar unit.
+FHD_DEV unsigned wave_max_u32(unsigned v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const unsigned t = (unsigned)__shfl_xor((int)v, o);
+    v = v > t ? v : t;
+  }
+  return v;
+}
+
 // A pair term scaled to |x| < 2^40 as a 64-bit integer; non-finite values saturate (NaN to the
 // negative bound), so the conversion is always defined.
 FHD_DEV long long fixed_term(float x) {
@@ -679,9 +691,10 @@ FHD_KERNEL_PAIR((template <int NPTS>), 256, k_src_prep, SpreadArgs, body_src_pre
 // depend on the order in which the contributions arrive.
 FHD_DEV unsigned long long fixed64(float x) {
   // Finite sources scale to |x| <= 2^30.  A non-finite source (an overflowed force) saturates:
-  // +-inf to +-2^30 and NaN to -2^30 (fmaxf returns its non-NaN operand), so the conversion is
-  // always defined; the field it lands in overflows and the interpolation pass flags the step.
-  const float c = fminf(fmaxf(x, -1073741824.0f), 1073741824.0f);
+  // +-inf to +-2^30 and NaN to -2^30 (a NaN operand makes v_med3_f32 return the smallest of the
+  // other two), so the conversion is always defined; the field it lands in overflows and the
+  // interpolation pass flags the step.  One median instruction instead of a max and a min.
+  const float c = __builtin_amdgcn_fmed3f(x, -1073741824.0f, 1073741824.0f);
   // truncation toward zero: at the 2^30 scale its bias (< 1 unit) is far below FP32 rounding
   const int q = (int)c;
   return (unsigned long long)(long long)q;
@@ -860,7 +873,9 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       if (k < kItemCap) items[k] = ref;
     }
   }
-  atomicMax(&maxbits, __float_as_uint(lmax));
+  // lmax >= 0, so its bit pattern orders as the value does
+  const unsigned wmax = wave_max_u32(__float_as_uint(lmax));
+  if (lane == 0) atomicMax(&maxbits, wmax);
   __syncthreads();
   const int ntot = nitem;
   const float M = __uint_as_float(maxbits);
