@@ -646,7 +646,11 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       rec.w[0][ax][q] = w.cen[ax].w[q];
-      rec.w[1][ax][q] = w.stg[ax].w[q] * vv[ax];
+      // an overflowed source (non-finite or beyond 3e38) is stored as +-3e38, NaN as -3e38:
+      // its tile's bound then reaches 3e38, and the spreading pass writes that tile's field as
+      // non-finite, which the interpolation pass flags
+      const float fw = w.stg[ax].w[q] * vv[ax];
+      rec.w[1][ax][q] = fabsf(fw) <= 3.0e38f ? fw : (fw > 0.0f ? 3.0e38f : -3.0e38f);
     }
   }
   rec.pad[0] = rec.pad[1] = 0;
@@ -659,8 +663,7 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(rec.w[1][ax][q]));
-  if (!(m <= 3.0e38f)) m = 3.0e38f;  // non-finite sources saturate
+    for (int q = 0; q < 4; ++q) m = fmaxf(m, fabsf(rec.w[1][ax][q]));  // <= 3e38
   // reach mask: per axis, bit e + 1 set when the stencils (nodes lo .. lo + NPTS, both
   // staggerings) reach the tile at offset e in {-1, 0, 1} from the source's sorting tile
   const int lo[3] = {min(w.cen[0].base, w.stg[0].base), min(w.cen[1].base, w.stg[1].base),
@@ -686,17 +689,14 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
 }
 FHD_KERNEL_PAIR((template <int NPTS>), 256, k_src_prep, SpreadArgs, body_src_prep<NPTS>(a, b))
 
-// Fixed-point image of one contribution (|x| <= 2^30 after scaling), as a 64-bit two's-complement
-// addend.  Integer sums are exact, so the accumulated field does not
-// depend on the order in which the contributions arrive.
+// Fixed-point image of one contribution, as a 64-bit two's-complement addend.  Integer sums are
+// exact, so the accumulated field does not depend on the order in which the contributions
+// arrive.  The records are finite with face weights of at most 3e38 < 2^128, and the tile's
+// scale 2^(30 - e) puts its largest face weight below 2^30; centred weights are at most 1, so
+// |x| <= 2^30 and the conversion is always defined without a clamp.
 FHD_DEV unsigned long long fixed64(float x) {
-  // Finite sources scale to |x| <= 2^30.  A non-finite source (an overflowed force) saturates:
-  // +-inf to +-2^30 and NaN to -2^30 (a NaN operand makes v_med3_f32 return the smallest of the
-  // other two), so the conversion is always defined; the field it lands in overflows and the
-  // interpolation pass flags the step.  One median instruction instead of a max and a min.
-  const float c = __builtin_amdgcn_fmed3f(x, -1073741824.0f, 1073741824.0f);
   // truncation toward zero: at the 2^30 scale its bias (< 1 unit) is far below FP32 rounding
-  const int q = (int)c;
+  const int q = (int)x;
   return (unsigned long long)(long long)q;
 }
 
@@ -770,6 +770,57 @@ FHD_DEV void add_source_row(unsigned long long* acc, const SrcRec* __restrict__ 
   }
 }
 
+// The narrow accumulator of 8 x 8 x 8 tiles with at most kNarrowCand candidates: 32-bit
+// fixed point, rows of kRow32 words covering x = -3 .. 10 (a halo of three columns each side,
+// so every node of a stencil row that starts in [-3, 7] has a slot and the atomics need no
+// column test; the halo is never read), planes of kPlane32 words (the 8 rows plus 1, so the
+// rows of the four lanes of one candidate start in different LDS banks).  A node's sum has at
+// most one contribution per candidate, so with contributions below 2^(30 - floor(log2 n)) for n
+// candidates it stays below 2^31.
+constexpr int kNarrowCand = 255;
+constexpr int kRow32 = kTileEdge + 6;
+constexpr int kPlane32 = kTileEdge * kRow32 + 1;
+constexpr int kComp32 = kTileEdge * kPlane32;
+
+template <int NPTS>
+FHD_DEV void add_source_row32(unsigned* acc, const SrcRec* __restrict__ R, int kz, int ox, int oy, int oz,
+                              const GridDev& g, float scale) {
+  constexpr int H = NPTS - 1;
+  const float4 cx = *reinterpret_cast<const float4*>(R->w[0][0]);
+  const float4 cy = *reinterpret_cast<const float4*>(R->w[0][1]);
+  const float4 fx = *reinterpret_cast<const float4*>(R->w[1][0]);
+  const float4 fy = *reinterpret_cast<const float4*>(R->w[1][1]);
+  const float cz = R->w[0][2][kz], fz = R->w[1][2][kz];
+  const int4 b0 = *reinterpret_cast<const int4*>(&R->base[0][0]);  // cen x, y, z, face x
+  const int2 b1 = *reinterpret_cast<const int2*>(&R->base[1][1]);  // face y, z
+#pragma unroll
+  for (int C = 0; C < 3; ++C) {
+    const int bx = C == 0 ? b0.w : b0.x, by = C == 1 ? b1.x : b0.y, bz = C == 2 ? b1.y : b0.z;
+    const float4 w4x = C == 0 ? fx : cx, w4y = C == 1 ? fy : cy;
+    const float wx[4] = {w4x.x, w4x.y, w4x.z, w4x.w}, wy[4] = {w4y.x, w4y.y, w4y.z, w4y.w};
+    const float wzs = (C == 2 ? fz : cz) * scale;
+    // first stencil node relative to the tile, folded to [-H, m - H)
+    int dx = bx - ox, dy = by - oy, dz = bz - oz;
+    dx += dx < -H ? g.mx : (dx >= g.mx - H ? -g.mx : 0);
+    dy += dy < -H ? g.my : (dy >= g.my - H ? -g.my : 0);
+    dz += dz < -H ? g.mz : (dz >= g.mz - H ? -g.mz : 0);
+    const int lz = dz + kz;
+    // this row misses the tile, or its columns do (then dx lies outside [-H, 7])
+    if ((unsigned)lz >= (unsigned)kTileEdge || dx > kTileEdge - 1) continue;
+    // word index of the stencil's first node; the stencil's rows and columns are constant
+    // offsets from it
+    const int first = C * kComp32 + lz * kPlane32 + dy * kRow32 + dx + 3;
+#pragma unroll
+    for (int ky = 0; ky < NPTS; ++ky) {
+      if ((unsigned)(dy + ky) >= (unsigned)kTileEdge) continue;
+      const float wyz = wy[ky] * wzs;
+#pragma unroll
+      for (int kx = 0; kx < NPTS; ++kx)
+        atomicAdd(&acc[first + ky * kRow32 + kx], (unsigned)(int)(wx[kx] * wyz));
+    }
+  }
+}
+
 FHD_DEV const SrcRec* rec_of(const SpreadArgs& a, int ref) {
   return a.grp[ref >> 30].rec + (ref & ((1 << 30) - 1));
 }
@@ -784,6 +835,7 @@ template <int NPTS, bool FAST>
 FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   constexpr int ZS = kTileEdge;
   constexpr int NACC = 3 * ZS * kRowStride;
+  static_assert(3 * kComp32 <= 2 * NACC, "the narrow accumulator shares the 64-bit one's LDS");
   constexpr int LPC = 4;  // lanes per candidate (one stencil z-row each)
   __shared__ unsigned long long acc[NACC];
   __shared__ int nb_start[64], nb_pref[65], nb_sel[64];
@@ -882,10 +934,20 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   int e2 = 0;
   if (M > 0.0f) frexpf(M, &e2);  // M < 2^e2
   e2 = max(e2, -90);
-  const float scale = ldexpf(1.0f, 30 - e2);
-  const float inv = ldexpf(1.0f, e2 - 30) * a.out_coef;
+  // narrow tiles: contributions below 2^S with S = 30 - floor(log2 ntot); others 2^30 in 64 bits
+  const bool narrow = FAST && ntot <= kNarrowCand;
+  const int S = narrow ? 30 - (31 - __clz(ntot > 0 ? ntot : 1)) : 30;
+  const float scale = ldexpf(1.0f, S - e2);
+  // a bound of 3e38 means an overflowed source among the candidates: the tile's field is
+  // written non-finite (every node +-inf or NaN), so the step is flagged
+  const float inv = M >= 3.0e38f ? __builtin_inff() : ldexpf(1.0f, e2 - S) * a.out_coef;
   const int kz = tid % LPC;
-  if (ntot <= kItemCap) {
+  unsigned* const acc32 = reinterpret_cast<unsigned*>(acc);
+  if (narrow) {
+    if (kz < NPTS)
+      for (int c = tid / LPC; c < ntot; c += kSpreadThreads / LPC)
+        add_source_row32<NPTS>(acc32, rec_of(a, items[c]), kz, ox, oy, oz, g, scale);
+  } else if (ntot <= kItemCap) {
     if (kz < NPTS)
       for (int c = tid / LPC; c < ntot; c += kSpreadThreads / LPC)
         add_source_row<NPTS, FAST>(acc, rec_of(a, items[c]), kz, ox, oy, oz, g, t, scale);
@@ -914,9 +976,10 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
     for (int k = 0; k < kOutPer; ++k) {
       const int e = tid + k * kSpreadThreads;
       const int C = e >> 9, lx = e & 7, ly = (e >> 3) & 7, lz = (e >> 6) & 7;
-      const int ai = (C * ZS + lz) * kRowStride + ly * kTileEdge + lx;
       const size_t gi = (size_t)C * g.N + (ox + lx) + (size_t)g.mx * ((oy + ly) + (size_t)g.my * (oz + lz));
-      const float val = (float)(long long)acc[ai] * inv;
+      const float q = narrow ? (float)(int)acc32[C * kComp32 + lz * kPlane32 + ly * kRow32 + lx + 3]
+                             : (float)(long long)acc[(C * ZS + lz) * kRowStride + ly * kTileEdge + lx];
+      const float val = q * inv;
       a.out[gi] = a.out_add ? pre[k] + val : val;
     }
     return;
