@@ -187,7 +187,7 @@ struct fhd_ctx {
   int4* scell[2] = {nullptr, nullptr};
   float4* soff[2] = {nullptr, nullptr};
   SrcRec* rec[2] = {nullptr, nullptr};   // spreading source records per group (2 np each)
-  int4* cull[2] = {nullptr, nullptr};
+  int* cull[2] = {nullptr, nullptr};
   int* errd = nullptr;
   // harmonic bonds (bondedForces); null when the run has none
   int *bpp_start = nullptr, *bfp_start = nullptr;
@@ -1784,7 +1784,7 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
         c->scell[k] = dalloc<int4>(c->np);
         c->soff[k] = dalloc<float4>(c->np);
         c->rec[k] = dalloc<SrcRec>(2 * (size_t)c->np);
-        c->cull[k] = dalloc<int4>(2 * (size_t)c->np);
+        c->cull[k] = dalloc<int>(2 * (size_t)c->np);
         HIPCHK(hipMemset(c->F[k], 0, sizeof(float4) * c->np));
       }
       c->u = dalloc<float4>(c->np);
@@ -2392,7 +2392,7 @@ long long ctx_bytes(const fhd_params& p) {
     for (int a = 0; a < 3; ++a) nt *= te[a] > 0 ? m[a] / te[a] : 1;
     b += 2 * np * (16 + 16 + 16 + 4 + 16 + 16) + 2 * (nt + 1) * 4;  // positions, forces, sorted sets
     b += np * 16 + nt * 4 + 2 * np * 4;                               // u, counts, slots, scratch
-    b += 2 * 2 * np * ((long long)sizeof(SrcRec) + 16);               // spreading records
+    b += 2 * 2 * np * ((long long)sizeof(SrcRec) + 4);                // spreading records
     if (p.scheme == FHD_SCHEME_QUASI_NEUTRALLY_BUOYANT && p.mass != 0.0) b += N3 * 4 + 5 * np * 16;
   }
   return b;

@@ -130,7 +130,8 @@ struct SpreadGroup {
   float disp;           // SRC_RFD: displacement magnitude delta/2
   int rfd_quirk;        // SRC_RFD: weights at the undisplaced position
   SrcRec* rec;          // nsrc x (1 or 2) records, in sorted order
-  int4* cull;           // per record: x = tile reach mask (3 bits per axis), w = bound on one contribution (float bits)
+  int* cull;            // per record: bits 0-8 tile reach mask (3 bits per axis), bits 16-24 e + 192 where
+                        // the record's largest face weight x value is below 2^e
 };
 
 struct SpreadArgs {

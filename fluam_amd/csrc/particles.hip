@@ -610,7 +610,7 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
     if (val.x == 0.0f && val.y == 0.0f && val.z == 0.0f) {
       // a zero source (e.g. a particle with no WCA neighbour) adds exactly zero to every node:
       // an empty reach mask makes every tile cull it, and its record is never read
-      G.cull[r] = make_int4(0, 0, 0, 0);
+      G.cull[r] = 0;
       return;
     }
   } else if (G.mode == SRC_DRIFT) {
@@ -647,7 +647,7 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
     for (int q = 0; q < 4; ++q) {
       rec.w[0][ax][q] = w.cen[ax].w[q];
       // an overflowed source (non-finite or beyond 3e38) is stored as +-3e38, NaN as -3e38:
-      // its tile's bound then reaches 3e38, and the spreading pass writes that tile's field as
+      // its tile's bound then reaches 2^127, and the spreading pass writes that tile's field as
       // non-finite, which the interpolation pass flags
       const float fw = w.stg[ax].w[q] * vv[ax];
       rec.w[1][ax][q] = fabsf(fw) <= 3.0e38f ? fw : (fw > 0.0f ? 3.0e38f : -3.0e38f);
@@ -685,7 +685,11 @@ FHD_BODY void body_src_prep(const SpreadArgs& a, const Blk& b) {
       if (reaches<NPTS>(lo[ax], tt * T[ax], T[ax], MM[ax], NT[ax])) mask |= 1 << (3 * ax + e + 1);
     }
   }
-  G.cull[r] = make_int4(mask, 0, 0, (int)__float_as_uint(m));
+  // the bound enters the spreading pass only through its binary exponent: m < 2^e (e = -191
+  // for m = 0, below any other record's), stored biased by 192 in bits 16-24
+  int e = -191;
+  if (m > 0.0f) frexpf(m, &e);
+  G.cull[r] = mask | ((e + 192) << 16);
 }
 FHD_KERNEL_PAIR((template <int NPTS>), 256, k_src_prep, SpreadArgs, body_src_prep<NPTS>(a, b))
 
@@ -893,7 +897,7 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
   __syncthreads();
   const int nall = nb_pref[ntab];
   // record index e -> reference; -1 when the record's stencils miss the tile
-  auto cull = [&](int e, float& mx) -> int {
+  auto cull = [&](int e, int& mx) -> int {
     // largest lo with nb_pref[lo] <= e: fixed-step search (no divergent loop); entries past
     // ntab hold nall, which no record index reaches
     int lo = 0;
@@ -902,15 +906,15 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       if (nb_pref[lo + step] <= e) lo += step;
     const int gi = lo >= nnb;
     const int ri = nb_start[lo] + (e - nb_pref[lo]);
-    const int4 cu = a.grp[gi].cull[ri];
+    const int cu = a.grp[gi].cull[ri];
     const int sel = nb_sel[lo];
-    if ((cu.x & sel) != sel) return -1;
-    mx = fmaxf(mx, __int_as_float(cu.w));
+    if ((cu & sel) != sel) return -1;
+    mx = max(mx, cu >> 16);
     return (gi << 30) | ri;
   };
   // pass 1: cull every record, keep the first kItemCap candidates, largest magnitude; each wave
   // appends its survivors with one LDS atomic
-  float lmax = 0.0f;
+  int lmax = 0;  // largest biased exponent of the candidates' bounds
   const int lane = tid & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
   for (int e0 = 0; e0 < nall; e0 += kSpreadThreads) {
@@ -925,22 +929,20 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       if (k < kItemCap) items[k] = ref;
     }
   }
-  // lmax >= 0, so its bit pattern orders as the value does
-  const unsigned wmax = wave_max_u32(__float_as_uint(lmax));
+  const unsigned wmax = wave_max_u32((unsigned)lmax);
   if (lane == 0) atomicMax(&maxbits, wmax);
   __syncthreads();
   const int ntot = nitem;
-  const float M = __uint_as_float(maxbits);
-  int e2 = 0;
-  if (M > 0.0f) frexpf(M, &e2);  // M < 2^e2
-  e2 = max(e2, -90);
+  // every candidate's largest face weight x value is below 2^e2
+  const int e2 = max((int)maxbits - 192, -90);
   // narrow tiles: contributions below 2^S with S = 30 - floor(log2 ntot); others 2^30 in 64 bits
   const bool narrow = FAST && ntot <= kNarrowCand;
   const int S = narrow ? 30 - (31 - __clz(ntot > 0 ? ntot : 1)) : 30;
   const float scale = ldexpf(1.0f, S - e2);
-  // a bound of 3e38 means an overflowed source among the candidates: the tile's field is
-  // written non-finite (every node +-inf or NaN), so the step is flagged
-  const float inv = M >= 3.0e38f ? __builtin_inff() : ldexpf(1.0f, e2 - S) * a.out_coef;
+  // a bound of 2^127 or more means an overflowed source among the candidates (the record pass
+  // stores those as +-3e38): the tile's field is written non-finite (every node +-inf or NaN),
+  // so the step is flagged
+  const float inv = e2 >= 128 ? __builtin_inff() : ldexpf(1.0f, e2 - S) * a.out_coef;
   const int kz = tid % LPC;
   unsigned* const acc32 = reinterpret_cast<unsigned*>(acc);
   if (narrow) {
@@ -957,7 +959,7 @@ FHD_BODY void body_spread(const SpreadArgs& a, const Blk& b) {
       __syncthreads();
       if (tid == 0) nitem = 0;
       __syncthreads();
-      float dummy = 0.0f;
+      int dummy = 0;
       for (int e = b0 + tid; e < min(nall, b0 + kItemCap); e += kSpreadThreads) {
         const int ref = cull(e, dummy);
         if (ref >= 0) items[atomicAdd(&nitem, 1)] = ref;
