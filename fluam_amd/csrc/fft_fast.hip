@@ -12,22 +12,26 @@
 
 namespace fhd {
 
-// One radix-R Stockham stage, lines l = 0..lines-1 at base l * LS, element stride ES.
+// One radix-R Stockham stage, lines l = 0..lines-1 (lines <= MAXL) at base l * LS, element
+// stride ES.  Consecutive lanes take butterfly j of consecutive lines: the lines are either
+// adjacent elements (ES > 1, LS = 1) or an odd number of elements apart (ES = 1), so the lanes
+// of a wave read and write different LDS banks in every stage, and share one twiddle.  (Taking
+// consecutive butterflies of one line instead makes the writes of the early stages R-way bank
+// conflicts.)
 template <int N, int R, int NS, int ES, int NT, int MAXL, int TWM>
 FHD_DEV void fstage(float2* buf, int lines, int LS, const float2* tw, float s) {
   constexpr int NBL = N / R;
   constexpr int KMAX = (MAXL * NBL + NT - 1) / NT;
   constexpr int STRIDE = N / (NS * R);
-  const int total = lines * NBL;
   float2 v[KMAX][R];
   int wb[KMAX];
 #pragma unroll
   for (int k = 0; k < KMAX; ++k) {
     const int b = threadIdx.x + k * NT;
+    const int j = b / MAXL;
+    const int l = b - j * MAXL;
     wb[k] = -1;
-    if (b < total) {
-      const int l = b / NBL;
-      const int j = b - l * NBL;
+    if (j < NBL && l < lines) {
       const int base = l * LS;
       const int kk = j % NS;
 #pragma unroll
