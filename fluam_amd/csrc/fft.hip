@@ -29,7 +29,9 @@ struct LineMap {
 // One Stockham radix-R stage over all lines (Govindaraju et al., SC'08 formulation):
 // butterfly j of a line reads x[j + r n/R], twiddles by w^{(j mod Ns) r}, w = e^{s 2 pi i/(Ns R)},
 // and writes to (j / Ns) Ns R + (j mod Ns) + r Ns.  In place: every thread reads its
-// butterflies into registers, the workgroup synchronises, then writes.
+// butterflies into registers, the workgroup synchronises, then writes.  Consecutive lanes take
+// butterfly j of consecutive lines, which keeps the early stages' strided writes out of each
+// other's LDS banks (consecutive butterflies of one line would collide R ways).
 template <int R, int KMAX>
 FHD_DEV void stockham_stage(float2* buf, const LineMap& lm, int n, int Ns, const float2* tw, int twmul,
                             float s) {
@@ -43,8 +45,8 @@ FHD_DEV void stockham_stage(float2* buf, const LineMap& lm, int n, int Ns, const
     int b = threadIdx.x + k * blockDim.x;
     wbase[k] = -1;
     if (b < total) {
-      int l = b / nbl;
-      int j = b - l * nbl;
+      int j = b / lm.lines;
+      int l = b - j * lm.lines;
       int base = lm.base(l);
       int kk = j % Ns;
 #pragma unroll
