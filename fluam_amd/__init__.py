@@ -288,12 +288,13 @@ class Simulation:
         return ms.value
 
     def schedule(self):
-        """The autotuned schedule: z-pass column tile, xy and z radix plans, W-kernel z chunks."""
-        out = (C.c_int * 5)()
+        """The autotuned schedule: z-pass column tile, xy (forward, inverse) and z radix plans,
+        W-kernel z chunks."""
+        out = (C.c_int * 6)()
         if not lib().fhd_get_schedule(self.h, out):
             raise FhdError(_err(self.h))
         return {"z_column_tile": out[0], "xy_radix_plan": out[1], "z_radix_plan": out[2],
-                "w_z_chunk": out[3], "w_z_chunk_batched": out[4]}
+                "w_z_chunk": out[3], "w_z_chunk_batched": out[4], "xy_radix_plan_inverse": out[5]}
 
     def profile(self, n, nkern=32):
         kms = np.zeros(2 * nkern)

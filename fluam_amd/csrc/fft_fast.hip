@@ -342,12 +342,13 @@ static void launch_AC_plan(const FftPass& x, int which, int mz, hipStream_t s) {
     launch(k_fftC<MX, MY, NT, PX, PY>, k_fftC_b<MX, MY, NT, PX, PY>, FAM_FFT_XY_INV, dim3(mz, x.f.ncomp), NT, x, s);
 }
 // The radix plans of the x and y transforms (plan_choice indices, px + 3 py) come from the
-// schedule autotuner for 96 x 96 and 128 x 128 planes (128 x 128 also has 9: the
-// register-transform kernels of fft128.hip); other planes use the default plan.
+// schedule autotuner for 96 x 96 and 128 x 128 planes, one for the forward and one for the
+// inverse pass (128 x 128 also has 9: the register-transform kernels of fft128.hip); other
+// planes use the default plan.
 template <int MX, int MY, int NT>
 static void launch_AC_nt(const FftPass& x, int which, int mz, hipStream_t s) {
   if constexpr ((MX == 128 && MY == 128) || (MX == 96 && MY == 96)) {
-    switch (x.f.plan_xy) {
+    switch (which == 0 ? x.f.plan_xy : x.f.plan_xy_inv) {
       case 1: launch_AC_plan<MX, MY, NT, 1, 0>(x, which, mz, s); return;
       case 2: launch_AC_plan<MX, MY, NT, 2, 0>(x, which, mz, s); return;
       case 3: launch_AC_plan<MX, MY, NT, 0, 1>(x, which, mz, s); return;

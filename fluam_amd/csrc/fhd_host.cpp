@@ -340,6 +340,7 @@ void build_tables(fhd_ctx* c) {
   a.ztile = std::max(1, std::min(16, 8448 / (3 * mz)));
   a.btile = 8;
   a.plan_xy = 0;
+  a.plan_xy_inv = 0;
   a.plan_zs = 0;
   a.zmode = 0;
   a.ncomp = 3;
@@ -440,7 +441,7 @@ FluidWArgs base_w(fhd_ctx* c) {
 // process: a replica's schedule depends on its shape only, never on what else runs beside it,
 // and neither choice changes any result bit.
 struct Sched {
-  int btile, zchunk, zchunk_b, plan_xy, plan_zs;
+  int btile, zchunk, zchunk_b, plan_xy, plan_xy_inv, plan_zs;
 };
 std::mutex g_sched_mu;
 std::map<std::array<int, 4>, Sched> g_sched;
@@ -474,7 +475,7 @@ void apply_schedule(fhd_ctx* c) {
   const bool tune = !(env && env[0] == '0');
   std::lock_guard<std::mutex> lock(g_sched_mu);
   auto it = g_sched.find(key);
-  Sched sc = {8, 8, 8, 0, 0};
+  Sched sc = {8, 8, 8, 0, 0, 0};
   if (it != g_sched.end()) {
     sc = it->second;
   } else if (tune) {
@@ -503,19 +504,21 @@ void apply_schedule(fhd_ctx* c) {
     c->fa.plan_zs = sc.plan_zs;
     // xy passes of 96 x 96 and 128 x 128 planes: the radix plans of the x and y transforms
     // (three each) of the LDS-stage kernels, and for 128 x 128 the register-transform kernels
-    // (plan 9), timed as a forward and an inverse pass
+    // (plan 9), chosen for the forward and the inverse pass separately
     const bool sq128 = c->p.mx == 128 && c->p.my == 128, sq96 = c->p.mx == 96 && c->p.my == 96;
     if (sq128 || sq96) {
       best = 1e30f;
+      float best_inv = 1e30f;
       for (int pxy = 0; pxy < (sq128 ? 10 : 9); ++pxy) {  // 9: the register-transform kernels
-        c->fa.plan_xy = pxy;
-        const float t = time_launches(c, [&] {
-          launch_fft_xy_fwd(c->fa, c->W, c->spec, c->s);
-          launch_fft_xy_inv(c->fa, c->spec, c->W, nullptr, 1.0f, 0.0f, c->s);
-        }, 2);
-        if (t < best) { best = t; sc.plan_xy = pxy; }
+        c->fa.plan_xy = c->fa.plan_xy_inv = pxy;
+        const float tf = time_launches(c, [&] { launch_fft_xy_fwd(c->fa, c->W, c->spec, c->s); }, 4);
+        const float ti =
+            time_launches(c, [&] { launch_fft_xy_inv(c->fa, c->spec, c->W, nullptr, 1.0f, 0.0f, c->s); }, 4);
+        if (tf < best) { best = tf; sc.plan_xy = pxy; }
+        if (ti < best_inv) { best_inv = ti; sc.plan_xy_inv = pxy; }
       }
       c->fa.plan_xy = sc.plan_xy;
+      c->fa.plan_xy_inv = sc.plan_xy_inv;
     }
     // W kernel (incompressible and QNB; the Stokes step has none): 2 planes per workgroup fill
     // the chip on small grids (64^3: 512 workgroups) when the context launches alone; a
@@ -542,6 +545,7 @@ void apply_schedule(fhd_ctx* c) {
   }
   c->fa.btile = sc.btile;
   c->fa.plan_xy = sc.plan_xy;
+  c->fa.plan_xy_inv = sc.plan_xy_inv;
   c->fa.plan_zs = sc.plan_zs;
   c->zchunk = sc.zchunk;
   c->zchunk_b = sc.zchunk_b;
@@ -550,7 +554,7 @@ void apply_schedule(fhd_ctx* c) {
   if (const char* e = getenv("FHD_PLAN_XY")) {
     const int v = atoi(e);
     if (v < 0 || v > 9) throw std::runtime_error("FHD_PLAN_XY must be 0-9");
-    c->fa.plan_xy = v;
+    c->fa.plan_xy = c->fa.plan_xy_inv = v;
   }
   if (const char* e = getenv("FHD_PLAN_Z")) {
     const int v = atoi(e);
@@ -2157,13 +2161,14 @@ int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_name
   return nout;
 }
 
-int fhd_get_schedule(const fhd_ctx* c, int out[5]) {
+int fhd_get_schedule(const fhd_ctx* c, int out[6]) {
   if (!c || !out) return 0;
   out[0] = c->fa.btile;
   out[1] = c->fa.plan_xy;
   out[2] = c->fa.plan_zs;
   out[3] = c->zchunk;
   out[4] = c->zchunk_b;
+  out[5] = c->fa.plan_xy_inv;
   return 1;
 }
 

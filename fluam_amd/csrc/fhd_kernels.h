@@ -18,7 +18,8 @@ struct FftArgs {
   int nst_h, nst_y, nst_z;
   int ztile;                      // kx columns per pass-B workgroup (generic kernels)
   int btile;                      // kx columns per pass-B workgroup (size-specialised: 2, 4, 8 or 16)
-  int plan_xy;                    // radix plans of the xy passes (128 x 128 planes): px + 3 py, 0 default
+  int plan_xy;                    // radix plans of the forward xy pass (96 x 96, 128 x 128 planes): px + 3 py, 0 default
+  int plan_xy_inv;                // the same for the inverse xy pass
   int plan_zs;                    // radix plan of the z pass (128-point columns), 0 default
   int zmode;                      // 0: FFT_z, operator, inverse FFT_z; 1: FFT_z only; 2: inverse only
   int ncomp;                      // velocity components transformed (3; 1 for a single field, transform-only modes)

@@ -163,12 +163,13 @@ int fhd_time_steps(fhd_ctx* c, int nsteps, double* ms);
 int fhd_profile_kernels(fhd_ctx* c, int nsteps, double* kern_ms, char* kern_names, int nkern);
 
 /* The schedule the context's autotuner chose for its grid shape (no fluam counterpart): out[0] the
- * z-pass column tile, out[1] the xy passes' radix plan (x plan + 3 y plan, 0-8, for 128 x 128
+ * z-pass column tile, out[1] the forward xy pass's radix plan (x plan + 3 y plan, 0-8, for 128 x 128
  * planes: x 16 4 | 4 16 | 8 8, y 16 8 | 8 16 | 4 4 8, and 9: the transforms in registers; for
  * 96 x 96 planes: x 16 3 | 3 16 | 4 4 3, y 16 2 3 | 8 4 3 | 3 8 4; 0 elsewhere), out[2] the z
  * pass's radix plan (0-2 for 48-, 96- and 128-point columns, the same candidates as y), out[3]
- * and out[4] the W kernel's z chunk alone and in batched dispatches. */
-int fhd_get_schedule(const fhd_ctx* c, int out[5]);
+ * and out[4] the W kernel's z chunk alone and in batched dispatches, out[5] the inverse xy pass's
+ * radix plan (the same candidates as out[1]). */
+int fhd_get_schedule(const fhd_ctx* c, int out[6]);
 
 /* Divergence diagnostics of the current velocity: max |D v| over cells, in units of
  * 1/time (discrete staggered divergence, src/kernelUpdateVIncompressible.cu:21-72 invariant). */
