@@ -1044,12 +1044,15 @@ FHD_DEV float interp_row(const float* __restrict__ f, const GridDev& g, const Ax
   return sy;  // weighted by wz.w[kz] when the rows are combined (static indices, no scratch)
 }
 
-// Combine the four z-rows held by lanes 4p .. 4p+3 into lane 4p: sum_k wz[k] row_k.  Fixed
-// order, so the result does not depend on scheduling.
+// Combine the four z-rows held by lanes 4p .. 4p+3 (every lane gets the sum): sum_k wz[k] row_k
+// in a fixed order, so the result does not depend on scheduling.  The four lanes of a particle
+// are a DPP quad, so the rows are broadcast by quad_perm moves (plain VALU; no LDS round trip).
+template <int K>
+FHD_DEV float quad_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
 FHD_DEV float combine_rows(float row, const AxisWeights& wz) {
-  const int base = threadIdx.x & ~3;
-  const float r0 = __shfl(row, base), r1 = __shfl(row, base + 1), r2 = __shfl(row, base + 2),
-              r3 = __shfl(row, base + 3);
+  const float r0 = quad_lane<0>(row), r1 = quad_lane<1>(row), r2 = quad_lane<2>(row), r3 = quad_lane<3>(row);
   return ((wz.w[0] * r0 + wz.w[1] * r1) + wz.w[2] * r2) + wz.w[3] * r3;
 }
 
@@ -1199,7 +1202,7 @@ FHD_BODY void body_interp(const InterpArgs& a, const Blk& b) {
     if (act && a.f1) r1 = interp_row<NPTS>(a.f1 + (size_t)comp * g.N, g, wx, wy, wz, q);
     if (act && a.f2) r2 = interp_row<NPTS>(a.f2 + (size_t)comp * g.N, g, wx, wy, wz, q);
     u1[comp] = combine_rows(r1, wz);
-    u2[comp] = combine_rows(r2, wz);
+    u2[comp] = a.f2 ? combine_rows(r2, wz) : 0.0f;
   }
   if (a.sk.s_cell) {
     // bookkeeping of the kept sort: the tile starts of the set this pass fills in, and the flag
