@@ -50,11 +50,14 @@ class Server:
     """Continuous-batching loop.  budget: device bytes the jobs may hold together (default 80%
     of the free device memory).  round_steps: steps each running job issues per round (streams
     mode), so that every stream holds several replays at once.  max_inflight: steps a job may
-    run ahead of its completed work, which bounds the host's lead over the GPU.  keep_results:
-    copy each job's final velocity and particles back before its memory is released."""
+    run ahead of its completed work, which bounds the host's lead over the GPU.  max_active: most
+    jobs admitted at once (None: as many as the memory budget holds); jobs large enough to fill
+    the GPU alone run faster one after another than side by side.  keep_results: copy each job's
+    final velocity and particles back before its memory is released."""
 
     def __init__(self, budget: Optional[int] = None, max_inflight: int = 8, keep_results: bool = False,
-                 graphs: bool = True, merged: bool = False, chunk: int = 8, round_steps: int = 4):
+                 graphs: bool = True, merged: bool = False, chunk: int = 8, round_steps: int = 4,
+                 max_active: Optional[int] = None):
         if budget is None:
             free, _ = mem_info()
             budget = int(0.8 * free)
@@ -65,6 +68,12 @@ class Server:
         self.merged = merged
         self.chunk = chunk
         self.round_steps = max(1, round_steps)
+        self.max_active = max_active
+
+    def _room(self, active, used, j: Job) -> bool:
+        if self.max_active is not None and len(active) >= self.max_active:
+            return False
+        return used + j._bytes <= self.budget
 
     def _admit(self, j: Job, t0: float):
         j._sim = Simulation(j.params)
@@ -103,7 +112,7 @@ class Server:
         while pending or active:
             now = time.perf_counter() - t0
             # admission: arrived jobs, in arrival order, while memory allows
-            while pending and pending[0].arrival <= now and used + pending[0]._bytes <= self.budget:
+            while pending and pending[0].arrival <= now and self._room(active, used, pending[0]):
                 j = pending.pop(0)
                 self._admit(j, t0)
                 used += j._bytes
@@ -178,7 +187,7 @@ class Server:
         while pending or active:
             now = time.perf_counter() - t0
             changed = False
-            while pending and pending[0].arrival <= now and used + pending[0]._bytes <= self.budget:
+            while pending and pending[0].arrival <= now and self._room(active, used, pending[0]):
                 j = pending.pop(0)
                 self._admit(j, t0)
                 if j.params.scheme == SCHEME_QUASI_NEUTRALLY_BUOYANT and j.params.np > 0:
