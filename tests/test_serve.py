@@ -91,6 +91,19 @@ def test_budget_limits_admission(fake):
     assert order[2].admitted >= first_done - 1e-6
 
 
+@pytest.mark.parametrize("merged", [False, True])
+def test_max_active_caps_running_jobs(fake, monkeypatch, merged):
+    # the memory budget would hold all six; the cap admits three at a time, the rest as they leave
+    if merged:
+        monkeypatch.setattr(S, "Group", FakeGroup)
+    jobs = [_job(f"j{i}", 16, 20 + 3 * i, 0.0) for i in range(6)]
+    r = S.Server(budget=10 ** 9, max_active=3, merged=merged).run(jobs)
+    assert r["jobs"] == 6
+    assert fake.peak == 3
+    order = sorted(jobs, key=lambda j: j.admitted)
+    assert order[3].admitted >= min(j.finished for j in order[:3]) - 1e-6
+
+
 def test_job_larger_than_budget_is_rejected(fake):
     with pytest.raises(ValueError):
         S.Server(budget=100).run([_job("big", 16, 5, 0.0)])
