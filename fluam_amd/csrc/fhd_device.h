@@ -18,7 +18,7 @@ FHD_DEV int xcd_item(int b, int n) {
 // Device error flags (bits of the context's error word; cf. fluam's errorKernel,
 // src/allocateErrorArray.cu:22-32): a non-finite particle displacement, and a displacement of
 // 2^24 cells or more that was not applied.
-enum { kErrNonFinite = 1, kErrClamped = 2 };
+enum { kErrNonFinite = 1, kErrClamped = 2, kErrBarrier = 4 };
 
 // Noise streams (counter word 1 = stream << 4 | sub).
 enum { FHD_STREAM_FLUID = 1, FHD_STREAM_FLUID_K = 2, FHD_STREAM_RFD = 3, FHD_STREAM_PART1 = 4,

@@ -171,6 +171,7 @@ struct fhd_ctx {
   int *count = nullptr, *slot = nullptr, *tmp = nullptr;  // count: 2 x ntiles (one per position set)
   int* slot_of = nullptr;  // sorted slot of each particle in set 0
   int* sflag = nullptr;    // [k]: position set k must be sorted before its next use
+  unsigned long long* sbar = nullptr;  // grid-barrier counter of the one-launch sort
   // inertial coupling with excess mass: a field of work space and per-particle vectors by id
   float* M1 = nullptr;     // the projected correction dv~, then 1/2 dt nu L dv
   float4 *dulag = nullptr, *du = nullptr, *dp = nullptr, *jvt = nullptr, *ms = nullptr;
@@ -363,7 +364,7 @@ void free_ctx(fhd_ctx* c) {
                   c->sid[1], c->scell[0], c->scell[1], c->soff[0], c->soff[1], c->errd, c->rec[0], c->rec[1],
                   c->cull[0], c->cull[1], c->bpp_start, c->bfp_start, c->bpp, c->bfpc, c->bfpo, c->sk_acc,
                   c->slot_of, c->sflag, c->M1, c->dulag, c->du, c->dp, c->jvt, c->ms, c->nb_idx, c->nb_cnt,
-                  c->nb_flag, c->nb_ref_cell, c->nb_ref_off};
+                  c->nb_flag, c->nb_ref_cell, c->nb_ref_off, c->sbar};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (auto& m : c->marks) (void)hipEventDestroy(m.second);
@@ -593,6 +594,8 @@ void sort_set(fhd_ctx* c, int k) {
   a.sid = c->sid[k];
   a.scell = c->scell[k];
   a.soff = c->soff[k];
+  a.bar = c->sbar;
+  a.err = c->errd;
   a.np = c->np;
   launch_sort(a, c->s);
   mark(c, "sort");
@@ -1175,6 +1178,9 @@ void check_device_flags(fhd_ctx* c) {
   if (e[0] & kErrClamped)
     throw std::runtime_error("device error: a particle displacement of 2^24 cells or more was not applied "
                              "(blown-up step)");
+  if (e[0] & kErrBarrier)
+    throw std::runtime_error("device error: the one-launch particle sort's workgroups did not all meet "
+                             "(its bounded wait ran out)");
 }
 
 void advance(fhd_ctx* c) {
@@ -1807,6 +1813,8 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
       c->sflag = dalloc<int>(2);
       const int flags0[2] = {1, 0};
       HIPCHK(hipMemcpy(c->sflag, flags0, sizeof(flags0), hipMemcpyHostToDevice));
+      c->sbar = dalloc<unsigned long long>(1);
+      HIPCHK(hipMemset(c->sbar, 0, sizeof(unsigned long long)));
       // skin of the WCA neighbour lists: a quarter of the cutoff, so that a list built at
       // radius rc + skin stays complete for hundreds of steps at the particles' ~1e-3 cells
       // per step; the list radius stays within a tile (axes with three or more tiles) and

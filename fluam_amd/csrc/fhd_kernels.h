@@ -100,6 +100,8 @@ struct SortArgs {
   float4* soff;
   int* slot_of;         // optional: sorted slot of each particle id
   const int* gate;      // optional: the sort runs only when *gate != 0 (the count clearing always runs)
+  unsigned long long* bar;  // grid-barrier counter of the one-launch sort (only grows; zero at creation)
+  int* err;             // the context's device error word
   int np;
 };
 

@@ -80,7 +80,7 @@ FHD_BODY const A& batch_entry(const int4* __restrict__ blk, const int4* __restri
 enum Family {
   FAM_BIN = 1, FAM_SCAN, FAM_SCATTER, FAM_TILE_SORT, FAM_WCA, FAM_ZERO, FAM_SRC_PREP, FAM_SPREAD,
   FAM_FLUID_W, FAM_FFT_XY_FWD, FAM_FFT_Z, FAM_FFT_XY_INV, FAM_INTERP, FAM_STEP, FAM_BONDED,
-  FAM_PVEC, FAM_LAG2, FAM_MASS_FLUID, FAM_WCA_LIST
+  FAM_PVEC, FAM_LAG2, FAM_MASS_FLUID, FAM_WCA_LIST, FAM_SORT_ONE
 };
 
 using BatchedFn = void (*)(const int4*, const int4*, const unsigned char*, int);

@@ -134,6 +134,75 @@ FHD_BODY void body_scan_scatter(const SortArgs& a, const Blk& b) {
 }
 FHD_KERNEL_PAIR((), 256, k_scan_scatter, SortArgs, body_scan_scatter(a, b))
 
+// Grid barrier for a launch whose workgroups are all resident at once.  The counter only grows
+// (64 bits; never reset, so repeated launches and graph replays reuse it): each workgroup takes
+// a ticket and waits until the counter reaches the end of its group of nblocks tickets.  The
+// wait is bounded (2^22 polls of about 0.2 us); returns false if it ran out.
+FHD_DEV bool grid_barrier(unsigned long long* ctr, unsigned nblocks) {
+  __shared__ int met;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned long long ticket = atomicAdd(ctr, 1ull);
+    const unsigned long long target = ticket - ticket % nblocks + nblocks;
+    bool done = false;
+    for (int spin = 0; spin < (1 << 22) && !done; ++spin) {
+      done = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+      if (!done) __builtin_amdgcn_s_sleep(8);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    met = done;
+  }
+  __syncthreads();
+  return met != 0;
+}
+
+// The sort in one launch, for a context launched on its own (a merged batch dispatch holds
+// more workgroups than can be resident, so it keeps k_bin + k_scan_scatter).  At most
+// kSortGroups workgroups, one per CU at most, all resident: they bin their particles
+// (grid-stride), meet at a grid barrier, then every workgroup forms the whole prefix of the tile
+// counts in LDS, workgroup 0 stores the tile starts, and each thread places its particles.  The
+// other set's counts are cleared whether or not the sort runs, as in k_scan_scatter.
+constexpr int kSortGroups = 256;
+FHD_BODY void body_sort_one(const SortArgs& a, const Blk& b) {
+  const GridDev& g = a.g;
+  const Tiles& t = a.t;
+  const int n = t.n, np = a.np, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int gtid = b.x * blockDim.x + tid, gsz = b.nx * blockDim.x;
+  for (int k = gtid; k < n; k += gsz) a.count_other[k] = 0;
+  if (!sort_on(a)) return;  // uniform over the grid
+  for (int i = gtid; i < np; i += gsz) a.slot[i] = atomicAdd(&a.count[tile_of(g, t, a.cell[i])], 1);
+  if (!grid_barrier(a.bar, b.nx)) {
+    if (tid == 0) atomicOr(a.err, kErrBarrier);
+    return;
+  }
+  __shared__ int pre[kScanTilesMax + 1];
+  __shared__ int wsum[4];
+  const int* __restrict__ count = a.count;
+  const int ch = (n + 255) / 256;
+  const int lo = min(n, tid * ch), hi = min(n, lo + ch);
+  int sum = 0;
+  for (int k = lo; k < hi; ++k) sum += count[k];
+  const int incl = wave_incl_scan(sum);
+  if (lane == 63) wsum[wv] = incl;
+  __syncthreads();
+  int run = incl - sum;
+  for (int w = 0; w < wv; ++w) run += wsum[w];
+  for (int k = lo; k < hi; ++k) {
+    pre[k] = run;
+    run += count[k];
+  }
+  if (tid == 255) pre[n] = run;
+  __syncthreads();
+  if (b.x == 0)
+    for (int k = tid; k <= n; k += 256) a.start[k] = pre[k];
+  for (int i = gtid; i < np; i += gsz) {
+    const int tile = tile_of(g, t, a.cell[i]);
+    place(a, i, pre[tile] + a.slot[i], tile);
+  }
+}
+FHD_KERNEL_PAIR((), 256, k_sort_one, SortArgs, body_sort_one(a, b))
+
 // d = difference of two wrapped cell coordinates, |d| < m: nearest periodic image.
 FHD_DEV int min_image(int d, int m) {
   if (2 * d > m) return d - m;
@@ -1306,6 +1375,11 @@ FHD_KERNEL_PAIR((template <int NPTS>), 256, k_lag2, Lag2Args, body_lag2<NPTS>(a,
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 void launch_sort(const SortArgs& a, hipStream_t s) {
+  if (t_recorder == nullptr && a.gate && a.bar && a.np > 0 && a.t.n <= kScanTilesMax) {
+    const int groups = nblk(a.np, 256) < kSortGroups ? nblk(a.np, 256) : kSortGroups;
+    launch(k_sort_one, k_sort_one_b, FAM_SORT_ONE, dim3(groups), 256, a, s);
+    return;
+  }
   if (a.np > 0) launch(k_bin, k_bin_b, FAM_BIN, dim3(nblk(a.np, 256)), 256, a, s);
   if (a.t.n <= kScanTilesMax) {
     launch(k_scan_scatter, k_scan_scatter_b, FAM_SCAN, dim3(nblk(a.np > 0 ? a.np : 1, 256)), 256, a, s);
