@@ -171,7 +171,7 @@ struct fhd_ctx {
   int *count = nullptr, *slot = nullptr, *tmp = nullptr;  // count: 2 x ntiles (one per position set)
   int* slot_of = nullptr;  // sorted slot of each particle in set 0
   int* sflag = nullptr;    // [k]: position set k must be sorted before its next use
-  unsigned long long* sbar = nullptr;  // grid-barrier counter of the one-launch sort
+  unsigned long long* sbar = nullptr;  // grid-barrier counters of the one-launch sort and WCA pass
   // inertial coupling with excess mass: a field of work space and per-particle vectors by id
   float* M1 = nullptr;     // the projected correction dv~, then 1/2 dt nu L dv
   float4 *dulag = nullptr, *du = nullptr, *dp = nullptr, *jvt = nullptr, *ms = nullptr;
@@ -655,11 +655,18 @@ void wca(fhd_ctx* c, int k, float4* F) {
   a.skin_len = c->skin_len;
   a.nb = nb_list(c);
   a.F = F;
+  a.bar = c->sbar + 1;
+  a.err = c->errd;
   a.np = c->np;
-  launch_wca(a, c->s);  // the list build, which returns at once while the lists are valid
-  mark(c, "wca_build");
-  launch_wca_list(a, c->s);
-  mark(c, "wca");
+  if (wca_one_launch(a)) {
+    launch_wca(a, c->s);  // the list build (while the lists are stale) and the force in one launch
+    mark(c, "wca");
+  } else {
+    launch_wca(a, c->s);  // the list build, which returns at once while the lists are valid
+    mark(c, "wca_build");
+    launch_wca_list(a, c->s);
+    mark(c, "wca");
+  }
   bonded(c, k, F);
 }
 // Attach the per-group record buffers, then prepare the source records and spread.
@@ -1813,8 +1820,8 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
       c->sflag = dalloc<int>(2);
       const int flags0[2] = {1, 0};
       HIPCHK(hipMemcpy(c->sflag, flags0, sizeof(flags0), hipMemcpyHostToDevice));
-      c->sbar = dalloc<unsigned long long>(1);
-      HIPCHK(hipMemset(c->sbar, 0, sizeof(unsigned long long)));
+      c->sbar = dalloc<unsigned long long>(2);  // [0] the sort's, [1] the WCA pass's
+      HIPCHK(hipMemset(c->sbar, 0, 2 * sizeof(unsigned long long)));
       // skin of the WCA neighbour lists: a quarter of the cutoff, so that a list built at
       // radius rc + skin stays complete for hundreds of steps at the particles' ~1e-3 cells
       // per step; the list radius stays within a tile (axes with three or more tiles) and

@@ -257,6 +257,8 @@ struct WcaArgs {
   float skin_len;  // how far (length) a listed particle may lie outside its tile
   NbList nb;
   float4* F;   // by id
+  unsigned long long* bar;  // grid-barrier counter of the one-launch WCA pass (only grows)
+  int* err;    // the context's device error word
   int np;
 };
 
@@ -298,6 +300,9 @@ void launch_advance_step(unsigned long long* step, hipStream_t s);
 void launch_sk_accum(const float2* spec, double* acc, size_t n, hipStream_t s);  // acc += |spec|^2
 
 void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per-tile ordering
+// A context launched on its own takes the list build and the list force in one launch
+// (launch_wca; launch_wca_list then does nothing); merged batch dispatches take two.
+bool wca_one_launch(const WcaArgs& a);
 void launch_wca(const WcaArgs& a, hipStream_t s);
 void launch_wca_list(const WcaArgs& a, hipStream_t s);
 void launch_bonded(const BondArgs& a, hipStream_t s);  // F += bond forces (after the WCA pass)

@@ -280,13 +280,12 @@ FHD_DEV bool wca_term(const WcaArgs& a, int4 ci, float4 di, int4 cj, float4 dj, 
 // the largest power of two with S x particles <= 256 in crowded tiles) and append every
 // candidate closer than the list radius to its list.  The order inside a list is that of the
 // LDS atomics; the force pass sums in fixed point, where the order does not matter.
-FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
+FHD_DEV void wca_build_tiles(const WcaArgs& a, const Blk& b) {
   __shared__ int4 ccell[kWcaCap];   // w: sorted slot
   __shared__ float4 coff[kWcaCap];
   __shared__ int nb_start[32], nb_pref[33];
   __shared__ int wsum[kWcaThreads / 64];
   __shared__ int ncnt[kWcaThreads];
-  if (*a.nb.flag == 0) return;  // lists still valid (uniform over the grid)
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -440,6 +439,10 @@ FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
     }
   }
 }
+FHD_BODY void body_wca(const WcaArgs& a, const Blk& b) {
+  if (*a.nb.flag == 0) return;  // lists still valid (uniform over the grid)
+  wca_build_tiles(a, b);
+}
 FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
 
 // The WCA force from the neighbour lists, one lane per particle (by id).  The force is summed
@@ -448,10 +451,7 @@ FHD_KERNEL_PAIR((), kWcaThreads, k_wca, WcaArgs, body_wca(a, b))
 // depend on the order of the list.  A particle whose list overflowed takes every particle of
 // the 27 tiles around its tile (the sorted set) instead.  Also clears the stale-list flag that
 // the build before it consumed.
-FHD_BODY void body_wca_list(const WcaArgs& a, const Blk& b) {
-  const int i = b.x * blockDim.x + threadIdx.x;
-  if (i == 0) *a.nb.flag = 0;
-  if (i >= a.np) return;
+FHD_DEV void wca_list_at(const WcaArgs& a, int i) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int4 c0 = a.cell[i];
@@ -551,7 +551,32 @@ FHD_BODY void body_wca_list(const WcaArgs& a, const Blk& b) {
   }
   a.F[i] = make_float4((float)sx * m, (float)sy * m, (float)sz * m, 0.0f);
 }
+FHD_BODY void body_wca_list(const WcaArgs& a, const Blk& b) {
+  const int i = b.x * blockDim.x + threadIdx.x;
+  if (i == 0) *a.nb.flag = 0;
+  if (i >= a.np) return;
+  wca_list_at(a, i);
+}
 FHD_KERNEL_PAIR((), 256, k_wca_list, WcaArgs, body_wca_list(a, b))
+
+// The list build and the list force in one launch, for a context launched on its own (merged
+// batch dispatches keep the two launches).  At most kWcaOneGroups workgroups, all resident: when
+// the lists are flagged stale they rebuild them (a stride loop over the tiles), meet at a grid
+// barrier, and one of them clears the flag, which every workgroup read before the barrier; then
+// each thread takes its particles' forces from the lists.
+constexpr int kWcaOneGroups = 256;
+FHD_BODY void body_wca_one(const WcaArgs& a, const Blk& b) {
+  if (*a.nb.flag != 0) {  // uniform over the grid
+    wca_build_tiles(a, b);
+    if (!grid_barrier(a.bar, b.nx)) {
+      if (threadIdx.x == 0) atomicOr(a.err, kErrBarrier);
+      return;
+    }
+    if (b.x == 0 && threadIdx.x == 0) *a.nb.flag = 0;
+  }
+  for (int i = b.x * blockDim.x + threadIdx.x; i < a.np; i += b.nx * blockDim.x) wca_list_at(a, i);
+}
+FHD_KERNEL_PAIR((), 256, k_wca_one, WcaArgs, body_wca_one(a, b))
 
 // Harmonic bonds, one lane per particle (src/forceBondedGPU.cu:21-121): for each bond partner
 // (another particle, then fixed points, each in file order) F_i += -k (1 - r0/r) (r_i - r_j),
@@ -1389,10 +1414,17 @@ void launch_sort(const SortArgs& a, hipStream_t s) {
     if (a.np > 0) launch(k_scatter, k_scatter_b, FAM_SCATTER, dim3(nblk(a.np, 256)), 256, a, s);
   }
 }
+bool wca_one_launch(const WcaArgs& a) { return t_recorder == nullptr && a.bar && a.np > 0; }
 void launch_wca(const WcaArgs& a, hipStream_t s) {
+  if (wca_one_launch(a)) {
+    const int groups = nblk(a.np, 256) < kWcaOneGroups ? nblk(a.np, 256) : kWcaOneGroups;
+    launch(k_wca_one, k_wca_one_b, FAM_WCA_LIST, dim3(groups), 256, a, s);
+    return;
+  }
   if (a.np > 0) launch(k_wca, k_wca_b, FAM_WCA, dim3(a.t.n < kWcaBuildGroups ? a.t.n : kWcaBuildGroups), kWcaThreads, a, s);
 }
 void launch_wca_list(const WcaArgs& a, hipStream_t s) {
+  if (wca_one_launch(a)) return;  // done by launch_wca's one launch
   if (a.np > 0) launch(k_wca_list, k_wca_list_b, FAM_WCA_LIST, dim3(nblk(a.np, 256)), 256, a, s);
 }
 void launch_bonded(const BondArgs& a, hipStream_t s) {
