@@ -578,7 +578,7 @@ void apply_schedule(fhd_ctx* c) {
 // ---------------- particles ----------------
 // Counting sort of position set k by tile (gated off on the device while every particle is
 // still inside its listed tile's skin).
-void sort_set(fhd_ctx* c, int k) {
+SortArgs sort_args(fhd_ctx* c, int k) {
   SortArgs a;
   a.g = c->g;
   a.t = c->t;
@@ -597,7 +597,10 @@ void sort_set(fhd_ctx* c, int k) {
   a.bar = c->sbar;
   a.err = c->errd;
   a.np = c->np;
-  launch_sort(a, c->s);
+  return a;
+}
+void sort_set(fhd_ctx* c, int k) {
+  launch_sort(sort_args(c, k), c->s);
   mark(c, "sort");
 }
 
@@ -636,12 +639,7 @@ NbList nb_list(fhd_ctx* c) {
   return n;
 }
 
-void wca(fhd_ctx* c, int k, float4* F) {
-  if (!c->p.nonbonded || c->p.cutoff <= 0) {
-    launch_zero_f4(F, c->np, c->s);
-    bonded(c, k, F);
-    return;
-  }
+WcaArgs wca_args(fhd_ctx* c, int k, float4* F) {
   WcaArgs a;
   a.g = c->g;
   a.t = c->t;
@@ -658,6 +656,16 @@ void wca(fhd_ctx* c, int k, float4* F) {
   a.bar = c->sbar + 1;
   a.err = c->errd;
   a.np = c->np;
+  return a;
+}
+
+void wca(fhd_ctx* c, int k, float4* F) {
+  if (!c->p.nonbonded || c->p.cutoff <= 0) {
+    launch_zero_f4(F, c->np, c->s);
+    bonded(c, k, F);
+    return;
+  }
+  const WcaArgs a = wca_args(c, k, F);
   if (wca_one_launch(a)) {
     launch_wca(a, c->s);  // the list build (while the lists are stale) and the force in one launch
     mark(c, "wca");
@@ -668,6 +676,22 @@ void wca(fhd_ctx* c, int k, float4* F) {
     mark(c, "wca");
   }
   bonded(c, k, F);
+}
+// Sort position set k (gated) and take the WCA forces (plus bonded ones) into F: for a context
+// launched on its own with WCA, the sort and the WCA pass are one launch.
+void sort_and_wca(fhd_ctx* c, int k, float4* F) {
+  if (c->p.nonbonded && c->p.cutoff > 0) {
+    const SortArgs s = sort_args(c, k);
+    const WcaArgs w = wca_args(c, k, F);
+    if (sort_wca_one_launch(s, w)) {
+      launch_sort_wca(s, w, c->s);
+      mark(c, "wca");
+      bonded(c, k, F);
+      return;
+    }
+  }
+  sort_set(c, k);
+  wca(c, k, F);
 }
 // Attach the per-group record buffers, then prepare the source records and spread.
 void run_spread(fhd_ctx* c, SpreadArgs& sa) {
@@ -795,8 +819,7 @@ void step_stokes(fhd_ctx* c) {
   const bool xm = p.extra_mobility != 0.0;
   // solve 1: W = S F^n, and the kept part of sqrt(2) D.Sigma_1, drawn in Fourier space
   if (c->np > 0) {
-    sort_set(c, 0);
-    wca(c, 0, c->F[0]);
+    sort_and_wca(c, 0, c->F[0]);
     SpreadArgs sa = spread_base(c);
     sa.ngroups = 1;
     sa.grp[0].set = sset(c, 0);
@@ -829,8 +852,7 @@ void step_stokes(fhd_ctx* c) {
     }
     launch_interp(ia, c->s);
     mark(c, "interp");
-    sort_set(c, 1);
-    wca(c, 1, c->F[1]);
+    sort_and_wca(c, 1, c->F[1]);
     SpreadArgs sa = spread_base(c);
     sa.ngroups = 2;
     sa.grp[0].set = sset(c, 1);
@@ -1007,8 +1029,7 @@ void step_qnb(fhd_ctx* c) {
     ia.a1 = (float)(0.5 * dt);
     launch_interp(ia, c->s);
     mark(c, "interp");
-    sort_set(c, 1);
-    wca(c, 1, c->F[1]);
+    sort_and_wca(c, 1, c->F[1]);
     const double mf = 1.5 * dV * p.volume_particle * rho;
     const double ratio = (p.mass + mf) != 0.0 ? mf / (p.mass + mf) : 1.0;
     SpreadArgs sa = spread_base(c);

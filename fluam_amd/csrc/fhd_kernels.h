@@ -262,6 +262,12 @@ struct WcaArgs {
   int np;
 };
 
+// The particle sort and the WCA pass of one force evaluation, in one launch.
+struct SortWcaArgs {
+  SortArgs s;
+  WcaArgs w;
+};
+
 // Harmonic bonds (V = k/2 (r - r0)^2) of each particle, in compressed rows by particle id:
 // particle i's particle-particle bonds are pp[pp_start[i] .. pp_start[i+1]) and its fixed-point
 // bonds fp*[fp_start[i] .. fp_start[i+1]).  A fixed point is stored like a position, as
@@ -303,6 +309,10 @@ void launch_sort(const SortArgs& a, hipStream_t s);  // bin, scan + scatter, per
 // A context launched on its own takes the list build and the list force in one launch
 // (launch_wca; launch_wca_list then does nothing); merged batch dispatches take two.
 bool wca_one_launch(const WcaArgs& a);
+// The sort (gated) and the WCA pass of one force evaluation in one launch, for a context
+// launched on its own.
+bool sort_wca_one_launch(const SortArgs& s, const WcaArgs& w);
+void launch_sort_wca(const SortArgs& s, const WcaArgs& w, hipStream_t st);
 void launch_wca(const WcaArgs& a, hipStream_t s);
 void launch_wca_list(const WcaArgs& a, hipStream_t s);
 void launch_bonded(const BondArgs& a, hipStream_t s);  // F += bond forces (after the WCA pass)

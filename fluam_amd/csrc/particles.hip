@@ -164,17 +164,19 @@ FHD_DEV bool grid_barrier(unsigned long long* ctr, unsigned nblocks) {
 // counts in LDS, workgroup 0 stores the tile starts, and each thread places its particles.  The
 // other set's counts are cleared whether or not the sort runs, as in k_scan_scatter.
 constexpr int kSortGroups = 256;
-FHD_BODY void body_sort_one(const SortArgs& a, const Blk& b) {
+// The sort's work; returns 0 when the gate was off, 1 when it sorted, -1 when a barrier's wait
+// ran out (the error flag raised).
+FHD_DEV int sort_one(const SortArgs& a, const Blk& b) {
   const GridDev& g = a.g;
   const Tiles& t = a.t;
   const int n = t.n, np = a.np, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int gtid = b.x * blockDim.x + tid, gsz = b.nx * blockDim.x;
   for (int k = gtid; k < n; k += gsz) a.count_other[k] = 0;
-  if (!sort_on(a)) return;  // uniform over the grid
+  if (!sort_on(a)) return 0;  // uniform over the grid
   for (int i = gtid; i < np; i += gsz) a.slot[i] = atomicAdd(&a.count[tile_of(g, t, a.cell[i])], 1);
   if (!grid_barrier(a.bar, b.nx)) {
     if (tid == 0) atomicOr(a.err, kErrBarrier);
-    return;
+    return -1;
   }
   __shared__ int pre[kScanTilesMax + 1];
   __shared__ int wsum[4];
@@ -200,7 +202,9 @@ FHD_BODY void body_sort_one(const SortArgs& a, const Blk& b) {
     const int tile = tile_of(g, t, a.cell[i]);
     place(a, i, pre[tile] + a.slot[i], tile);
   }
+  return 1;
 }
+FHD_BODY void body_sort_one(const SortArgs& a, const Blk& b) { (void)sort_one(a, b); }
 FHD_KERNEL_PAIR((), 256, k_sort_one, SortArgs, body_sort_one(a, b))
 
 // d = difference of two wrapped cell coordinates, |d| < m: nearest periodic image.
@@ -577,6 +581,22 @@ FHD_BODY void body_wca_one(const WcaArgs& a, const Blk& b) {
   for (int i = b.x * blockDim.x + threadIdx.x; i < a.np; i += b.nx * blockDim.x) wca_list_at(a, i);
 }
 FHD_KERNEL_PAIR((), 256, k_wca_one, WcaArgs, body_wca_one(a, b))
+
+// The sort and the WCA pass of one force evaluation in one launch (same grid: at most 256
+// workgroups of 256 lanes, all resident).  When the sort ran, the workgroups meet once more
+// before the WCA pass reads the sorted set; the WCA pass's barriers use the sort's counter.
+FHD_BODY void body_sort_wca(const SortWcaArgs& x, const Blk& b) {
+  const int ran = sort_one(x.s, b);
+  if (ran < 0) return;
+  if (ran > 0 && !grid_barrier(x.s.bar, b.nx)) {
+    if (threadIdx.x == 0) atomicOr(x.s.err, kErrBarrier);
+    return;
+  }
+  WcaArgs a = x.w;
+  a.bar = x.s.bar;
+  body_wca_one(a, b);
+}
+FHD_KERNEL_PAIR((), 256, k_sort_wca, SortWcaArgs, body_sort_wca(a, b))
 
 // Harmonic bonds, one lane per particle (src/forceBondedGPU.cu:21-121): for each bond partner
 // (another particle, then fixed points, each in file order) F_i += -k (1 - r0/r) (r_i - r_j),
@@ -1415,6 +1435,16 @@ void launch_sort(const SortArgs& a, hipStream_t s) {
   }
 }
 bool wca_one_launch(const WcaArgs& a) { return t_recorder == nullptr && a.bar && a.np > 0; }
+bool sort_wca_one_launch(const SortArgs& s, const WcaArgs& w) {
+  return wca_one_launch(w) && s.gate && s.bar && s.np == w.np && s.t.n <= kScanTilesMax;
+}
+void launch_sort_wca(const SortArgs& s, const WcaArgs& w, hipStream_t st) {
+  SortWcaArgs x;
+  x.s = s;
+  x.w = w;
+  const int groups = nblk(w.np, 256) < kWcaOneGroups ? nblk(w.np, 256) : kWcaOneGroups;
+  launch(k_sort_wca, k_sort_wca_b, FAM_WCA_LIST, dim3(groups), 256, x, st);
+}
 void launch_wca(const WcaArgs& a, hipStream_t s) {
   if (wca_one_launch(a)) {
     const int groups = nblk(a.np, 256) < kWcaOneGroups ? nblk(a.np, 256) : kWcaOneGroups;
