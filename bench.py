@@ -31,7 +31,9 @@ def kernel_bytes(name, mx, my, mz, npart):
     pass.  Per cell: spread writes 12 B; the W + xy-forward pass reads 12 and writes 12 (its noise
     is generated in registers); the z pass reads and writes the half spectrum, 24; the xy-inverse
     pass 24; the interpolation reads 12.  Per particle: sort 32, force 24, spread 24,
-    interpolation and update 24.  Passes the formula does not charge (the spreading records;
+    interpolation and update 24 (the "wca" group is one launch that also holds the gated sort,
+    which runs only when a particle left its tile's skin).  Passes the formula does not charge
+    (the spreading records;
     the W pass of the other schemes) get 0 B; their cost is time a fused pipeline would not
     spend.  The Stokes step's noise is drawn in registers inside the z pass."""
     N = mx * my * mz
