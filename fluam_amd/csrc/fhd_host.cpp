@@ -1824,6 +1824,13 @@ fhd_ctx* fhd_create(const fhd_params* pin) {
         c->rec[k] = dalloc<SrcRec>(2 * (size_t)c->np);
         c->cull[k] = dalloc<int>(2 * (size_t)c->np);
         HIPCHK(hipMemset(c->F[k], 0, sizeof(float4) * c->np));
+        // the sorted sets start empty and in range (every tile empty, every slot particle 0),
+        // so a sort that never completes (its grid barrier gave up and raised the device error)
+        // leaves the later passes of that step reading valid indices
+        HIPCHK(hipMemset(c->start[k], 0, sizeof(int) * (t.n + 1)));
+        HIPCHK(hipMemset(c->sid[k], 0, sizeof(int) * c->np));
+        HIPCHK(hipMemset(c->scell[k], 0, sizeof(int4) * c->np));
+        HIPCHK(hipMemset(c->soff[k], 0, sizeof(float4) * c->np));
       }
       c->u = dalloc<float4>(c->np);
       HIPCHK(hipMemset(c->u, 0, sizeof(float4) * c->np));
